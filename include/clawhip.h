@@ -1,0 +1,115 @@
+/*
+ * clawhip.h -- C ABI of libclawhip.so, the MI355X (gfx950) implementation of
+ * PyClaw's 2D time step.  Plain pointers and sizes only; every array argument
+ * is a device pointer; every call is stream-ordered on the stream passed in
+ * and returns an int status (CLAW_OK or a CLAW_E* code, never aborts).
+ *
+ * Replaces the reference's f2py module ABI:
+ *   classic2.step2   -- src/pyclaw/classic/step2.f90:2-5   (called at classic/solver.py:537-539)
+ *   classic2.step2ds -- src/pyclaw/classic/step2ds.f90:2-5 (called at classic/solver.py:525-531)
+ *   sharpclaw2.flux2 -- src/pyclaw/sharpclaw/flux2.f90:2   (called at sharpclaw/solver.py:577)
+ * and the Riemann-solver plugin (the rpn2/rpt2 procedure pointers passed at
+ * classic/solver.py:518-519, with scalar parameters in the /cparam/ common block
+ * set at state.py:162-182) by a Riemann-solver id plus parameters.
+ *
+ * Array layout ("planar padded"): component m of cell (i, j), 0-based interior
+ * indices, lives at  a[m*plane + (j+ng)*ldq + (i+ng)],  x fastest.  ng ghost
+ * layers are stored but are only read for sides whose boundary code is
+ * CLAW_BC_STORED; the other codes synthesise ghost values in the kernel.
+ * Kernels write interior cells only.
+ */
+#ifndef CLAWHIP_H
+#define CLAWHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CLAW_OK = 0,
+    CLAW_EINVAL = 1,       /* invalid argument or unsupported option */
+    CLAW_EDEVICE = 2,      /* HIP runtime error */
+    CLAW_ENOMEM = 3,       /* device allocation failed */
+    CLAW_ETOOLARGE = 4     /* size beyond a documented maximum */
+};
+
+/* Riemann solvers (clawpack.riemann names). */
+enum {
+    CLAW_RP_ACOUSTICS = 0,     /* rp2_acoustics:    p[0..3] = rho, bulk, cc, zz     */
+    CLAW_RP_VC_ACOUSTICS = 1,  /* rp2_vc_acoustics: aux = (rho, c) per cell        */
+    CLAW_RP_EULER = 2,         /* rp2_euler_5wave:  p[0..1] = gamma, gamma1; efix  */
+    CLAW_RP_SHALLOW = 3        /* rp2_shallow_roe_with_efix: p[0] = grav           */
+};
+
+/* Boundary codes (pyclaw.BC, solver.py:12-18); STORED reads the ghost cells as
+   they are in the array (the caller filled them, e.g. for user/custom BCs). */
+enum { CLAW_BC_STORED = 0, CLAW_BC_EXTRAP = 1, CLAW_BC_PERIODIC = 2, CLAW_BC_WALL = 3 };
+
+enum { CLAW_F64 = 0, CLAW_F32 = 1 };
+
+typedef struct {
+    int32_t mx, my;      /* interior cells                                     */
+    int32_t ng;          /* ghost layers stored in q/aux (>= 2 classic, 3 SharpClaw) */
+    int32_t pad_;
+    int64_t ldq;         /* row pitch in elements (>= mx + 2 ng)               */
+    int64_t plane;       /* component pitch in elements (>= ldq * (my + 2 ng)) */
+    double dx, dy;       /* cell sizes (Dimension.delta, geometry.py:425-427)  */
+} ClawGrid;
+
+typedef struct {
+    int32_t order;        /* 1 Godunov, 2 with limited corrections (method(2))  */
+    int32_t transverse;   /* 0 none, 1 trans_inc, 2 trans_cor (method(3))       */
+    int32_t split;        /* 0 unsplit step2, 1 dimensional splitting step2ds   */
+    int32_t fwave;        /* must be 0: no f-wave solver is provided            */
+    int32_t num_waves;
+    int32_t mthlim[8];    /* limiter per wave family (philim ids; 0 = none)     */
+} ClawMethod;
+
+typedef struct {
+    int32_t kind;         /* CLAW_RP_*                                          */
+    int32_t efix;         /* Euler: entropy fix on (reference default)          */
+    double p[8];          /* solver parameters, see CLAW_RP_*                   */
+} ClawRiemann;
+
+typedef struct {
+    int32_t lower[2], upper[2];          /* q: per dimension (x, y)            */
+    int32_t aux_lower[2], aux_upper[2];  /* aux                                */
+} ClawBC;
+
+/* Context: owns scratch buffers (split intermediate state) and launch
+   choices.  One context per device; calls on one context are stream-ordered
+   and must not overlap on different host threads. */
+int claw_ctx_create(int device, void **ctx);
+int claw_ctx_destroy(void *ctx);
+
+/* One classic step (unsplit or split) from qold into qnew (distinct buffers).
+   Replaces ClawSolver2D.step_hyperbolic's step2/step2ds calls
+   (src/pyclaw/classic/solver.py:498-547, step2.f90, step2ds.f90).
+   dt_dev: device double, the step size.  cfl_dev: device double, receives the
+   Courant number of the step (max over the same interfaces as the reference);
+   it is overwritten.  precision: CLAW_F64 (reference arithmetic) or CLAW_F32.
+   aux may be NULL when the solver uses no aux fields. */
+int claw2d_classic_step(void *ctx, const ClawGrid *grid, const ClawMethod *method,
+                        const ClawRiemann *rp, const ClawBC *bc, int precision,
+                        const void *qold, void *qnew, const void *aux,
+                        const double *dt_dev, double *cfl_dev, void *stream);
+
+/* Same, with flags.  CLAW_STEP_ACCUMULATE_CFL: do not reset *cfl_dev; the
+   step's Courant number is max-combined into it (one launch per unsplit step,
+   which lets a caller time the step kernel alone or run several steps and
+   read one maximum). */
+enum { CLAW_STEP_ACCUMULATE_CFL = 1 };
+int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *method,
+                           const ClawRiemann *rp, const ClawBC *bc, int precision,
+                           const void *qold, void *qnew, const void *aux,
+                           const double *dt_dev, double *cfl_dev, int flags, void *stream);
+
+/* Number of kernel launches one classic step issues (for timing). */
+int claw2d_classic_launches(const ClawMethod *method);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLAWHIP_H */

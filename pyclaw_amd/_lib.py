@@ -1,0 +1,115 @@
+"""ctypes binding of libclawhip.so (the C ABI declared in include/clawhip.h).
+
+Loads the in-tree library built by pyclaw_amd.build.  There is no fallback:
+every entry point that needs the HIP library raises if it is missing.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libclawhip.so")
+
+CLAW_OK, CLAW_EINVAL, CLAW_EDEVICE, CLAW_ENOMEM, CLAW_ETOOLARGE = 0, 1, 2, 3, 4
+RP_ACOUSTICS, RP_VC_ACOUSTICS, RP_EULER, RP_SHALLOW = 0, 1, 2, 3
+BC_STORED, BC_EXTRAP, BC_PERIODIC, BC_WALL = 0, 1, 2, 3
+F64, F32 = 0, 1
+STEP_ACCUMULATE_CFL = 1
+
+_ERRORS = {
+    CLAW_EINVAL: "invalid argument or unsupported option",
+    CLAW_EDEVICE: "HIP runtime error",
+    CLAW_ENOMEM: "device allocation failed",
+    CLAW_ETOOLARGE: "size beyond the documented maximum",
+}
+
+
+class ClawGrid(ctypes.Structure):
+    _fields_ = [("mx", ctypes.c_int32), ("my", ctypes.c_int32), ("ng", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("ldq", ctypes.c_int64), ("plane", ctypes.c_int64),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double)]
+
+
+class ClawMethod(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int32), ("transverse", ctypes.c_int32), ("split", ctypes.c_int32),
+                ("fwave", ctypes.c_int32), ("num_waves", ctypes.c_int32), ("mthlim", ctypes.c_int32 * 8)]
+
+
+class ClawRiemann(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("efix", ctypes.c_int32), ("p", ctypes.c_double * 8)]
+
+
+class ClawBC(ctypes.Structure):
+    _fields_ = [("lower", ctypes.c_int32 * 2), ("upper", ctypes.c_int32 * 2),
+                ("aux_lower", ctypes.c_int32 * 2), ("aux_upper", ctypes.c_int32 * 2)]
+
+
+class ClawError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    """The loaded HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ClawError("libclawhip.so is not built (run pyclaw_amd.build.build() or "
+                            "__graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.claw_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.claw_ctx_destroy.argtypes = [vp]
+        L.claw2d_classic_step.argtypes = [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod),
+                                          ctypes.POINTER(ClawRiemann), ctypes.POINTER(ClawBC), ctypes.c_int,
+                                          vp, vp, vp, vp, vp, vp]
+        L.claw2d_classic_step_ex.argtypes = L.claw2d_classic_step.argtypes[:11] + [ctypes.c_int, vp]
+        L.claw2d_classic_launches.argtypes = [ctypes.POINTER(ClawMethod)]
+        for name, argtypes in _optional_symbols().items():
+            fn = getattr(L, name, None)
+            if fn is not None:
+                fn.argtypes = argtypes
+        _lib = L
+    return _lib
+
+
+def _optional_symbols():
+    vp = ctypes.c_void_p
+    return {
+        "claw2d_sharpclaw_dq": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.POINTER(ClawBC),
+                                ctypes.c_int, vp, vp, vp, vp, vp, vp],
+        "claw2d_sharpclaw_stage": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann),
+                                   ctypes.POINTER(ClawBC), ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp,
+                                   vp, vp],
+    }
+
+
+def check(rc, what):
+    if rc != CLAW_OK:
+        raise ClawError("%s failed: %s (code %d)" % (what, _ERRORS.get(rc, "unknown error"), rc))
+
+
+class Context(object):
+    """Owns a libclawhip context (scratch buffers) for one device."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        check(lib().claw_ctx_create(int(device), ctypes.byref(self._ctx)), "claw_ctx_create")
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def __del__(self):
+        try:
+            if self._ctx and _lib is not None:
+                _lib.claw_ctx_destroy(self._ctx)
+                self._ctx = ctypes.c_void_p()
+        except Exception:
+            pass

@@ -1,0 +1,767 @@
+// Fused classic (Clawpack wave-propagation) 2D step for gfx950.
+//
+// Replaces the reference's unsplit step (src/pyclaw/classic/step2.f90:2-251),
+// which calls flux2 (classic/flux2.f90:5-210) on every row and every column
+// with the Riemann solver, the wave limiter (limiter.f90, philim.f90) and two
+// transverse solves per interface, accumulating into qnew slice by slice.
+//
+// Design (one wavefront = one independent worker, no LDS, no barriers):
+//   * A wavefront owns a strip of 64 cell columns (61 of them produced, the
+//     other 3 are the halo needed by the limiter and the transverse terms)
+//     and marches up a band of H rows.  Lane L holds column c0-1+L.
+//   * Per row it solves the x-interface to the left of its column and the
+//     y-interface below its cell.  x-neighbour data move between lanes with
+//     ds_bpermute; y-neighbour data stay in the lane's registers (a one-row
+//     lag lets each y-interface be limited with the waves above and below it).
+//   * Ghost cells are synthesised on load from the boundary codes, in the
+//     reference's fill order (x sides, then y sides over the padded width).
+//   * Per cell the contributions are accumulated in exactly the reference's
+//     order (row j-1, row j, row j+1, then column i-1, i, i+1; step2.f90:139-146,
+//     222-229; flux2.f90:108-111, 188-206), so FP64 results equal the reference
+//     bit for bit when built without floating-point contraction.
+//   * The Courant number is reduced per wavefront and combined with an integer
+//     atomic max on the non-negative bit pattern: exact and order-independent.
+// No amdq/apdq/waves ever reach HBM: one read of q (and aux) per cell plus the
+// halo, one write of q.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "common.hpp"
+#include "riemann.hpp"
+
+namespace clawhip {
+
+constexpr int XS_USEFUL = WAVE - 3;   // produced columns per wavefront
+constexpr int CNG = 2;               // ghost width of the classic method (classic/solver.py:82)
+
+template <class RP, typename R>
+struct ClassicParams {
+    const R *qold;
+    R *qnew;
+    const R *aux;
+    GridDev g;
+    BCDev bq, ba;
+    typename RP::Params P;
+    const double *dt;
+    double *cfl;
+    double dx, dy;
+    int order, trans;
+    int mthlim[8];
+    int H;          // rows per wavefront band
+    int nsx, nsy;   // column strips, row bands
+    int nsyb;       // row-band groups (one group per block)
+    int wpb;        // wavefronts per block
+};
+
+// Virtual block index: consecutive workgroups are dispatched round-robin over
+// the 8 XCDs; remap so each XCD (own L2) gets a contiguous range of strips.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, xcd = b % 8, loc = b / 8;
+    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
+    constexpr int NE = RP::NE, NW = RP::NW;
+    constexpr bool HAS_AUX = RP::NA > 0;
+    constexpr int NA = HAS_AUX ? RP::NA : 1;
+    using Cache = typename RP::Cache;
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int vb = xcd_block();
+    const int sx = vb / p.nsyb;
+    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+    if (sx >= p.nsx || sy >= p.nsy) return;
+
+    const GridDev g = p.g;
+    const int c0 = sx * XS_USEFUL;
+    const int i = c0 - 1 + lane;
+    const int R0 = sy * p.H;
+    const int R1 = min(R0 + p.H, g.my);
+    const int up = ((lane + WAVE - 1) & (WAVE - 1)) << 2;
+    const int dn = ((lane + 1) & (WAVE - 1)) << 2;
+    const bool l0 = (lane == 0);
+    const bool produce = lane >= 1 && lane <= XS_USEFUL && i < g.mx;
+
+    const double dtd = *p.dt;
+    const R dtdx = R(dtd / p.dx), dtdy = R(dtd / p.dy);
+    const R hx = R(0.5) * dtdx, hy = R(0.5) * dtdy;
+    const R avx = R(0.5) * (dtdx + dtdx), avy = R(0.5) * (dtdy + dtdy);
+    const int order = p.order, trans = p.trans;
+    const auto &P = p.P;
+
+    R cfl = R(0);
+    R qprev[NE], qlprev[NE];
+    R a_m2[NA], a_m1[NA], a_0[NA], al_m2[NA], al_m1[NA], al_0[NA];
+    R accOld[NE], accNew[NE], g2Xs[NE];
+    R wY[NW][NE], sY[NW], amY[NE], apY[NE], dotlY[NW];
+    Cache cY;
+    R uAp[NE], uF[NE], uBm[NE], uBp[NE];
+#pragma unroll
+    for (int m = 0; m < NE; ++m) {
+        accOld[m] = R(0); accNew[m] = R(0); g2Xs[m] = R(0);
+        uAp[m] = R(0); uF[m] = R(0); uBm[m] = R(0); uBp[m] = R(0);
+        amY[m] = R(0); apY[m] = R(0);
+    }
+#pragma unroll
+    for (int mw = 0; mw < NW; ++mw) {
+        sY[mw] = R(0); dotlY[mw] = R(0);
+#pragma unroll
+        for (int m = 0; m < NE; ++m) wY[mw][m] = R(0);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) { a_m2[a] = a_m1[a] = a_0[a] = al_m2[a] = al_m1[a] = al_0[a] = R(1); }
+
+    load_q<NE>(p.qold, g, p.bq, i, R0 - 2, qprev);
+    if (l0) load_q<NE>(p.qold, g, p.bq, i - 1, R0 - 2, qlprev);
+    if constexpr (HAS_AUX) {
+        load_aux<NA>(p.aux, g, p.ba, i, R0 - 2, a_m1);
+        if (l0) load_aux<NA>(p.aux, g, p.ba, i - 1, R0 - 2, al_m1);
+    }
+
+    for (int L = R0 - 1; L <= R1 + 1; ++L) {
+        // ---- 1. load the lead row L ----------------------------------------
+        R qc[NE], qlc[NE];
+        load_q<NE>(p.qold, g, p.bq, i, L, qc);
+        if (l0) load_q<NE>(p.qold, g, p.bq, i - 1, L, qlc);
+        if constexpr (HAS_AUX) {
+            load_aux<NA>(p.aux, g, p.ba, i, L, a_0);
+            if (l0) load_aux<NA>(p.aux, g, p.ba, i - 1, L, al_0);
+        }
+
+        // ---- 2. x-interface (i-1/2, L-1): solve, limit, transverse, x-part of cell (i, L-1)
+        if (L >= R0) {
+            const int row = L - 1;
+            R lo[NE], wave[NW][NE], s[NW], amdq[NE], apdq[NE], fadd[NE], cq[NE];
+            R alo[NA], alm2[NA], al0[NA];
+            Cache cx;
+#pragma unroll
+            for (int m = 0; m < NE; ++m) { lo[m] = lane_get(qprev[m], up); if (l0) lo[m] = qlprev[m]; }
+            if constexpr (HAS_AUX) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    alo[a] = lane_get(a_m1[a], up); alm2[a] = lane_get(a_m2[a], up); al0[a] = lane_get(a_0[a], up);
+                    if (l0) { alo[a] = al_m1[a]; alm2[a] = al_m2[a]; al0[a] = al_0[a]; }
+                }
+            }
+            RP::template rpn<0>(P, lo, qprev, alo, a_m1, wave, s, amdq, apdq, cx);
+            if (row >= -1 && row <= g.my && i >= 0 && i <= g.mx) {
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdx * s[mw]); cfl = rmax2<R>(cfl, -dtdx * s[mw]); }
+            }
+#pragma unroll
+            for (int m = 0; m < NE; ++m) { fadd[m] = R(0); cq[m] = R(0); }
+            if (order == 2) {
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw) {
+                    R wn[NE];
+#pragma unroll
+                    for (int m = 0; m < NE; ++m)
+                        wn[m] = RP::template WNZ<0>(m, mw) ? lane_get(wave[mw][m], dn) : R(0);
+                    const int mth = p.mthlim[mw];
+                    if (mth == 0) continue;
+                    R wnorm2 = R(0), dotr = R(0);
+#pragma unroll
+                    for (int m = 0; m < NE; ++m)
+                        if (RP::template WNZ<0>(m, mw)) { wnorm2 = wnorm2 + wave[mw][m] * wave[mw][m]; dotr = dotr + wave[mw][m] * wn[m]; }
+                    R dotl = lane_get(dotr, up);
+                    if (wnorm2 != R(0)) {
+                        R phi = philim<R>(wnorm2, s[mw] > R(0) ? dotl : dotr, mth);
+#pragma unroll
+                        for (int m = 0; m < NE; ++m)
+                            if (RP::template WNZ<0>(m, mw)) wave[mw][m] = phi * wave[mw][m];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    R c = R(0);
+#pragma unroll
+                    for (int mw = 0; mw < NW; ++mw)
+                        if (RP::template WNZ<0>(m, mw)) c = c + rabs(s[mw]) * (R(1) - rabs(s[mw]) * avx) * wave[mw][m];
+                    cq[m] = c;
+                    fadd[m] = R(0) + R(0.5) * c;
+                }
+            }
+            R bmA[NE], bpA[NE], bmP[NE], bpP[NE];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) { bmA[m] = bpA[m] = bmP[m] = bpP[m] = R(0); }
+            if (trans > 0) {
+                if (order == 2 && trans == 2) {
+#pragma unroll
+                    for (int m = 0; m < NE; ++m) { amdq[m] = amdq[m] + cq[m]; apdq[m] = apdq[m] - cq[m]; }
+                }
+                RP::template rpt<0>(P, 1, amdq, alm2, alo, al0, bmA, bpA, cx);
+                RP::template rpt<0>(P, 2, apdq, a_m2, a_m1, a_0, bmP, bpP, cx);
+            }
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                R dAm = lane_get(amdq[m], dn), dF = lane_get(fadd[m], dn);
+                R dBm = lane_get(bmA[m], dn), dBp = lane_get(bpA[m], dn);
+                R qadd = (R(0) - dtdx * apdq[m]) - dtdx * dAm;
+                R g1 = (R(0) - hx * dBm) - hx * bmP[m];
+                R g2 = (R(0) - hx * dBp) - hx * bpP[m];
+                R v = qprev[m] + dtdy * g2Xs[m];
+                v = ((v + qadd) - dtdx * (dF - fadd[m])) - dtdy * (g2 - g1);
+                accOld[m] = accOld[m] - dtdy * g1;
+                accNew[m] = v;
+                g2Xs[m] = g2;
+            }
+        }
+
+        // ---- 3. y-interface (i, L-1/2): normal solve (unlimited) --------------
+        R wN[NW][NE], sN[NW], amN[NE], apN[NE], dotN[NW];
+        Cache cN;
+        RP::template rpn<1>(P, qprev, qc, a_m1, a_0, wN, sN, amN, apN, cN);
+        if (L >= 0 && L <= g.my && i >= -1 && i <= g.mx) {
+#pragma unroll
+            for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdy * sN[mw]); cfl = rmax2<R>(cfl, -dtdy * sN[mw]); }
+        }
+#pragma unroll
+        for (int mw = 0; mw < NW; ++mw) {
+            R d = R(0);
+#pragma unroll
+            for (int m = 0; m < NE; ++m)
+                if (RP::template WNZ<1>(m, mw)) d = d + wY[mw][m] * wN[mw][m];
+            dotN[mw] = d;
+        }
+
+        // ---- 4. finish y-interface (i, L-3/2) and cell row L-2 ---------------
+        if (L >= R0 + 1) {
+            R fY[NE], cqY[NE];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) { fY[m] = R(0); cqY[m] = R(0); }
+            if (order == 2) {
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw) {
+                    const int mth = p.mthlim[mw];
+                    if (mth == 0) continue;
+                    R wnorm2 = R(0);
+#pragma unroll
+                    for (int m = 0; m < NE; ++m)
+                        if (RP::template WNZ<1>(m, mw)) wnorm2 = wnorm2 + wY[mw][m] * wY[mw][m];
+                    if (wnorm2 != R(0)) {
+                        R phi = philim<R>(wnorm2, sY[mw] > R(0) ? dotlY[mw] : dotN[mw], mth);
+#pragma unroll
+                        for (int m = 0; m < NE; ++m)
+                            if (RP::template WNZ<1>(m, mw)) wY[mw][m] = phi * wY[mw][m];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    R c = R(0);
+#pragma unroll
+                    for (int mw = 0; mw < NW; ++mw)
+                        if (RP::template WNZ<1>(m, mw)) c = c + rabs(sY[mw]) * (R(1) - rabs(sY[mw]) * avy) * wY[mw][m];
+                    cqY[m] = c;
+                    fY[m] = R(0) + R(0.5) * c;
+                }
+            }
+            R bmA[NE], bpA[NE], bmP[NE], bpP[NE];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) { bmA[m] = bpA[m] = bmP[m] = bpP[m] = R(0); }
+            if (trans > 0) {
+                if (order == 2 && trans == 2) {
+#pragma unroll
+                    for (int m = 0; m < NE; ++m) { amY[m] = amY[m] + cqY[m]; apY[m] = apY[m] - cqY[m]; }
+                }
+                R am2l[NA], am2r[NA], am1l[NA], am1r[NA];
+                if constexpr (HAS_AUX) {
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) {
+                        am2l[a] = lane_get(a_m2[a], up); am2r[a] = lane_get(a_m2[a], dn);
+                        am1l[a] = lane_get(a_m1[a], up); am1r[a] = lane_get(a_m1[a], dn);
+                        if (l0) { am2l[a] = al_m2[a]; am1l[a] = al_m1[a]; }
+                    }
+                }
+                RP::template rpt<1>(P, 1, amY, am2l, a_m2, am2r, bmA, bpA, cY);
+                RP::template rpt<1>(P, 2, apY, am1l, a_m1, am1r, bmP, bpP, cY);
+            }
+            if (L >= R0 + 2) {
+                const int row = L - 2;
+                const long long off = (long long)(row + g.ng) * g.ldq + (i + g.ng);
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    R qadd = (R(0) - dtdy * uAp[m]) - dtdy * amY[m];
+                    R g1 = (R(0) - hy * bmA[m]) - hy * uBm[m];
+                    R g2 = (R(0) - hy * bpA[m]) - hy * uBp[m];
+                    R g2l = lane_get(g2, up), g1r = lane_get(g1, dn);
+                    R v = accOld[m] + dtdx * g2l;
+                    v = v + ((qadd - dtdy * (fY[m] - uF[m])) - dtdx * (g2 - g1));
+                    v = v - dtdx * g1r;
+                    if (produce) p.qnew[m * g.plane + off] = v;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NE; ++m) { uAp[m] = apY[m]; uF[m] = fY[m]; uBm[m] = bmP[m]; uBp[m] = bpP[m]; }
+        }
+
+        // ---- 5. rotate ---------------------------------------------------------
+#pragma unroll
+        for (int mw = 0; mw < NW; ++mw) {
+            sY[mw] = sN[mw]; dotlY[mw] = dotN[mw];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) wY[mw][m] = wN[mw][m];
+        }
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            amY[m] = amN[m]; apY[m] = apN[m]; accOld[m] = accNew[m];
+            qprev[m] = qc[m]; qlprev[m] = qlc[m];
+        }
+        cY = cN;
+        if constexpr (HAS_AUX) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                a_m2[a] = a_m1[a]; a_m1[a] = a_0[a];
+                al_m2[a] = al_m1[a]; al_m1[a] = al_0[a];
+            }
+        }
+    }
+
+    double c = wave_max((double)cfl);
+    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+}
+
+// ---------------------------------------------------------------------------
+// Dimensional splitting (classic/step2ds.f90): an x pass over every row,
+// ghost rows included, then a y pass over every column that reads the
+// x-updated values (ghost columns keep their boundary values).  Each pass is
+// a marching kernel like the one above, without transverse terms.
+// Pass X: lane L holds column c0-1+L, the band marches over rows; each row's
+// x-interfaces are independent of the other rows, so the "march" is only a
+// loop.  Pass Y: lane holds a column, marches up with a one-row lag so each
+// y-interface is limited with the waves below and above it.
+// ---------------------------------------------------------------------------
+template <class RP, typename R>
+struct SplitParams {
+    const R *qin;      // qold (pass X) or the x-updated state (pass Y)
+    R *qout;
+    const R *aux;
+    const R *qbc_src;  // pass Y: qold, supplies ghost columns as the ghost fill left them
+    GridDev g;
+    BCDev bq, ba;
+    typename RP::Params P;
+    const double *dt;
+    double *cfl;
+    double dx, dy;
+    int order;
+    int mthlim[8];
+    int H, nsx, nsy, nsyb, wpb;
+};
+
+// Pass X: rows [-2, my+2), updates interior columns of every row (ghost rows
+// included, step2ds.f90:96-167) into qout (padded, same layout).
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p) {
+    constexpr int NE = RP::NE, NW = RP::NW;
+    constexpr bool HAS_AUX = RP::NA > 0;
+    constexpr int NA = HAS_AUX ? RP::NA : 1;
+    using Cache = typename RP::Cache;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int vb = xcd_block();
+    const int sx = vb / p.nsyb;
+    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+    if (sx >= p.nsx || sy >= p.nsy) return;
+    const GridDev g = p.g;
+    const int c0 = sx * XS_USEFUL;
+    const int i = c0 - 1 + lane;
+    const int R0 = -CNG + sy * p.H;
+    const int R1 = min(R0 + p.H, g.my + CNG);
+    const int up = ((lane + WAVE - 1) & (WAVE - 1)) << 2;
+    const int dn = ((lane + 1) & (WAVE - 1)) << 2;
+    const bool l0 = (lane == 0);
+    const bool produce = lane >= 1 && lane <= XS_USEFUL && i < g.mx;
+    const double dtd = *p.dt;
+    const R dtdx = R(dtd / p.dx);
+    const R avx = R(0.5) * (dtdx + dtdx);
+    const auto &P = p.P;
+    R cfl = R(0);
+    for (int row = R0; row < R1; ++row) {
+        R hi[NE], lo[NE], ahi[NA], alo[NA];
+        load_q<NE>(p.qin, g, p.bq, i, row, hi);
+        R lo0[NE];
+        if (l0) load_q<NE>(p.qin, g, p.bq, i - 1, row, lo0);
+#pragma unroll
+        for (int m = 0; m < NE; ++m) { lo[m] = lane_get(hi[m], up); if (l0) lo[m] = lo0[m]; }
+        if constexpr (HAS_AUX) {
+            load_aux<NA>(p.aux, g, p.ba, i, row, ahi);
+            R a0l[NA];
+            if (l0) load_aux<NA>(p.aux, g, p.ba, i - 1, row, a0l);
+#pragma unroll
+            for (int a = 0; a < NA; ++a) { alo[a] = lane_get(ahi[a], up); if (l0) alo[a] = a0l[a]; }
+        }
+        R wave[NW][NE], s[NW], amdq[NE], apdq[NE], fadd[NE];
+        Cache cx;
+        RP::template rpn<0>(P, lo, hi, alo, ahi, wave, s, amdq, apdq, cx);
+        if (i >= 0 && i <= g.mx) {
+#pragma unroll
+            for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdx * s[mw]); cfl = rmax2<R>(cfl, -dtdx * s[mw]); }
+        }
+#pragma unroll
+        for (int m = 0; m < NE; ++m) fadd[m] = R(0);
+        if (p.order == 2) {
+#pragma unroll
+            for (int mw = 0; mw < NW; ++mw) {
+                R wn[NE];
+#pragma unroll
+                for (int m = 0; m < NE; ++m)
+                    wn[m] = RP::template WNZ<0>(m, mw) ? lane_get(wave[mw][m], dn) : R(0);
+                const int mth = p.mthlim[mw];
+                if (mth == 0) continue;
+                R wnorm2 = R(0), dotr = R(0);
+#pragma unroll
+                for (int m = 0; m < NE; ++m)
+                    if (RP::template WNZ<0>(m, mw)) { wnorm2 = wnorm2 + wave[mw][m] * wave[mw][m]; dotr = dotr + wave[mw][m] * wn[m]; }
+                R dotl = lane_get(dotr, up);
+                if (wnorm2 != R(0)) {
+                    R phi = philim<R>(wnorm2, s[mw] > R(0) ? dotl : dotr, mth);
+#pragma unroll
+                    for (int m = 0; m < NE; ++m)
+                        if (RP::template WNZ<0>(m, mw)) wave[mw][m] = phi * wave[mw][m];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                R c = R(0);
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw)
+                    if (RP::template WNZ<0>(m, mw)) c = c + rabs(s[mw]) * (R(1) - rabs(s[mw]) * avx) * wave[mw][m];
+                fadd[m] = R(0) + R(0.5) * c;
+            }
+        }
+        const long long off = (long long)(row + g.ng) * g.ldq + (i + g.ng);
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            R dAm = lane_get(amdq[m], dn), dF = lane_get(fadd[m], dn);
+            R qadd = (R(0) - dtdx * apdq[m]) - dtdx * dAm;
+            R v = (hi[m] + qadd) - dtdx * (dF - fadd[m]);
+            if (produce) p.qout[m * g.plane + off] = v;
+        }
+    }
+    double c = wave_max((double)cfl);
+    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+}
+
+// Pass Y: columns [-2, mx+2) (lane L of strip sx holds column -2 + sx*64 + L,
+// no halo columns needed); marches rows [R0-2, R1+1].  Interior columns read
+// the x-updated state (qin, ghost rows included); ghost columns read the state
+// the ghost fill left (qbc_src with the boundary codes), as in the reference,
+// where the x pass does not touch them.  Writes interior cells only.
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p) {
+    constexpr int NE = RP::NE, NW = RP::NW;
+    constexpr bool HAS_AUX = RP::NA > 0;
+    constexpr int NA = HAS_AUX ? RP::NA : 1;
+    using Cache = typename RP::Cache;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int vb = xcd_block();
+    const int sx = vb / p.nsyb;
+    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+    if (sx >= p.nsx || sy >= p.nsy) return;
+    const GridDev g = p.g;
+    const int i = -CNG + sx * WAVE + lane;
+    const bool interior_col = (i >= 0 && i < g.mx);
+    const bool valid_col = (i < g.mx + CNG);
+    const int R0 = sy * p.H;
+    const int R1 = min(R0 + p.H, g.my);
+    const double dtd = *p.dt;
+    const R dtdy = R(dtd / p.dy);
+    const R avy = R(0.5) * (dtdy + dtdy);
+    const auto &P = p.P;
+    R cfl = R(0);
+
+    auto loadq = [&](int row, R (&out)[NE]) {
+        if (interior_col) {
+            // x-updated state, ghost rows included (rows in [-ng, my+ng))
+            const long long off = (long long)(row + g.ng) * g.ldq + (i + g.ng);
+#pragma unroll
+            for (int m = 0; m < NE; ++m) out[m] = p.qin[m * g.plane + off];
+        } else {
+            load_q<NE>(p.qbc_src, g, p.bq, i, row, out);
+        }
+    };
+    R qprev[NE], aprev[NA], a0[NA];
+    loadq(R0 - 2, qprev);
+    if constexpr (HAS_AUX) load_aux<NA>(p.aux, g, p.ba, i, R0 - 2, aprev);
+    R wY[NW][NE], sY[NW], amY[NE], apY[NE], dotlY[NW];
+    R uAp[NE], uF[NE];
+#pragma unroll
+    for (int m = 0; m < NE; ++m) { amY[m] = apY[m] = uAp[m] = uF[m] = R(0); }
+#pragma unroll
+    for (int mw = 0; mw < NW; ++mw) {
+        sY[mw] = dotlY[mw] = R(0);
+#pragma unroll
+        for (int m = 0; m < NE; ++m) wY[mw][m] = R(0);
+    }
+    R qmid[NE];   // q at row L-2 (the cell being finished)
+#pragma unroll
+    for (int m = 0; m < NE; ++m) qmid[m] = R(0);
+    R qm1[NE];
+#pragma unroll
+    for (int m = 0; m < NE; ++m) qm1[m] = qprev[m];
+    for (int L = R0 - 1; L <= R1 + 1; ++L) {
+        R qc[NE];
+        loadq(L, qc);
+        if constexpr (HAS_AUX) load_aux<NA>(p.aux, g, p.ba, i, L, a0);
+        R wN[NW][NE], sN[NW], amN[NE], apN[NE], dotN[NW];
+        Cache cN;
+        RP::template rpn<1>(P, qm1, qc, aprev, a0, wN, sN, amN, apN, cN);
+        if (L >= 0 && L <= g.my && valid_col) {
+#pragma unroll
+            for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdy * sN[mw]); cfl = rmax2<R>(cfl, -dtdy * sN[mw]); }
+        }
+#pragma unroll
+        for (int mw = 0; mw < NW; ++mw) {
+            R d = R(0);
+#pragma unroll
+            for (int m = 0; m < NE; ++m)
+                if (RP::template WNZ<1>(m, mw)) d = d + wY[mw][m] * wN[mw][m];
+            dotN[mw] = d;
+        }
+        if (L >= R0 + 1) {
+            R fY[NE];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) fY[m] = R(0);
+            if (p.order == 2) {
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw) {
+                    const int mth = p.mthlim[mw];
+                    if (mth == 0) continue;
+                    R wnorm2 = R(0);
+#pragma unroll
+                    for (int m = 0; m < NE; ++m)
+                        if (RP::template WNZ<1>(m, mw)) wnorm2 = wnorm2 + wY[mw][m] * wY[mw][m];
+                    if (wnorm2 != R(0)) {
+                        R phi = philim<R>(wnorm2, sY[mw] > R(0) ? dotlY[mw] : dotN[mw], mth);
+#pragma unroll
+                        for (int m = 0; m < NE; ++m)
+                            if (RP::template WNZ<1>(m, mw)) wY[mw][m] = phi * wY[mw][m];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    R c = R(0);
+#pragma unroll
+                    for (int mw = 0; mw < NW; ++mw)
+                        if (RP::template WNZ<1>(m, mw)) c = c + rabs(sY[mw]) * (R(1) - rabs(sY[mw]) * avy) * wY[mw][m];
+                    fY[m] = R(0) + R(0.5) * c;
+                }
+            }
+            if (L >= R0 + 2 && interior_col) {
+                const int row = L - 2;
+                const long long off = (long long)(row + g.ng) * g.ldq + (i + g.ng);
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    R qadd = (R(0) - dtdy * uAp[m]) - dtdy * amY[m];
+                    R v = (qmid[m] + qadd) - dtdy * (fY[m] - uF[m]);
+                    p.qout[m * g.plane + off] = v;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NE; ++m) { uAp[m] = apY[m]; uF[m] = fY[m]; }
+        }
+#pragma unroll
+        for (int mw = 0; mw < NW; ++mw) {
+            sY[mw] = sN[mw]; dotlY[mw] = dotN[mw];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) wY[mw][m] = wN[mw][m];
+        }
+#pragma unroll
+        for (int m = 0; m < NE; ++m) { amY[m] = amN[m]; apY[m] = apN[m]; qmid[m] = qm1[m]; qm1[m] = qc[m]; }
+        if constexpr (HAS_AUX) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) aprev[a] = a0[a];
+        }
+    }
+    double c = wave_max((double)cfl);
+    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+}
+
+// The y pass's Courant number also covers the ghost rows' interfaces: the
+// reference's step2ds y sweep runs over all columns and, on each, over
+// interfaces 1..my+1 only (flux2 loop i=1..mx+1 in slice coordinates), so the
+// CFL set is columns [-ng, mx+ng) x interfaces [0, my].  Rows of the march
+// cover [R0-1, R1+1]; the bound above keeps interfaces within [0, my].
+
+__global__ void zero_scalar_kernel(double *p) { *p = 0.0; }
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct Ctx {
+    int device;
+    void *scratch = nullptr;   // split intermediate state
+    size_t scratch_bytes = 0;
+};
+
+static int band_height(int rows, int strips_x) {
+    // about 2048 wavefronts (two per SIMD) with bands of at least 16 rows
+    long long target = 2048;
+    int h = (int)std::max<long long>(16, ((long long)rows * strips_x + target - 1) / target);
+    return std::min(h, std::max(rows, 1));
+}
+
+template <class RP, typename R>
+static typename RP::Params make_params(const ClawRiemann *rp);
+
+template <> Acoustics<double>::Params make_params<Acoustics<double>, double>(const ClawRiemann *rp) {
+    return {rp->p[0], rp->p[1], rp->p[2], rp->p[3]};
+}
+template <> Acoustics<float>::Params make_params<Acoustics<float>, float>(const ClawRiemann *rp) {
+    return {(float)rp->p[0], (float)rp->p[1], (float)rp->p[2], (float)rp->p[3]};
+}
+template <> VCAcoustics<double>::Params make_params<VCAcoustics<double>, double>(const ClawRiemann *) { return {0.0}; }
+template <> VCAcoustics<float>::Params make_params<VCAcoustics<float>, float>(const ClawRiemann *) { return {0.0f}; }
+template <> Euler5<double>::Params make_params<Euler5<double>, double>(const ClawRiemann *rp) {
+    return {rp->p[0], rp->p[1], rp->efix};
+}
+template <> Euler5<float>::Params make_params<Euler5<float>, float>(const ClawRiemann *rp) {
+    return {(float)rp->p[0], (float)rp->p[1], rp->efix};
+}
+template <> ShallowRoe<double>::Params make_params<ShallowRoe<double>, double>(const ClawRiemann *rp) { return {rp->p[0]}; }
+template <> ShallowRoe<float>::Params make_params<ShallowRoe<float>, float>(const ClawRiemann *rp) { return {(float)rp->p[0]}; }
+
+static int check(hipError_t e) { return e == hipSuccess ? CLAW_OK : CLAW_EDEVICE; }
+
+template <class RP, typename R>
+static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                          const ClawBC *bc, const void *qold, void *qnew, const void *aux,
+                          const double *dt, double *cfl, int flags, hipStream_t st) {
+    const int wpb = 4;
+    GridDev g{grid->mx, grid->my, grid->ng, grid->ldq, grid->plane};
+    BCDev bq{{bc->lower[0], bc->lower[1]}, {bc->upper[0], bc->upper[1]}};
+    BCDev ba{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
+    if (!(flags & CLAW_STEP_ACCUMULATE_CFL))
+        hipLaunchKernelGGL(zero_scalar_kernel, dim3(1), dim3(1), 0, st, cfl);
+    if (!meth->split) {
+        ClassicParams<RP, R> p;
+        p.qold = (const R *)qold; p.qnew = (R *)qnew; p.aux = (const R *)aux;
+        p.g = g; p.bq = bq; p.ba = ba; p.P = make_params<RP, R>(rp);
+        p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy;
+        p.order = meth->order; p.trans = meth->transverse;
+        for (int k = 0; k < 8; ++k) p.mthlim[k] = meth->mthlim[k];
+        p.nsx = (grid->mx + XS_USEFUL - 1) / XS_USEFUL;
+        p.H = band_height(grid->my, p.nsx);
+        p.nsy = (grid->my + p.H - 1) / p.H;
+        p.wpb = wpb;
+        p.nsyb = (p.nsy + wpb - 1) / wpb;
+        int nblocks = p.nsx * p.nsyb;
+        hipLaunchKernelGGL((classic_unsplit_kernel<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
+        return check(hipGetLastError());
+    }
+    // split: x pass into scratch (padded layout, ghost rows included), then y pass
+    size_t bytes = (size_t)grid->plane * RP::NE * sizeof(R);
+    if (ctx->scratch_bytes < bytes) {
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        ctx->scratch = nullptr; ctx->scratch_bytes = 0;
+        if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return CLAW_ENOMEM;
+        ctx->scratch_bytes = bytes;
+    }
+    SplitParams<RP, R> px;
+    px.qin = (const R *)qold; px.qout = (R *)ctx->scratch; px.aux = (const R *)aux;
+    px.qbc_src = (const R *)qold;
+    px.g = g; px.bq = bq; px.ba = ba; px.P = make_params<RP, R>(rp);
+    px.dt = dt; px.cfl = cfl; px.dx = grid->dx; px.dy = grid->dy; px.order = meth->order;
+    for (int k = 0; k < 8; ++k) px.mthlim[k] = meth->mthlim[k];
+    px.wpb = wpb;
+    {
+        SplitParams<RP, R> q = px;
+        q.nsx = (grid->mx + XS_USEFUL - 1) / XS_USEFUL;
+        int rows = grid->my + 2 * CNG;
+        q.H = band_height(rows, q.nsx);
+        q.nsy = (rows + q.H - 1) / q.H;
+        q.nsyb = (q.nsy + wpb - 1) / wpb;
+        hipLaunchKernelGGL((split_x_kernel<RP, R>), dim3(q.nsx * q.nsyb), dim3(WAVE * wpb), 0, st, q);
+    }
+    {
+        SplitParams<RP, R> q = px;
+        q.qin = (const R *)ctx->scratch; q.qout = (R *)qnew;
+        int cols = grid->mx + 2 * CNG;
+        q.nsx = (cols + WAVE - 1) / WAVE;
+        q.H = band_height(grid->my, q.nsx);
+        q.nsy = (grid->my + q.H - 1) / q.H;
+        q.nsyb = (q.nsy + wpb - 1) / wpb;
+        hipLaunchKernelGGL((split_y_kernel<RP, R>), dim3(q.nsx * q.nsyb), dim3(WAVE * wpb), 0, st, q);
+    }
+    return check(hipGetLastError());
+}
+
+}  // namespace clawhip
+
+using namespace clawhip;
+
+extern "C" {
+
+int claw_ctx_create(int device, void **out) {
+    if (!out) return CLAW_EINVAL;
+    Ctx *c = new (std::nothrow) Ctx();
+    if (!c) return CLAW_ENOMEM;
+    c->device = device;
+    *out = c;
+    return CLAW_OK;
+}
+
+int claw_ctx_destroy(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return CLAW_EINVAL;
+    if (c->scratch) (void)hipFree(c->scratch);
+    delete c;
+    return CLAW_OK;
+}
+
+int claw2d_classic_launches(const ClawMethod *m) { return m && m->split ? 3 : 2; }
+
+int claw2d_classic_step(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                        const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
+                        const double *dt_dev, double *cfl_dev, void *stream) {
+    return claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev, 0,
+                                  stream);
+}
+
+int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                           const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
+                           const double *dt_dev, double *cfl_dev, int flags, void *stream) {
+    if (!ctx || !grid || !meth || !rp || !bc || !qold || !qnew || !dt_dev || !cfl_dev) return CLAW_EINVAL;
+    if (qold == qnew) return CLAW_EINVAL;
+    if (grid->mx < 2 || grid->my < 2 || grid->ng < 2) return CLAW_EINVAL;
+    if (grid->ldq < grid->mx + 2 * grid->ng || grid->plane < grid->ldq * (grid->my + 2 * grid->ng))
+        return CLAW_EINVAL;
+    if (meth->fwave) return CLAW_EINVAL;
+    if (meth->order != 1 && meth->order != 2) return CLAW_EINVAL;
+    if (meth->transverse < 0 || meth->transverse > 2) return CLAW_EINVAL;
+    for (int s = 0; s < 2; ++s) {
+        if (bc->lower[s] < 0 || bc->lower[s] > 3 || bc->upper[s] < 0 || bc->upper[s] > 3) return CLAW_EINVAL;
+        // periodic needs both sides periodic (solver.py:391-393 / :426-428)
+        if ((bc->lower[s] == CLAW_BC_PERIODIC) != (bc->upper[s] == CLAW_BC_PERIODIC)) return CLAW_EINVAL;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    Ctx *c = (Ctx *)ctx;
+    int nw_expected;
+    switch (rp->kind) {
+    case CLAW_RP_ACOUSTICS: nw_expected = 2; break;
+    case CLAW_RP_VC_ACOUSTICS: nw_expected = 2; if (!aux) return CLAW_EINVAL; break;
+    case CLAW_RP_EULER: nw_expected = 5; break;
+    case CLAW_RP_SHALLOW: nw_expected = 3; break;
+    default: return CLAW_EINVAL;
+    }
+    if (meth->num_waves != nw_expected) return CLAW_EINVAL;
+    const bool f32 = (precision == CLAW_F32);
+    if (precision != CLAW_F32 && precision != CLAW_F64) return CLAW_EINVAL;
+#define DISPATCH(RPT)                                                                                      \
+    return f32 ? launch_classic<RPT<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st) \
+               : launch_classic<RPT<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st)
+    switch (rp->kind) {
+    case CLAW_RP_ACOUSTICS: DISPATCH(Acoustics);
+    case CLAW_RP_VC_ACOUSTICS: DISPATCH(VCAcoustics);
+    case CLAW_RP_EULER: DISPATCH(Euler5);
+    default: DISPATCH(ShallowRoe);
+    }
+#undef DISPATCH
+}
+
+}  // extern "C"

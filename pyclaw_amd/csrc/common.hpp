@@ -1,0 +1,134 @@
+// Shared device helpers: lane exchange, boundary synthesis, the wave limiter.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/clawhip.h"
+
+namespace clawhip {
+
+constexpr int WAVE = 64;
+
+// ---- lane exchange (ds_bpermute; addresses are byte offsets lane*4) -------
+__device__ __forceinline__ float lane_get(float v, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_get(double v, int addr) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffll));
+    int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// ---- grid description on the device ---------------------------------------
+struct GridDev {
+    int mx, my, ng;
+    long long ldq, plane;
+};
+
+struct BCDev {
+    int lo[2], hi[2];
+};
+
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Map a 1-D ghost index to its source (solver.py:386-434 for one side).
+// Returns the source index in [0, n-1]; sets flip for wall.
+__device__ __forceinline__ int bc_map1(int k, int n, int code, bool &flip) {
+    int s;
+    if (k < 0) {
+        s = code == CLAW_BC_PERIODIC ? k + n : (code == CLAW_BC_WALL ? -1 - k : 0);
+    } else {
+        s = code == CLAW_BC_PERIODIC ? k - n : (code == CLAW_BC_WALL ? 2 * n - 1 - k : n - 1);
+    }
+    flip = (code == CLAW_BC_WALL);
+    return iclamp(s, 0, n - 1);
+}
+
+// Element offset of the value that the reference's ghost fill (x sides first,
+// then y sides over the full padded width) leaves at cell (i, j), plus which
+// momentum components a wall reflection negates.  Out-of-range lanes are
+// clamped into the array (their results are never used).
+struct SrcCell { long long off; bool fx, fy; };
+
+__device__ __forceinline__ SrcCell bc_source(int i, int j, const GridDev &g, const BCDev &bc) {
+    SrcCell r;
+    r.fx = false; r.fy = false;
+    int ii = i, jj = j;
+    bool stored_all = false;
+    if (j < 0 || j >= g.my) {
+        int code = j < 0 ? bc.lo[1] : bc.hi[1];
+        if (code == CLAW_BC_STORED) {
+            stored_all = true;
+            jj = iclamp(j, -g.ng, g.my + g.ng - 1);
+            ii = iclamp(i, -g.ng, g.mx + g.ng - 1);
+        } else {
+            jj = bc_map1(j, g.my, code, r.fy);
+        }
+    }
+    if (!stored_all && (i < 0 || i >= g.mx)) {
+        int code = i < 0 ? bc.lo[0] : bc.hi[0];
+        if (code == CLAW_BC_STORED) ii = iclamp(i, -g.ng, g.mx + g.ng - 1);
+        else ii = bc_map1(i, g.mx, code, r.fx);
+    }
+    r.off = (long long)(jj + g.ng) * g.ldq + (ii + g.ng);
+    return r;
+}
+
+template <int NE, typename R>
+__device__ __forceinline__ void load_q(const R *__restrict__ q, const GridDev &g, const BCDev &bc,
+                                       int i, int j, R (&out)[NE]) {
+    SrcCell s = bc_source(i, j, g, bc);
+#pragma unroll
+    for (int m = 0; m < NE; ++m) {
+        R v = q[m * g.plane + s.off];
+        if ((m == 1 && s.fx) || (m == 2 && s.fy)) v = -v;
+        out[m] = v;
+    }
+}
+
+template <int NA, typename R>
+__device__ __forceinline__ void load_aux(const R *__restrict__ a, const GridDev &g, const BCDev &bc,
+                                         int i, int j, R (&out)[NA]) {
+    SrcCell s = bc_source(i, j, g, bc);
+#pragma unroll
+    for (int m = 0; m < NA; ++m) out[m] = a[m * g.plane + s.off];
+}
+
+__device__ __forceinline__ double rabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float rabs(float x) { return __builtin_fabsf(x); }
+
+// ---- wave limiter (classic/philim.f90:4-57) ---------------------------------
+template <typename R> __device__ __forceinline__ R rmax2(R a, R b) { return a > b ? a : b; }
+template <typename R> __device__ __forceinline__ R rmin2(R a, R b) { return a < b ? a : b; }
+
+template <typename R>
+__device__ __forceinline__ R philim(R a, R b, int meth) {
+    R r = b / a;
+    switch (meth) {
+    case 2: return rmax2<R>(rmax2<R>(R(0), rmin2<R>(R(1), R(2) * r)), rmin2<R>(R(2), r));
+    case 3: { R ar = rabs(r); return (r + ar) / (R(1) + ar); }
+    case 4: { R c = (R(1) + r) / R(2); return rmax2<R>(R(0), rmin2<R>(rmin2<R>(c, R(2)), R(2) * r)); }
+    case 5: return r;
+    default: return rmax2<R>(R(0), rmin2<R>(R(1), r));
+    }
+}
+
+
+// Deterministic non-negative max: the bit pattern of a non-negative IEEE value
+// orders like the value, so an integer atomic max is exact and order-free.
+__device__ __forceinline__ void atomic_max_nonneg(double *p, double v) {
+    if (!(v > 0.0)) return;
+    atomicMax(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+}  // namespace clawhip

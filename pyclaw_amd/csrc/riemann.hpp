@@ -1,0 +1,377 @@
+// Riemann solvers for the fused 2D wave-propagation kernels (gfx950).
+//
+// Each solver is a struct with compile-time sizes and two device functions:
+//   rpn<DIR>(...)  normal solve at one interface:  waves, speeds, A^-dq, A^+dq
+//   rpt<DIR>(...)  transverse split of one fluctuation into B^-A*dq, B^+A*dq
+// DIR = 0 for an x-interface (normal momentum is component 1), DIR = 1 for a
+// y-interface (normal momentum is component 2), as mu/mv in the reference.
+// "lo" is the cell below/left of the interface (the reference's qr(i-1)),
+// "hi" the cell above/right (ql(i)).
+//
+// Operation order in every expression follows the reference Fortran (or, for
+// the solvers the reference takes from clawpack.riemann, this repository's
+// restatement of them), so that an FP64 build compiled without contraction
+// gives the same bits as the CPU reference.
+//   Euler 5-wave Roe + entropy fix: development/rp_approaches/rpn2_euler_5wave.f:87-298,
+//                                   rpt2_euler_5wave.f:349-395
+//   acoustics / vc-acoustics / shallow-water Roe: external to the reference;
+//   call sites apps/acoustics_2d_homogeneous/acoustics.py:25-26,
+//   apps/acoustics_2d_variable/acoustics.py:25-26, apps/shallow_2d/shallow2D.py:44-45.
+//
+// WNZ(m, mw) marks wave components that are not identically zero; kernels use
+// it to skip zero terms in cross-lane exchanges and dot products (adding an
+// exact zero does not change a floating-point sum).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace clawhip {
+
+enum RiemannKind { RP_ACOUSTICS = 0, RP_VC_ACOUSTICS = 1, RP_EULER = 2, RP_SHALLOW = 3 };
+
+template <typename R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+template <typename R> __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+
+template <typename R> __device__ __forceinline__ R rsqrt_exact(R x);
+template <> __device__ __forceinline__ double rsqrt_exact<double>(double x) { return __builtin_sqrt(x); }
+template <> __device__ __forceinline__ float rsqrt_exact<float>(float x) { return __builtin_sqrtf(x); }
+
+// ---------------------------------------------------------------------------
+// Constant-coefficient acoustics: q = (p, u, v), parameters rho, bulk, c, Z.
+// ---------------------------------------------------------------------------
+template <typename R> struct Acoustics {
+    static constexpr int NE = 3, NW = 2, NA = 0;
+    struct Params { R rho, bulk, cc, zz; };
+    struct Cache {};
+    template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
+    template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
+    template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
+        return m != (DIR == 0 ? 2 : 1);
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpn(const Params &P, const R (&lo)[NE], const R (&hi)[NE],
+                                               const R *alo, const R *ahi, R (&wave)[NW][NE], R (&s)[NW],
+                                               R (&amdq)[NE], R (&apdq)[NE], Cache &) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        R d0 = hi[0] - lo[0];
+        R d1 = hi[mu] - lo[mu];
+        R a1 = (-d0 + P.zz * d1) / (R(2) * P.zz);
+        R a2 = (d0 + P.zz * d1) / (R(2) * P.zz);
+        wave[0][0] = -a1 * P.zz; wave[0][mu] = a1; wave[0][mv] = R(0); s[0] = -P.cc;
+        wave[1][0] = a2 * P.zz; wave[1][mu] = a2; wave[1][mv] = R(0); s[1] = P.cc;
+#pragma unroll
+        for (int m = 0; m < NE; ++m) { amdq[m] = s[0] * wave[0][m]; apdq[m] = s[1] * wave[1][m]; }
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpt(const Params &P, int imp, const R (&asdq)[NE], const R *a1_,
+                                               const R *a2_, const R *a3_, R (&bm)[NE], R (&bp)[NE],
+                                               const Cache &) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        R a1 = (-asdq[0] + P.zz * asdq[mv]) / (R(2) * P.zz);
+        R a2 = (asdq[0] + P.zz * asdq[mv]) / (R(2) * P.zz);
+        bm[0] = P.cc * a1 * P.zz; bm[mu] = R(0); bm[mv] = -P.cc * a1;
+        bp[0] = P.cc * a2 * P.zz; bp[mu] = R(0); bp[mv] = P.cc * a2;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Variable-coefficient acoustics: aux = (rho, c) per cell.
+// ---------------------------------------------------------------------------
+template <typename R> struct VCAcoustics {
+    static constexpr int NE = 3, NW = 2, NA = 2;
+    struct Params { R unused; };
+    struct Cache {};
+    template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
+    template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
+    template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
+        return m != (DIR == 0 ? 2 : 1);
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpn(const Params &, const R (&lo)[NE], const R (&hi)[NE],
+                                               const R *alo, const R *ahi, R (&wave)[NW][NE], R (&s)[NW],
+                                               R (&amdq)[NE], R (&apdq)[NE], Cache &) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        R d0 = hi[0] - lo[0];
+        R d1 = hi[mu] - lo[mu];
+        R zi = ahi[0] * ahi[1];
+        R zim = alo[0] * alo[1];
+        R a1 = (-d0 + zi * d1) / (zim + zi);
+        R a2 = (d0 + zim * d1) / (zim + zi);
+        wave[0][0] = -a1 * zim; wave[0][mu] = a1; wave[0][mv] = R(0); s[0] = -alo[1];
+        wave[1][0] = a2 * zi; wave[1][mu] = a2; wave[1][mv] = R(0); s[1] = ahi[1];
+#pragma unroll
+        for (int m = 0; m < NE; ++m) { amdq[m] = s[0] * wave[0][m]; apdq[m] = s[1] * wave[1][m]; }
+    }
+    // am/a0/ap: aux of the cell that receives the fluctuation (a0) and of its
+    // transverse neighbours below (am) and above (ap).
+    template <int DIR>
+    __device__ static __forceinline__ void rpt(const Params &, int imp, const R (&asdq)[NE], const R *am,
+                                               const R *a0, const R *ap, R (&bm)[NE], R (&bp)[NE],
+                                               const Cache &) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        R cm = am[1], cp = ap[1];
+        R zm = am[0] * am[1];
+        R zz = a0[0] * a0[1];
+        R zp = ap[0] * ap[1];
+        R a1 = (-asdq[0] + asdq[mv] * zz) / (zm + zz);
+        R a2 = (asdq[0] + asdq[mv] * zz) / (zz + zp);
+        bm[0] = cm * a1 * zm; bm[mu] = R(0); bm[mv] = -cm * a1;
+        bp[0] = cp * a2 * zp; bp[mu] = R(0); bp[mv] = cp * a2;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Euler equations with a tracer: q = (rho, rho u, rho v, E, rho*phi... tracer)
+// 5 waves: u-a, shear, entropy, u+a, tracer.
+// ---------------------------------------------------------------------------
+template <typename R> struct Euler5 {
+    static constexpr int NE = 5, NW = 5, NA = 0;
+    struct Params { R gamma, gamma1; int efix; };
+    struct Cache { R u, v, enth, a, u2v2, g1a2, euv; };
+    template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
+    template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
+    template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
+        // wave 0,2,3: components 0..3; wave 1 (shear): mv and 3; wave 4 (tracer): 4
+        return (mw == 0 || mw == 2 || mw == 3) ? (m < 4)
+             : (mw == 1) ? (m == (DIR == 0 ? 2 : 1) || m == 3)
+             : (m == 4);
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpn(const Params &P, const R (&lo)[NE], const R (&hi)[NE],
+                                               const R *, const R *, R (&wave)[NW][NE], R (&s)[NW],
+                                               R (&amdq)[NE], R (&apdq)[NE], Cache &C) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        const R gamma1 = P.gamma1;
+        R rhsqrtl = rsqrt_exact<R>(lo[0]);
+        R rhsqrtr = rsqrt_exact<R>(hi[0]);
+        R pl = gamma1 * (lo[3] - R(0.5) * (lo[1] * lo[1] + lo[2] * lo[2]) / lo[0]);
+        R pr = gamma1 * (hi[3] - R(0.5) * (hi[1] * hi[1] + hi[2] * hi[2]) / hi[0]);
+        R rhsq2 = rhsqrtl + rhsqrtr;
+        R u = (lo[mu] / rhsqrtl + hi[mu] / rhsqrtr) / rhsq2;
+        R v = (lo[mv] / rhsqrtl + hi[mv] / rhsqrtr) / rhsq2;
+        R enth = ((lo[3] + pl) / rhsqrtl + (hi[3] + pr) / rhsqrtr) / rhsq2;
+        R u2v2 = u * u + v * v;
+        R a2 = gamma1 * (enth - R(0.5) * u2v2);
+        R a = rsqrt_exact<R>(a2);
+        R g1a2 = gamma1 / a2;
+        R euv = enth - u2v2;
+        C.u = u; C.v = v; C.enth = enth; C.a = a; C.u2v2 = u2v2; C.g1a2 = g1a2; C.euv = euv;
+
+        R d0 = hi[0] - lo[0];
+        R d1 = hi[mu] - lo[mu];
+        R d2 = hi[mv] - lo[mv];
+        R d3 = hi[3] - lo[3];
+        R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
+        R a2w = d2 - v * d0;
+        R a4 = (d1 + (a - u) * d0 - a * a3) / (R(2) * a);
+        R a1 = d0 - a3 - a4;
+        wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v;
+        wave[0][3] = a1 * (enth - u * a); wave[0][4] = R(0); s[0] = u - a;
+        wave[1][0] = R(0); wave[1][mu] = R(0); wave[1][mv] = a2w; wave[1][3] = a2w * v;
+        wave[1][4] = R(0); s[1] = u;
+        wave[2][0] = a3; wave[2][mu] = a3 * u; wave[2][mv] = a3 * v;
+        wave[2][3] = a3 * R(0.5) * u2v2; wave[2][4] = R(0); s[2] = u;
+        wave[3][0] = a4; wave[3][mu] = a4 * (u + a); wave[3][mv] = a4 * v;
+        wave[3][3] = a4 * (enth + u * a); wave[3][4] = R(0); s[3] = u + a;
+        wave[4][0] = R(0); wave[4][mu] = R(0); wave[4][mv] = R(0); wave[4][3] = R(0);
+        wave[4][4] = hi[4] - lo[4]; s[4] = u;
+
+        if (!P.efix) {
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                R am = R(0), ap = R(0);
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw) {
+                    if (s[mw] < R(0)) am = am + s[mw] * wave[mw][m];
+                    else ap = ap + s[mw] * wave[mw][m];
+                }
+                amdq[m] = am; apdq[m] = ap;
+            }
+            return;
+        }
+        const R gamma = P.gamma;
+        // rpn2_euler_5wave.f:205-286: amdq with the transonic-rarefaction fix
+        do {
+            R sfract;
+            R rhoim1 = lo[0];
+            R pim1 = gamma1 * (lo[3] - R(0.5) * (lo[mu] * lo[mu] + lo[mv] * lo[mv]) / rhoim1);
+            R cim1 = rsqrt_exact<R>(gamma * pim1 / rhoim1);
+            R s0 = lo[mu] / rhoim1 - cim1;
+            if (s0 >= R(0) && s[0] > R(0)) {
+#pragma unroll
+                for (int m = 0; m < NE; ++m) amdq[m] = R(0);
+                break;
+            }
+            R rho1 = lo[0] + wave[0][0];
+            R rhou1 = lo[mu] + wave[0][mu];
+            R rhov1 = lo[mv] + wave[0][mv];
+            R en1 = lo[3] + wave[0][3];
+            R p1 = gamma1 * (en1 - R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+            R c1 = rsqrt_exact<R>(gamma * p1 / rho1);
+            R s1 = rhou1 / rho1 - c1;
+            if (s0 < R(0) && s1 > R(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+            else if (s[0] < R(0)) sfract = s[0];
+            else sfract = R(0);
+#pragma unroll
+            for (int m = 0; m < NE; ++m) amdq[m] = sfract * wave[0][m];
+            if (s[1] >= R(0)) break;
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                amdq[m] = amdq[m] + s[1] * wave[1][m];
+                amdq[m] = amdq[m] + s[2] * wave[2][m];
+                amdq[m] = amdq[m] + s[4] * wave[4][m];
+            }
+            R rhoi = hi[0];
+            R pi = gamma1 * (hi[3] - R(0.5) * (hi[mu] * hi[mu] + hi[mv] * hi[mv]) / rhoi);
+            R ci = rsqrt_exact<R>(gamma * pi / rhoi);
+            R s3 = hi[mu] / rhoi + ci;
+            R rho2 = hi[0] - wave[3][0];
+            R rhou2 = hi[mu] - wave[3][mu];
+            R rhov2 = hi[mv] - wave[3][mv];
+            R en2 = hi[3] - wave[3][3];
+            R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+            R c2 = rsqrt_exact<R>(gamma * p2 / rho2);
+            R s2 = rhou2 / rho2 + c2;
+            if (s2 < R(0) && s3 > R(0)) sfract = s2 * (s3 - s[3]) / (s3 - s2);
+            else if (s[3] < R(0)) sfract = s[3];
+            else break;
+#pragma unroll
+            for (int m = 0; m < NE; ++m) amdq[m] = amdq[m] + sfract * wave[3][m];
+        } while (0);
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            R df = R(0);
+#pragma unroll
+            for (int mw = 0; mw < NW; ++mw) df = df + s[mw] * wave[mw][m];
+            apdq[m] = df - amdq[m];
+        }
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpt(const Params &, int, const R (&asdq)[NE], const R *, const R *,
+                                               const R *, R (&bm)[NE], R (&bp)[NE], const Cache &C) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        const R u = C.u, v = C.v, a = C.a, enth = C.enth;
+        R a3 = C.g1a2 * (C.euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
+        R a2 = asdq[mu] - u * asdq[0];
+        R a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
+        R a1 = asdq[0] - a3 - a4;
+        R wb[4][NE], sb[4];
+        wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); wb[0][3] = a1 * (enth - v * a);
+        wb[0][4] = R(0); sb[0] = v - a;
+        wb[1][0] = a3; wb[1][mu] = a3 * u + a2; wb[1][mv] = a3 * v; wb[1][3] = a3 * R(0.5) * C.u2v2 + a2 * u;
+        wb[1][4] = R(0); sb[1] = v;
+        wb[2][0] = a4; wb[2][mu] = a4 * u; wb[2][mv] = a4 * (v + a); wb[2][3] = a4 * (enth + v * a);
+        wb[2][4] = R(0); sb[2] = v + a;
+        wb[3][0] = R(0); wb[3][mu] = R(0); wb[3][mv] = R(0); wb[3][3] = R(0); wb[3][4] = asdq[4];
+        sb[3] = v;
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            R b_m = R(0), b_p = R(0);
+#pragma unroll
+            for (int mw = 0; mw < 4; ++mw) {
+                b_m = b_m + rmin<R>(sb[mw], R(0)) * wb[mw][m];
+                b_p = b_p + rmax<R>(sb[mw], R(0)) * wb[mw][m];
+            }
+            bm[m] = b_m; bp[m] = b_p;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Shallow water: q = (h, hu, hv), Roe solver with entropy fix, gravity grav.
+// ---------------------------------------------------------------------------
+template <typename R> struct ShallowRoe {
+    static constexpr int NE = 3, NW = 3, NA = 0;
+    struct Params { R grav; };
+    struct Cache { R u, v, a; };
+    template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
+    template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
+    template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
+        return mw != 1 || m == (DIR == 0 ? 2 : 1);
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpn(const Params &P, const R (&lo)[NE], const R (&hi)[NE],
+                                               const R *, const R *, R (&wave)[NW][NE], R (&s)[NW],
+                                               R (&amdq)[NE], R (&apdq)[NE], Cache &C) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        const R grav = P.grav;
+        R hl = lo[0], hr = hi[0];
+        R hbar = (hl + hr) * R(0.5);
+        R hsqrtl = rsqrt_exact<R>(hl), hsqrtr = rsqrt_exact<R>(hr);
+        R hsq2 = hsqrtl + hsqrtr;
+        R u = (lo[mu] / hsqrtl + hi[mu] / hsqrtr) / hsq2;
+        R v = (lo[mv] / hsqrtl + hi[mv] / hsqrtr) / hsq2;
+        R a = rsqrt_exact<R>(grav * hbar);
+        C.u = u; C.v = v; C.a = a;
+        R d0 = hr - hl;
+        R d1 = hi[mu] - lo[mu];
+        R d2 = hi[mv] - lo[mv];
+        R a1 = ((u + a) * d0 - d1) / (R(2) * a);
+        R a2 = -v * d0 + d2;
+        R a3 = (-(u - a) * d0 + d1) / (R(2) * a);
+        wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
+        wave[1][0] = R(0); wave[1][mu] = R(0); wave[1][mv] = a2; s[1] = u;
+        wave[2][0] = a3; wave[2][mu] = a3 * (u + a); wave[2][mv] = a3 * v; s[2] = u + a;
+        do {
+            R sfract;
+            R s0 = lo[mu] / hl - rsqrt_exact<R>(grav * hl);
+            if (s0 >= R(0) && s[0] > R(0)) {
+#pragma unroll
+                for (int m = 0; m < NE; ++m) amdq[m] = R(0);
+                break;
+            }
+            R h1 = hl + wave[0][0];
+            R hu1 = lo[mu] + wave[0][mu];
+            R s1 = hu1 / h1 - rsqrt_exact<R>(grav * h1);
+            if (s0 < R(0) && s1 > R(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+            else if (s[0] < R(0)) sfract = s[0];
+            else sfract = R(0);
+#pragma unroll
+            for (int m = 0; m < NE; ++m) amdq[m] = sfract * wave[0][m];
+            if (s[1] >= R(0)) break;
+#pragma unroll
+            for (int m = 0; m < NE; ++m) amdq[m] = amdq[m] + s[1] * wave[1][m];
+            R s03 = hi[mu] / hr + rsqrt_exact<R>(grav * hr);
+            R h3 = hr - wave[2][0];
+            R hu3 = hi[mu] - wave[2][mu];
+            R s3 = hu3 / h3 + rsqrt_exact<R>(grav * h3);
+            if (s3 < R(0) && s03 > R(0)) sfract = s3 * (s03 - s[2]) / (s03 - s3);
+            else if (s[2] < R(0)) sfract = s[2];
+            else break;
+#pragma unroll
+            for (int m = 0; m < NE; ++m) amdq[m] = amdq[m] + sfract * wave[2][m];
+        } while (0);
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            R df = R(0);
+#pragma unroll
+            for (int mw = 0; mw < NW; ++mw) df = df + s[mw] * wave[mw][m];
+            apdq[m] = df - amdq[m];
+        }
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpt(const Params &, int, const R (&asdq)[NE], const R *, const R *,
+                                               const R *, R (&bm)[NE], R (&bp)[NE], const Cache &C) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        const R u = C.u, v = C.v, a = C.a;
+        R q0 = asdq[0], qu = asdq[mu], qv = asdq[mv];
+        R a1 = (R(0.5) / a) * ((v + a) * q0 - qv);
+        R a2 = qu - u * q0;
+        R a3 = (R(0.5) / a) * (-(v - a) * q0 + qv);
+        R wb[3][NE], sb[3];
+        wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
+        wb[1][0] = R(0); wb[1][mu] = a2; wb[1][mv] = R(0); sb[1] = v;
+        wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            R b_m = R(0), b_p = R(0);
+#pragma unroll
+            for (int mw = 0; mw < 3; ++mw) {
+                b_m = b_m + rmin<R>(sb[mw], R(0)) * wb[mw][m];
+                b_p = b_p + rmax<R>(sb[mw], R(0)) * wb[mw][m];
+            }
+            bm[m] = b_m; bp[m] = b_p;
+        }
+    }
+};
+
+}  // namespace clawhip

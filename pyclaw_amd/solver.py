@@ -1,0 +1,382 @@
+"""Solver base class: boundary conditions, the time-step loop, gauges.
+
+Behaviour follows the reference's Solver (src/pyclaw/solver.py:25-742):
+  * the BC enumeration and the ghost-fill rules (extrap / periodic / wall /
+    custom callback; x sides then y sides, each fill spanning the full padded
+    extent of the other dimension, solver.py:299-434 and aux :438-576);
+  * evolve_to_time (solver.py:582-699): the same end-time clipping, accept /
+    reject on cfl <= cfl_max, dt = min(dt_max, dt*cfl_desired/cfl), status
+    dictionary, and the same exceptions;
+  * unknown attribute assignment after construction raises TypeError
+    (solver.py:121-126).
+On the GPU the step writes into a second buffer, so the reference's q_backup
+copy is replaced by keeping the previous buffer until the step is accepted.
+"""
+import logging
+
+import numpy as np
+import torch
+
+from . import _lib
+from .cfl import CFL
+from .state import NG_STORE
+
+
+class CFLError(Exception):
+    def __init__(self, msg):
+        super(CFLError, self).__init__(msg)
+
+
+class BC(object):
+    """Boundary condition codes (solver.py:12-18)."""
+    custom = 0
+    extrap = 1
+    periodic = 2
+    wall = 3
+
+
+def default_compute_gauge_values(q, aux):
+    return q
+
+
+def fill_ghosts(field, ng, bc_lower, bc_upper, negate, state=None, t=0.0, user_lower=None,
+                user_upper=None, user_args=None):
+    """Fill the ng ghost layers around a padded field view, reference order.
+
+    field: tensor (ncomp, mx+2ng, my+2ng) indexed [m, i, j] (the reference's qbc).
+    negate: True for q (wall negates the normal component), False for aux."""
+    for idim in range(2):
+        for side in (0, 1):
+            code = (bc_lower if side == 0 else bc_upper)[idim]
+            qb = field if idim == 0 else field.permute(0, 2, 1)   # dimension idim moved to axis 1
+            if code == BC.custom:
+                fn = user_lower if side == 0 else user_upper
+                if fn is None:
+                    raise Exception("Custom BC function has not been set.")
+                dim = state.patch.dimensions[idim] if state is not None else idim
+                fn(state, dim, t, field, ng)
+            elif code == BC.extrap:
+                for i in range(ng):
+                    if side == 0:
+                        qb[:, i, ...] = qb[:, ng, ...]
+                    else:
+                        qb[:, -i - 1, ...] = qb[:, -ng - 1, ...]
+            elif code == BC.periodic:
+                if side == 0:
+                    qb[:, :ng, ...] = qb[:, -2 * ng:-ng, ...].clone()
+                else:
+                    qb[:, -ng:, ...] = qb[:, ng:2 * ng, ...].clone()
+            elif code == BC.wall:
+                for i in range(ng):
+                    if side == 0:
+                        qb[:, i, ...] = qb[:, 2 * ng - 1 - i, ...]
+                        if negate:
+                            qb[idim + 1, i, ...] = -qb[idim + 1, 2 * ng - 1 - i, ...]
+                    else:
+                        qb[:, -i - 1, ...] = qb[:, -2 * ng + i, ...]
+                        if negate:
+                            qb[idim + 1, -i - 1, ...] = -qb[idim + 1, -2 * ng + i, ...]
+            elif code is None:
+                raise Exception("One or more of the boundary conditions has not been specified.")
+            else:
+                raise NotImplementedError("Boundary condition %s not implemented" % code)
+
+
+class Solver(object):
+    """Base class of the 2D solvers."""
+
+    def __setattr__(self, key, value):
+        if not hasattr(self, '_isinitialized'):
+            self.__dict__['_isinitialized'] = False
+        if self._isinitialized and not hasattr(self, key):
+            raise TypeError("%s has no attribute %s" % (self.__class__, key))
+        object.__setattr__(self, key, value)
+
+    def __init__(self, riemann_solver=None, claw_package=None):
+        self.logger = logging.getLogger('evolve')
+        self.dt_initial = 0.1
+        self.dt_max = 1e99
+        self.max_steps = 10000
+        self.dt_variable = True
+        self.num_waves = None
+        self.num_eqn = None
+        self.rp = None
+        self._is_set_up = False
+        self.dt = self.dt_initial
+        self.cfl = CFL(self.cfl_desired)
+        self.status = {'cflmax': self.cfl.get_cached_max(), 'dtmin': self.dt, 'dtmax': self.dt,
+                       'numsteps': 0}
+        self.bc_lower = [None] * self.num_dim
+        self.bc_upper = [None] * self.num_dim
+        self.aux_bc_lower = [None] * self.num_dim
+        self.aux_bc_upper = [None] * self.num_dim
+        self.user_bc_lower = None
+        self.user_bc_upper = None
+        self.user_aux_bc_lower = None
+        self.user_aux_bc_upper = None
+        self.custom_bc_static = False
+        self.compute_gauge_values = default_compute_gauge_values
+        self.qbc = None
+        self.auxbc = None
+        self._ctx = None
+        self._qalt = None
+        self._dt_dev = None
+        self._cfl_dev = None
+        self._static_filled = set()
+        self._backup_buf = None
+        if riemann_solver is not None:
+            self.rp = riemann_solver
+            self.num_eqn = riemann_solver.num_eqn
+            self.num_waves = riemann_solver.num_waves
+        self._isinitialized = True
+
+    @property
+    def all_bcs(self):
+        return self.bc_lower, self.bc_upper
+
+    @all_bcs.setter
+    def all_bcs(self, all_bcs):
+        for i in range(self.num_dim):
+            self.bc_lower[i] = all_bcs
+            self.bc_upper[i] = all_bcs
+
+    def is_valid(self):
+        valid = True
+        if any(b == BC.custom for b in self.bc_lower) and self.user_bc_lower is None:
+            self.logger.debug('Lower custom BC function has not been set.')
+            valid = False
+        if any(b == BC.custom for b in self.bc_upper) and self.user_bc_upper is None:
+            self.logger.debug('Upper custom BC function has not been set.')
+            valid = False
+        return valid
+
+    def setup(self, solution):
+        pass
+
+    def teardown(self):
+        pass
+
+    def __str__(self):
+        return "Solver Status:\n" + "\n".join("%s = %s" % (k.rjust(25), v) for k, v in self.status.items())
+
+    # ---- device resources -------------------------------------------------------
+    def _device_setup(self, state):
+        """Move the state to the GPU and allocate the step buffers."""
+        if not torch.cuda.is_available():
+            raise _lib.ClawError("pyclaw_amd solvers run on the GPU; no GPU is visible")
+        _lib.lib()
+        dev = torch.device('cuda', torch.cuda.current_device())
+        if state.device != dev:
+            state.to(device=dev)
+        state.sync_to_device()
+        if self._ctx is None:
+            self._ctx = _lib.Context(dev.index)
+        buf = state._q.buf
+        if self._qalt is None or self._qalt.shape != buf.shape or self._qalt.dtype != buf.dtype \
+                or self._qalt.device != buf.device:
+            self._qalt = torch.zeros_like(buf)
+        if self._dt_dev is None:
+            self._dt_dev = torch.zeros((), dtype=torch.float64, device=dev)
+            self._cfl_dev = torch.zeros((), dtype=torch.float64, device=dev)
+        self._static_filled = set()
+
+    def _grid_struct(self, state):
+        g = _lib.ClawGrid()
+        g.mx, g.my = state.patch.num_cells_global
+        g.ng = NG_STORE
+        buf = state._q.buf
+        g.ldq = buf.shape[2]
+        g.plane = buf.shape[1] * buf.shape[2]
+        g.dx, g.dy = state.grid.delta
+        return g
+
+    def _kernel_bcs(self, state):
+        """Boundary codes for the kernel; fills ghost cells on the host where the
+        reference calls a user function."""
+        q_custom = any(b == BC.custom for b in self.bc_lower + self.bc_upper)
+        a_custom = state.num_aux > 0 and any(b == BC.custom for b in self.aux_bc_lower + self.aux_bc_upper)
+        bc = _lib.ClawBC()
+        ng = self.num_ghost
+        if q_custom:
+            if self.custom_bc_static:
+                if any(b == BC.custom for b in (self.bc_lower[1], self.bc_upper[1])):
+                    raise ValueError("custom_bc_static supports custom x boundaries only")
+                key = id(state._q.buf)
+                if key not in self._static_filled:
+                    self._fill_custom_sides(state, state._q.buf)
+                    self._fill_custom_sides(state, self._qalt)
+                    self._static_filled.add(key)
+                    self._static_filled.add(id(self._qalt))
+                for s in range(2):
+                    bc.lower[s] = _lib.BC_STORED if self.bc_lower[s] == BC.custom else self.bc_lower[s]
+                    bc.upper[s] = _lib.BC_STORED if self.bc_upper[s] == BC.custom else self.bc_upper[s]
+            else:
+                self._fill_all_q(state)
+                for s in range(2):
+                    bc.lower[s] = bc.upper[s] = _lib.BC_STORED
+        else:
+            for s in range(2):
+                if self.bc_lower[s] is None or self.bc_upper[s] is None:
+                    raise Exception("One or more of the boundary conditions has not been specified.")
+                bc.lower[s], bc.upper[s] = int(self.bc_lower[s]), int(self.bc_upper[s])
+        if state.num_aux > 0:
+            if a_custom:
+                field = state._aux.padded()[:, NG_STORE - ng:field_end(state, 0, ng), NG_STORE - ng:field_end(state, 1, ng)]
+                fill_ghosts(field, ng, self.aux_bc_lower, self.aux_bc_upper, False, state, state.t,
+                            self.user_aux_bc_lower, self.user_aux_bc_upper)
+                for s in range(2):
+                    bc.aux_lower[s] = bc.aux_upper[s] = _lib.BC_STORED
+            else:
+                for s in range(2):
+                    if self.aux_bc_lower[s] is None or self.aux_bc_upper[s] is None:
+                        raise Exception("One or more of the aux boundary conditions has not been specified.")
+                    bc.aux_lower[s], bc.aux_upper[s] = int(self.aux_bc_lower[s]), int(self.aux_bc_upper[s])
+        else:
+            for s in range(2):
+                bc.aux_lower[s] = bc.aux_upper[s] = _lib.BC_EXTRAP
+        return bc
+
+    def _qbc_view(self, state, buf):
+        ng = self.num_ghost
+        return state._q.padded(buf)[:, NG_STORE - ng:field_end(state, 0, ng), NG_STORE - ng:field_end(state, 1, ng)]
+
+    def _fill_all_q(self, state):
+        fill_ghosts(self._qbc_view(state, state._q.buf), self.num_ghost, self.bc_lower, self.bc_upper, True,
+                    state, state.t, self.user_bc_lower, self.user_bc_upper)
+
+    def _fill_custom_sides(self, state, buf):
+        view = self._qbc_view(state, buf)
+        for side, fn in ((0, self.user_bc_lower), (1, self.user_bc_upper)):
+            if (self.bc_lower if side == 0 else self.bc_upper)[0] == BC.custom:
+                fn(state, state.patch.dimensions[0], state.t, view, self.num_ghost)
+
+    def _swap_in(self, state):
+        """Make the step's output buffer current; keep the old one as backup."""
+        old = state._q.buf
+        state._q.buf = self._qalt
+        self._qalt = old
+        state._device_updated()
+
+    # ---- time stepping ------------------------------------------------------------
+    def evolve_to_time(self, solution, tend=None):
+        """solver.py:582-699."""
+        if not self._is_set_up:
+            self.setup(solution)
+        take_one_step = tend is None
+        tstart = solution.t
+        self.status['cflmax'] = self.cfl.get_cached_max()
+        self.status['dtmin'] = self.dt
+        self.status['dtmax'] = self.dt
+        self.status['numsteps'] = 0
+        if not self.dt_variable:
+            if take_one_step:
+                self.max_steps = 1
+            else:
+                self.max_steps = int((tend - tstart + 1e-10) / self.dt)
+                if abs(self.max_steps * self.dt - (tend - tstart)) > 1e-5 * (tend - tstart):
+                    raise Exception('dt does not divide (tend-tstart) and dt is fixed!')
+        if self.dt_variable == 1 and self.cfl_desired > self.cfl_max:
+            raise Exception('Variable time-stepping and desired CFL > maximum CFL')
+        if not take_one_step and tend <= tstart:
+            self.logger.info("Already at or beyond end time: no evolution required.")
+            self.max_steps = 0
+
+        for n in range(self.max_steps):
+            state = solution.state
+            if not take_one_step:
+                if solution.t + self.dt > tend and tstart < tend:
+                    self.dt = tend - solution.t
+                if tend - solution.t - self.dt < 1.e-14 * solution.t:
+                    self.dt = tend - solution.t
+            if self.dt_variable:
+                backup = self._take_backup(state)
+                told = solution.t
+            self.step(solution)
+            cfl = self.cfl.get_cached_max()
+            if cfl <= self.cfl_max:
+                self.status['cflmax'] = max(cfl, self.status['cflmax'])
+                if self.dt_variable:
+                    solution.t += self.dt
+                else:
+                    solution.t = tstart + (n + 1) * self.dt
+                self.logger.debug("Step %i  CFL = %f   dt = %f   t = %f" % (n, cfl, self.dt, solution.t))
+                self.write_gauge_values(solution)
+                self.status['numsteps'] += 1
+            else:
+                self.logger.debug("Rejecting time step, CFL number too large")
+                if self.dt_variable:
+                    self._restore_backup(state, backup)
+                    solution.t = told
+                else:
+                    self.status['cflmax'] = max(cfl, self.status['cflmax'])
+                    raise Exception('CFL too large, giving up!')
+            if self.dt_variable:
+                if cfl > 0.0:
+                    self.dt = min(self.dt_max, self.dt * self.cfl_desired / cfl)
+                    self.status['dtmin'] = min(self.dt, self.status['dtmin'])
+                    self.status['dtmax'] = max(self.dt, self.status['dtmax'])
+                else:
+                    self.dt = self.dt_max
+            if solution.t >= tend if tend is not None else False:
+                break
+            if take_one_step:
+                break
+        if self.dt_variable and tend is not None and solution.t < tend \
+                and self.status['numsteps'] == self.max_steps:
+            raise Exception("Maximum number of timesteps have been taken")
+        return self.status
+
+    def _hooks_active(self):
+        return False
+
+    def _take_backup(self, state):
+        """Keep q as it was before the step.  Without user hooks the step's input
+        buffer is untouched by the kernel, so holding a reference suffices."""
+        state.sync_to_device()
+        if self._hooks_active():
+            return ('copy', state._q.buf.clone())
+        return ('buf', state._q.buf)
+
+    def _restore_backup(self, state, backup):
+        kind, buf = backup
+        if kind == 'buf':
+            if state._q.buf is not buf:
+                self._qalt = state._q.buf
+                state._q.buf = buf
+        else:
+            state._q.buf.copy_(buf)
+        state._device_updated()
+
+    def step(self, solution):
+        raise NotImplementedError("No stepping routine has been defined!")
+
+    # ---- gauges (solver.py:713-742) ------------------------------------------------------
+    def write_gauge_values(self, solution):
+        gauges = solution.state.grid.gauges
+        if not gauges:
+            return
+        state = solution.state
+        idx = torch.tensor(gauges, dtype=torch.long)
+        q = state._q.interior()
+        qg = q[:, idx[:, 0].to(q.device), idx[:, 1].to(q.device)].to('cpu', torch.float64).numpy()
+        aux = None
+        if state._aux is not None:
+            a = state._aux.interior()
+            aux = a[:, idx[:, 0].to(a.device), idx[:, 1].to(a.device)].to('cpu', torch.float64).numpy()
+        t = solution.t
+        for i in range(len(gauges)):
+            p = self.compute_gauge_values(qg[:, i], None if aux is None else aux[:, i])
+            if state.keep_gauges:
+                gd = state.gauge_data
+                if len(gd) == len(gauges):
+                    gd[i] = np.vstack((gd[i], np.append(t, p)))
+                else:
+                    gd.append(np.append(t, p))
+            files = state.grid.gauge_files
+            if i < len(files):
+                files[i].write(str(t) + ' ' + ' '.join(str(j) for j in p) + '\n')
+
+
+def field_end(state, idim, ng):
+    n = state.patch.num_cells_global[idim]
+    return NG_STORE + n + ng

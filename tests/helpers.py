@@ -1,0 +1,125 @@
+"""Shared test helpers: seeded inputs, and calling the C ABI with torch buffers."""
+import ctypes
+
+import numpy as np
+
+import oracle as O
+
+NG = 3  # ghost layers in the device layout used by the tests
+
+
+def euler_state(rng, nx, ny, amp=0.5):
+    rho = 1 + amp * rng.random((nx, ny))
+    u = rng.standard_normal((nx, ny)) * amp
+    v = rng.standard_normal((nx, ny)) * amp
+    p = 1 + rng.random((nx, ny))
+    q = np.zeros((5, nx, ny))
+    q[0] = rho; q[1] = rho * u; q[2] = rho * v
+    q[3] = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    q[4] = rng.random((nx, ny))
+    return q
+
+
+def shallow_state(rng, nx, ny):
+    h = 1 + 0.5 * rng.random((nx, ny))
+    return np.stack([h, 0.3 * rng.standard_normal((nx, ny)), 0.3 * rng.standard_normal((nx, ny))])
+
+
+def acoustic_state(rng, nx, ny):
+    return rng.standard_normal((3, nx, ny))
+
+
+def vc_aux(rng, nx, ny):
+    return np.stack([1 + 3 * rng.random((nx, ny)), 0.5 + 1.5 * rng.random((nx, ny))])
+
+
+RP_CASES = {
+    'acoustics': dict(rp=O.RP_ACOUSTICS, nw=2, naux=0, pd=dict(rho=1.0, bulk=4.0, cc=2.0, zz=2.0),
+                      make=acoustic_state, dt=0.02),
+    'vc_acoustics': dict(rp=O.RP_VC_ACOUSTICS, nw=2, naux=2, pd={}, make=acoustic_state, dt=0.02),
+    'euler': dict(rp=O.RP_EULER, nw=5, naux=0, pd=dict(gamma=1.4, gamma1=0.4), make=euler_state, dt=0.01),
+    'shallow': dict(rp=O.RP_SHALLOW, nw=3, naux=0, pd=dict(grav=1.0), make=shallow_state, dt=0.02),
+}
+
+
+def padded_with_bc(interior, ng, bc_lower, bc_upper, negate, ghost_rng=None):
+    """Reference padded array (ncomp, mx+2ng, my+2ng) F-order; standard codes are
+    filled like the reference; code 0 ('custom') gets arbitrary ghost values."""
+    ne, mx, my = interior.shape
+    qbc = np.zeros((ne, mx + 2 * ng, my + 2 * ng), order='F')
+    qbc[:, ng:-ng, ng:-ng] = interior
+    if ghost_rng is not None:
+        mask = np.ones(qbc.shape[1:], bool)
+        mask[ng:-ng, ng:-ng] = False
+        qbc[:, mask] = interior.mean() + 0.1 * ghost_rng.random((ne, mask.sum()))
+    # custom sides: the stored values stay; other sides are filled in reference order
+    O.fill_bc(qbc, ng, [c if c != 0 else 99 for c in bc_lower], [c if c != 0 else 99 for c in bc_upper],
+              negate)
+    return qbc
+
+
+def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False, mthlim=None,
+                     bc_lower=(1, 1), bc_upper=(1, 1), abc_lower=(1, 1), abc_upper=(1, 1),
+                     dx=0.1, dy=0.07, precision='f64'):
+    """Run claw2d_classic_step on the GPU.  qbc_ref/auxbc_ref are reference-layout
+    padded arrays (ghost width 2 for the classic step); they are copied into the
+    device layout with NG ghost layers (the outermost layer is padding)."""
+    import torch
+    from pyclaw_amd import _lib
+    ne, nxp, nyp = qbc_ref.shape
+    g = (nxp - (nxp - 4)) // 2
+    mx, my = nxp - 2 * g, nyp - 2 * g
+    dtype = torch.float64 if precision == 'f64' else torch.float32
+    dev = torch.device('cuda', 0)
+
+    def to_dev(a):
+        buf = torch.zeros((a.shape[0], my + 2 * NG, mx + 2 * NG), dtype=dtype)
+        off = NG - g
+        buf[:, off:off + nyp, off:off + nxp] = torch.from_numpy(np.ascontiguousarray(a.transpose(0, 2, 1)))
+        return buf.to(dev)
+
+    qd = to_dev(qbc_ref)
+    qn = torch.zeros_like(qd)
+    ad = to_dev(auxbc_ref) if auxbc_ref is not None else None
+    grid = _lib.ClawGrid(mx, my, NG, 0, mx + 2 * NG, (mx + 2 * NG) * (my + 2 * NG), dx, dy)
+    m = _lib.ClawMethod()
+    m.order, m.transverse, m.split, m.fwave, m.num_waves = order, trans, int(split), 0, case['nw']
+    lim = mthlim if mthlim is not None else [4] * case['nw']
+    for k, v in enumerate(lim):
+        m.mthlim[k] = v
+    r = _lib.ClawRiemann()
+    r.kind = case['rp']
+    r.efix = 1
+    vals = {O.RP_ACOUSTICS: ['rho', 'bulk', 'cc', 'zz'], O.RP_EULER: ['gamma', 'gamma1'],
+            O.RP_SHALLOW: ['grav'], O.RP_VC_ACOUSTICS: []}[case['rp']]
+    for k, key in enumerate(vals):
+        r.p[k] = case['pd'][key]
+    bc = _lib.ClawBC()
+    for s in range(2):
+        bc.lower[s], bc.upper[s] = bc_lower[s], bc_upper[s]
+        bc.aux_lower[s], bc.aux_upper[s] = abc_lower[s], abc_upper[s]
+    dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
+    cfl = torch.zeros((), dtype=torch.float64, device=dev)
+    ctx = _lib.Context(0)
+    rc = _lib.lib().claw2d_classic_step(ctx.handle, grid, m, r, bc, _lib.F32 if precision == 'f32' else _lib.F64,
+                                        qd.data_ptr(), qn.data_ptr(), ad.data_ptr() if ad is not None else None,
+                                        dtd.data_ptr(), cfl.data_ptr(),
+                                        torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc, 'claw2d_classic_step')
+    torch.cuda.synchronize()
+    out = qn[:, NG:NG + my, NG:NG + mx].permute(0, 2, 1).to('cpu', torch.float64).numpy()
+    return out, float(cfl.item())
+
+
+def oracle_classic_step(qbc, auxbc, case, dt, order=2, trans=1, split=False, mthlim=None, dx=0.1, dy=0.07):
+    O.set_params(case['rp'], case['pd'])
+    meth = O.method_array(order, trans, split)
+    lim = O.mthlim_array(mthlim if mthlim is not None else 4, case['nw'])
+    if not split:
+        qn, cfl = O.step2(case['rp'], qbc, auxbc, dx, dy, dt, meth, lim, case['nw'])
+        return qn[:, 2:-2, 2:-2], cfl
+    qold = np.asfortranarray(qbc)
+    qn = qold.copy(order='F')
+    c1 = O.step2ds(case['rp'], qold, qn, auxbc, dx, dy, dt, meth, lim, case['nw'], 1)
+    c2 = O.step2ds(case['rp'], qn, qn, auxbc, dx, dy, dt, meth, lim, case['nw'], 2)
+    return qn[:, 2:-2, 2:-2], max(c1, c2)
