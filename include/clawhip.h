@@ -109,6 +109,42 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
 /* Number of kernel launches one classic step issues (for timing). */
 int claw2d_classic_launches(const ClawMethod *method);
 
+/* ---- SharpClaw (WENO5 component-wise, lim_type=2, char_decomp=0) ---------- */
+
+/* dq = dt * dq/dt over the interior, with ng >= 3 ghost layers.  Replaces
+   sharpclaw2.flux2 (src/pyclaw/sharpclaw/flux2.f90:2-101, called from
+   SharpClawSolver2D.dq_hyperbolic, sharpclaw/solver.py:535-582).  dq uses the
+   same padded layout as q; its ghost cells are not written.  *cfl_dev is
+   overwritten with the Courant number of the evaluation. */
+int claw2d_sharpclaw_dq(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                        int precision, const void *q, void *dq, const void *aux,
+                        const double *dt_dev, double *cfl_dev, void *stream);
+
+/* One Runge-Kutta stage: evaluates dq on qin and writes the stage
+   combination (sharpclaw/solver.py:201-251) into out:
+     0 out = dq                    1 out = qin + dq   (Euler; SSP33 stage 1)
+     2 out = qin + dq/6.           3 t = qin + dq/6.; s2 = q0/25. + (9./25)*t; out = 15.*s2 - 5.*t
+     4 out = (s2 + 0.6*qin) + 0.1*dq
+     5 out = 0.75*q0 + 0.25*(qin + dq)        6 out = 1./3.*q0 + 2./3.*(qin + dq)
+   q0 is the step's initial state (modes 3, 5, 6); s2 is written by mode 3
+   and read by mode 4.  *cfl_dev is overwritten. */
+int claw2d_sharpclaw_stage(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                           int precision, int mode, const void *qin, const void *q0, void *s2,
+                           void *out, const void *aux, const double *dt_dev, double *cfl_dev,
+                           void *stream);
+
+/* A whole SSP104 step (10 stages, sharpclaw/solver.py:221-251) from q into
+   qnew, with s1a/s1b/s2 as stage registers (all five buffers distinct).
+   cfl10_dev receives the 10 stages' Courant numbers; the caller applies the
+   reference's rule (the first stage above cfl_max aborts the step, else the
+   last stage's value is the step's).  Ghost codes CLAW_BC_STORED are read
+   from each stage register, so callers with user boundary functions fill
+   them per stage with claw2d_sharpclaw_stage instead. */
+int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiemann *rp,
+                                 const ClawBC *bc, int precision, const void *q, void *qnew,
+                                 void *s1a, void *s1b, void *s2, const void *aux,
+                                 const double *dt_dev, double *cfl10_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,7 @@ def lib():
         L.orc_philim.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_rpn2.argtypes = [ctypes.c_int] * 7 + [_dp] * 8 + [ctypes.c_int]
         L.orc_rpt2.argtypes = [ctypes.c_int] * 7 + [_dp] * 5 + [ctypes.c_int] + [_dp] * 3 + [ctypes.c_int]
+        L.orc_weno5.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_rp_pointer.restype = ctypes.c_void_p
         L.orc_rp_pointer.argtypes = [ctypes.c_int, ctypes.c_int]
         _lib = L
@@ -161,6 +162,46 @@ def sharp_dq(rp, qbc, auxbc, dx, dy, dt, num_waves):
 
 def philim(a, b, meth):
     return lib().orc_philim(a, b, meth)
+
+
+def weno5(q):
+    """weno.f90:5-83 on a slice q (meqn, n): returns (ql, qr) with left/right
+    edge values for cells 2..n-3 (0-based); other entries are zero."""
+    q = np.asfortranarray(q, dtype=np.float64)
+    ne, n = q.shape
+    ql = np.zeros_like(q, order='F')
+    qr = np.zeros_like(q, order='F')
+    lib().orc_weno5(_ptr(q), _ptr(ql), _ptr(qr), ne, n)
+    return ql, qr
+
+
+def rpn2(rp, ixy, qlo, qhi, auxlo=None, auxhi=None, num_waves=None):
+    """Riemann solve at single interfaces: qlo/qhi (meqn, k) are the states left
+    and right of k interfaces.  Returns (wave (meqn, mw, k), s (mw, k), amdq, apdq)."""
+    qlo = np.asarray(qlo, dtype=np.float64); qhi = np.asarray(qhi, dtype=np.float64)
+    ne, k = qlo.shape
+    nw = num_waves
+    mbc = 1
+    # slice of length 2k: cells (2j, 2j+1) hold (lo, hi) of interface j; rpn2 computes
+    # interfaces between consecutive cells, the odd ones are the wanted ones.
+    n = 2 * k
+    q = np.zeros((ne, n + 2 * mbc), order='F')
+    q[:, mbc:mbc + n:2] = qlo
+    q[:, mbc + 1:mbc + n:2] = qhi
+    na = 2
+    a = np.ones((na, n + 2 * mbc), order='F')
+    if auxlo is not None:
+        a[:, mbc:mbc + n:2] = auxlo
+        a[:, mbc + 1:mbc + n:2] = auxhi
+    wave = np.zeros((ne, nw, n + 2 * mbc), order='F')
+    s = np.zeros((nw, n + 2 * mbc), order='F')
+    amdq = np.zeros((ne, n + 2 * mbc), order='F')
+    apdq = np.zeros((ne, n + 2 * mbc), order='F')
+    lib().orc_rpn2(rp, ixy, n, ne, nw, mbc, n, _ptr(q), _ptr(q), _ptr(a), _ptr(a), _ptr(wave), _ptr(s),
+                   _ptr(amdq), _ptr(apdq), na)
+    # interface between slice cells c-1 and c lives at Fortran index c (array index c-1+mbc)
+    idx = np.arange(k) * 2 + 1 + mbc
+    return wave[:, :, idx], s[:, idx], amdq[:, idx], apdq[:, idx]
 
 
 # ---------------------------------------------------------------------------

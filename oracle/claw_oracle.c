@@ -1074,3 +1074,9 @@ double orc_sharp_dq(int rp, int meqn, int mwaves, int num_aux, int mbc, int mx, 
     free(q1d); free(dq1d); free(a1d);
     return cfl;
 }
+
+/* WENO5 reconstruction over a slice q(meqn, 1:n) for k = 3..n-2 (unit tests). */
+void orc_weno5(const double *q, double *ql, double *qr, int meqn, int n)
+{
+    weno5_slice(q, ql, qr, meqn, 3, n - 2);
+}

@@ -14,6 +14,7 @@ RP_ACOUSTICS, RP_VC_ACOUSTICS, RP_EULER, RP_SHALLOW = 0, 1, 2, 3
 BC_STORED, BC_EXTRAP, BC_PERIODIC, BC_WALL = 0, 1, 2, 3
 F64, F32 = 0, 1
 STEP_ACCUMULATE_CFL = 1
+SM_DQ, SM_EULER, SM_SSP104_A, SM_SSP104_MID, SM_SSP104_END, SM_SSP33_2, SM_SSP33_3 = range(7)
 
 _ERRORS = {
     CLAW_EINVAL: "invalid argument or unsupported option",
@@ -86,6 +87,9 @@ def _optional_symbols():
         "claw2d_sharpclaw_stage": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann),
                                    ctypes.POINTER(ClawBC), ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp,
                                    vp, vp],
+        "claw2d_sharpclaw_ssp104_step": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann),
+                                         ctypes.POINTER(ClawBC), ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, vp,
+                                         vp],
     }
 
 

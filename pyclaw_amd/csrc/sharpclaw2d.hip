@@ -1,0 +1,400 @@
+// SharpClaw 2D: fused WENO5 reconstruction + Riemann solves + Runge-Kutta
+// stage update, for gfx950.
+//
+// Replaces the reference's dq evaluation (src/pyclaw/sharpclaw/flux2.f90:2-101
+// -> flux1_2.f90:2-196 -> reconstruct.f90:85-116 -> weno.f90:5-83) and the
+// numpy stage combinations of SSP104 / SSP33 / Euler
+// (src/pyclaw/sharpclaw/solver.py:185-255).  One launch evaluates
+//   dq(m,i,j) = (0 + dq1d_x(m,i)) + dq1d_y(m,j)
+//   dq1d(m,i) = 0 - dtdx*(((amdq(i+1) + apdq(i)) + amdq2(i)) + apdq2(i))
+// (flux1_2.f90:190-193, flux2.f90:59-61, 91-93) and combines it with the
+// stage registers in the kernel epilogue, so the only HBM traffic is the
+// stage's inputs and outputs.
+//
+// Work decomposition: one wavefront per strip of 64 columns (58 produced:
+// WENO needs two cells on each side, the interface fluctuations one more),
+// marching up a band of rows with a five-row register window for the
+// y-direction reconstruction.  x-neighbours come from other lanes
+// (ds_bpermute), y-neighbours from the lane's own registers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+
+#include "common.hpp"
+#include "riemann.hpp"
+
+namespace clawhip {
+
+constexpr int SX_USEFUL = WAVE - 6;
+constexpr int SNG = 3;   // SharpClaw ghost width ((weno_order+1)/2, sharpclaw/solver.py:159)
+
+// Stage epilogues (sharpclaw/solver.py:201-251); q0 is the step's initial state.
+enum StageMode {
+    SM_DQ = 0,          // out = dq
+    SM_EULER = 1,       // out = in + dq
+    SM_SSP104_A = 2,    // out = in + dq/6.
+    SM_SSP104_MID = 3,  // t = in + dq/6.; s2 = q0/25. + (9./25)*t; out = 15.*s2 - 5.*t
+    SM_SSP104_END = 4,  // out = (s2 + 0.6*in) + 0.1*dq
+    SM_SSP33_2 = 5,     // out = 0.75*q0 + 0.25*(in + dq)
+    SM_SSP33_3 = 6      // out = 1./3.*q0 + 2./3.*(in + dq)
+};
+
+// weno.f90:32-79 for one component: left (ql) and right (qr) edge values of
+// the cell whose average is q0, from the averages qm2..qp2.
+template <typename R>
+__device__ __forceinline__ void weno5(R qm2, R qm1, R q0, R qp1, R qp2, R &ql, R &qr) {
+    const R eps = R((double)1.0e-36f);
+    R sigma0 = R(3.3333333333333333333333333333333333) * (q0 * q0)
+        - R(10.333333333333333333333333333333333) * q0 * qp1
+        + R(3.6666666666666666666666666666666667) * q0 * qp2
+        + R(8.3333333333333333333333333333333333) * (qp1 * qp1)
+        - R(6.3333333333333333333333333333333333) * qp1 * qp2
+        + R(1.3333333333333333333333333333333333) * (qp2 * qp2);
+    R sigma1 = R(4.3333333333333333333333333333333333) * (q0 * q0)
+        - R(4.3333333333333333333333333333333333) * q0 * qp1
+        - R(4.3333333333333333333333333333333333) * q0 * qm1
+        + R(1.3333333333333333333333333333333333) * (qp1 * qp1)
+        + R(1.6666666666666666666666666666666667) * qp1 * qm1
+        + R(1.3333333333333333333333333333333333) * (qm1 * qm1);
+    R sigma2 = R(3.3333333333333333333333333333333333) * (q0 * q0)
+        - R(10.333333333333333333333333333333333) * q0 * qm1
+        + R(3.6666666666666666666666666666666667) * q0 * qm2
+        + R(8.3333333333333333333333333333333333) * (qm1 * qm1)
+        - R(6.3333333333333333333333333333333333) * qm1 * qm2
+        + R(1.3333333333333333333333333333333333) * (qm2 * qm2);
+    R t0 = eps + sigma0, t1 = eps + sigma1, t2 = eps + sigma2;
+    R t00 = t0 * t0, t11 = t1 * t1, t22 = t2 * t2;
+    R omega0 = R(0.1) / t00, omega1 = R(0.6) / t11, omega2 = R(0.3) / t22;
+    R omega3 = R(0.3) / t00, omega4 = R(0.6) / t11, omega5 = R(0.1) / t22;
+    R fr0 = R(1.8333333333333333333333333333333333) * q0 - R(1.1666666666666666666666666666666667) * qp1
+        + R(0.33333333333333333333333333333333333) * qp2;
+    R fr1 = R(0.83333333333333333333333333333333333) * q0 - R(0.16666666666666666666666666666666667) * qp1
+        + R(0.33333333333333333333333333333333333) * qm1;
+    R fr2 = R(0.33333333333333333333333333333333333) * q0 + R(0.83333333333333333333333333333333333) * qm1
+        - R(0.16666666666666666666666666666666667) * qm2;
+    R fr3 = R(0.33333333333333333333333333333333333) * q0 + R(0.83333333333333333333333333333333333) * qp1
+        - R(0.16666666666666666666666666666666667) * qp2;
+    R fr4 = R(0.83333333333333333333333333333333333) * q0 + R(0.33333333333333333333333333333333333) * qp1
+        - R(0.16666666666666666666666666666666667) * qm1;
+    R fr5 = R(1.8333333333333333333333333333333333) * q0 - R(1.1666666666666666666666666666666667) * qm1
+        + R(0.33333333333333333333333333333333333) * qm2;
+    ql = (fr0 * omega0 + fr1 * omega1 + fr2 * omega2) / (omega0 + omega1 + omega2);
+    qr = (fr3 * omega3 + fr4 * omega4 + fr5 * omega5) / (omega3 + omega4 + omega5);
+}
+
+template <class RP, typename R>
+struct SharpParams {
+    const R *qin;     // state dq is evaluated on
+    const R *q0;      // step's initial state (SSP104_MID, SSP33)
+    R *s2;            // SSP104 second register (written by MID, read by END)
+    R *out;           // stage output
+    const R *aux;
+    GridDev g;
+    BCDev bq, ba;
+    typename RP::Params P;
+    const double *dt;
+    double *cfl;      // this stage's Courant number (max-combined)
+    double dx, dy;
+    int mode;
+    int H, nsx, nsy, nsyb, wpb;
+};
+
+__device__ __forceinline__ int xcd_block_s() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, xcd = b % 8, loc = b / 8;
+    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void sharp_stage_kernel(const SharpParams<RP, R> p) {
+    constexpr int NE = RP::NE, NW = RP::NW;
+    constexpr bool HAS_AUX = RP::NA > 0;
+    constexpr int NA = HAS_AUX ? RP::NA : 1;
+    using Cache = typename RP::Cache;
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int vb = xcd_block_s();
+    const int sx = vb / p.nsyb;
+    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+    if (sx >= p.nsx || sy >= p.nsy) return;
+
+    const GridDev g = p.g;
+    const int c0 = sx * SX_USEFUL;
+    const int i = c0 - 3 + lane;
+    const int R0 = sy * p.H;
+    const int R1 = min(R0 + p.H, g.my);
+    const int up1 = ((lane + WAVE - 1) & (WAVE - 1)) << 2;
+    const int up2 = ((lane + WAVE - 2) & (WAVE - 1)) << 2;
+    const int dn1 = ((lane + 1) & (WAVE - 1)) << 2;
+    const int dn2 = ((lane + 2) & (WAVE - 1)) << 2;
+    const bool produce = lane >= 3 && lane < 3 + SX_USEFUL && i < g.mx;
+
+    const double dtd = *p.dt;
+    const R dtdx = R(dtd / p.dx), dtdy = R(dtd / p.dy);
+    const auto &P = p.P;
+    const int mode = p.mode;
+    R cfl = R(0);
+
+    // five-row window of q (rows w-2..w+2) and aux (rows w-1, w)
+    R win[5][NE];
+    R aw[5][NA];
+    const int wlast = R1 + (R1 == g.my ? 1 : 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        load_q<NE>(p.qin, g, p.bq, i, R0 - 3 + k, win[k + 1]);
+        if constexpr (HAS_AUX) load_aux<NA>(p.aux, g, p.ba, i, R0 - 3 + k, aw[k + 1]);
+    }
+    R qrY_prev[NE], apY_prev[NE], am2Y_prev[NE], ap2Y_prev[NE];
+#pragma unroll
+    for (int m = 0; m < NE; ++m) { qrY_prev[m] = apY_prev[m] = am2Y_prev[m] = ap2Y_prev[m] = R(0); }
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) if (!HAS_AUX) aw[k][a] = R(1);
+
+    for (int w = R0 - 1; w <= wlast; ++w) {
+        // shift the window and load row w+2
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int m = 0; m < NE; ++m) win[k][m] = win[k + 1][m];
+#pragma unroll
+            for (int a = 0; a < NA; ++a) aw[k][a] = aw[k + 1][a];
+        }
+        load_q<NE>(p.qin, g, p.bq, i, w + 2, win[4]);
+        if constexpr (HAS_AUX) load_aux<NA>(p.aux, g, p.ba, i, w + 2, aw[4]);
+        // win[k] = row w-2+k
+
+        // ---- y reconstruction at (i, w) and the y-interface (i, w-1/2) --------
+        R qlY[NE], qrY[NE];
+#pragma unroll
+        for (int m = 0; m < NE; ++m) weno5<R>(win[0][m], win[1][m], win[2][m], win[3][m], win[4][m], qlY[m], qrY[m]);
+        R amY[NE], apY[NE], am2Y[NE], ap2Y[NE];
+        if (w >= R0) {
+            R wave[NW][NE], s[NW];
+            Cache c;
+            RP::template rpn<1>(P, qrY_prev, qlY, aw[1], aw[2], wave, s, amY, apY, c);
+            if (w <= g.my && i >= -1 && i <= g.mx) {
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdy * s[mw]); cfl = rmax2<R>(cfl, -dtdy * s[mw]); }
+            }
+            RP::template rpn<1>(P, qlY, qrY, aw[2], aw[2], wave, s, am2Y, ap2Y, c);
+        }
+
+        // ---- output row o = w-1: x part and combination --------------------------
+        const int o = w - 1;
+        const bool out_row = (o >= R0 && o < R1);
+        const bool cfl_row = (o == -1 && R0 == 0) || (o == g.my && R1 == g.my);
+        if (out_row || cfl_row) {
+            // row o is win[1]; WENO in x at (i, o)
+            R xm2[NE], xm1[NE], xp1[NE], xp2[NE], qlX[NE], qrX[NE], qrXl[NE];
+#pragma unroll
+            for (int m = 0; m < NE; ++m) {
+                xm2[m] = lane_get(win[1][m], up2); xm1[m] = lane_get(win[1][m], up1);
+                xp1[m] = lane_get(win[1][m], dn1); xp2[m] = lane_get(win[1][m], dn2);
+                weno5<R>(xm2[m], xm1[m], win[1][m], xp1[m], xp2[m], qlX[m], qrX[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < NE; ++m) qrXl[m] = lane_get(qrX[m], up1);
+            R alx[NA];
+            if constexpr (HAS_AUX) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a) alx[a] = lane_get(aw[1][a], up1);
+            }
+            R amX[NE], apX[NE], am2X[NE], ap2X[NE];
+            {
+                R wave[NW][NE], s[NW];
+                Cache c;
+                RP::template rpn<0>(P, qrXl, qlX, alx, aw[1], wave, s, amX, apX, c);
+                if (i >= 0 && i <= g.mx && lane >= 3 && lane <= WAVE - 3) {
+#pragma unroll
+                    for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdx * s[mw]); cfl = rmax2<R>(cfl, -dtdx * s[mw]); }
+                }
+                RP::template rpn<0>(P, qlX, qrX, aw[1], aw[1], wave, s, am2X, ap2X, c);
+            }
+            if (out_row) {
+                const long long off = (long long)(o + g.ng) * g.ldq + (i + g.ng);
+#pragma unroll
+                for (int m = 0; m < NE; ++m) {
+                    R amXr = lane_get(amX[m], dn1);
+                    R dqx = R(0) - dtdx * (((amXr + apX[m]) + am2X[m]) + ap2X[m]);
+                    R dqy = R(0) - dtdy * (((amY[m] + apY_prev[m]) + am2Y_prev[m]) + ap2Y_prev[m]);
+                    R dq = (R(0) + dqx) + dqy;
+                    if (produce) {
+                        const long long at = m * g.plane + off;
+                        R v;
+                        switch (mode) {
+                        case SM_DQ: v = dq; break;
+                        case SM_EULER: v = p.qin[at] + dq; break;
+                        case SM_SSP104_A: v = p.qin[at] + dq / R(6); break;
+                        case SM_SSP104_MID: {
+                            R t = p.qin[at] + dq / R(6);
+                            R s2v = p.q0[at] / R(25) + R(9. / 25) * t;
+                            p.s2[at] = s2v;
+                            v = R(15) * s2v - R(5) * t;
+                            break;
+                        }
+                        case SM_SSP104_END: v = (p.s2[at] + R(0.6) * p.qin[at]) + R(0.1) * dq; break;
+                        case SM_SSP33_2: v = R(0.75) * p.q0[at] + R(0.25) * (p.qin[at] + dq); break;
+                        default: v = R(1. / 3.) * p.q0[at] + R(2. / 3.) * (p.qin[at] + dq); break;
+                        }
+                        p.out[at] = v;
+                    }
+                }
+            }
+        }
+        // ---- carry the y-interface data to the next row ---------------------------
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            qrY_prev[m] = qrY[m];
+            if (w >= R0) { apY_prev[m] = apY[m]; am2Y_prev[m] = am2Y[m]; ap2Y_prev[m] = ap2Y[m]; }
+        }
+    }
+    double c = wave_max((double)cfl);
+    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+}
+
+__global__ void zero_slot_kernel(double *p) { *p = 0.0; }
+
+template <class RP, typename R>
+static typename RP::Params make_sharp_params(const ClawRiemann *rp);
+template <> Acoustics<double>::Params make_sharp_params<Acoustics<double>, double>(const ClawRiemann *rp) {
+    return {rp->p[0], rp->p[1], rp->p[2], rp->p[3]};
+}
+template <> Acoustics<float>::Params make_sharp_params<Acoustics<float>, float>(const ClawRiemann *rp) {
+    return {(float)rp->p[0], (float)rp->p[1], (float)rp->p[2], (float)rp->p[3]};
+}
+template <> VCAcoustics<double>::Params make_sharp_params<VCAcoustics<double>, double>(const ClawRiemann *) { return {0.0}; }
+template <> VCAcoustics<float>::Params make_sharp_params<VCAcoustics<float>, float>(const ClawRiemann *) { return {0.0f}; }
+template <> Euler5<double>::Params make_sharp_params<Euler5<double>, double>(const ClawRiemann *rp) {
+    return {rp->p[0], rp->p[1], rp->efix};
+}
+template <> Euler5<float>::Params make_sharp_params<Euler5<float>, float>(const ClawRiemann *rp) {
+    return {(float)rp->p[0], (float)rp->p[1], rp->efix};
+}
+template <> ShallowRoe<double>::Params make_sharp_params<ShallowRoe<double>, double>(const ClawRiemann *rp) { return {rp->p[0]}; }
+template <> ShallowRoe<float>::Params make_sharp_params<ShallowRoe<float>, float>(const ClawRiemann *rp) { return {(float)rp->p[0]}; }
+
+static int sharp_band_height(int rows, int strips_x) {
+    long long target = 2048;
+    int h = (int)std::max<long long>(16, ((long long)rows * strips_x + target - 1) / target);
+    return std::min(h, std::max(rows, 1));
+}
+
+template <class RP, typename R>
+static int launch_sharp(const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int mode,
+                        const void *qin, const void *q0, void *s2, void *out, const void *aux,
+                        const double *dt, double *cfl, bool zero_cfl, hipStream_t st) {
+    const int wpb = 4;
+    SharpParams<RP, R> p;
+    p.qin = (const R *)qin; p.q0 = (const R *)q0; p.s2 = (R *)s2; p.out = (R *)out; p.aux = (const R *)aux;
+    p.g = GridDev{grid->mx, grid->my, grid->ng, grid->ldq, grid->plane};
+    p.bq = BCDev{{bc->lower[0], bc->lower[1]}, {bc->upper[0], bc->upper[1]}};
+    p.ba = BCDev{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
+    p.P = make_sharp_params<RP, R>(rp);
+    p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy; p.mode = mode;
+    p.nsx = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
+    p.H = sharp_band_height(grid->my, p.nsx);
+    p.nsy = (grid->my + p.H - 1) / p.H;
+    p.wpb = wpb;
+    p.nsyb = (p.nsy + wpb - 1) / wpb;
+    if (zero_cfl) hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, st, cfl);
+    hipLaunchKernelGGL((sharp_stage_kernel<RP, R>), dim3(p.nsx * p.nsyb), dim3(WAVE * wpb), 0, st, p);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+static int sharp_validate(const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int precision,
+                          const void *aux) {
+    if (!grid || !rp || !bc) return CLAW_EINVAL;
+    if (grid->mx < 3 || grid->my < 3 || grid->ng < SNG) return CLAW_EINVAL;
+    if (grid->ldq < grid->mx + 2 * grid->ng || grid->plane < grid->ldq * (grid->my + 2 * grid->ng)) return CLAW_EINVAL;
+    for (int s = 0; s < 2; ++s) {
+        if (bc->lower[s] < 0 || bc->lower[s] > 3 || bc->upper[s] < 0 || bc->upper[s] > 3) return CLAW_EINVAL;
+        if ((bc->lower[s] == CLAW_BC_PERIODIC) != (bc->upper[s] == CLAW_BC_PERIODIC)) return CLAW_EINVAL;
+    }
+    if (precision != CLAW_F64 && precision != CLAW_F32) return CLAW_EINVAL;
+    if (rp->kind == CLAW_RP_VC_ACOUSTICS && !aux) return CLAW_EINVAL;
+    if (rp->kind < 0 || rp->kind > 3) return CLAW_EINVAL;
+    return CLAW_OK;
+}
+
+static int sharp_dispatch(const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int precision, int mode,
+                          const void *qin, const void *q0, void *s2, void *out, const void *aux,
+                          const double *dt, double *cfl, bool zero_cfl, hipStream_t st) {
+    const bool f32 = precision == CLAW_F32;
+#define SDISPATCH(RPT)                                                                                        \
+    return f32 ? launch_sharp<RPT<float>, float>(grid, rp, bc, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st) \
+               : launch_sharp<RPT<double>, double>(grid, rp, bc, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st)
+    switch (rp->kind) {
+    case CLAW_RP_ACOUSTICS: SDISPATCH(Acoustics);
+    case CLAW_RP_VC_ACOUSTICS: SDISPATCH(VCAcoustics);
+    case CLAW_RP_EULER: SDISPATCH(Euler5);
+    default: SDISPATCH(ShallowRoe);
+    }
+#undef SDISPATCH
+}
+
+}  // namespace clawhip
+
+using namespace clawhip;
+
+extern "C" {
+
+int claw2d_sharpclaw_dq(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int precision,
+                        const void *q, void *dq, const void *aux, const double *dt_dev, double *cfl_dev,
+                        void *stream) {
+    (void)ctx;
+    int rc = sharp_validate(grid, rp, bc, precision, aux);
+    if (rc) return rc;
+    if (!q || !dq || q == dq || !dt_dev || !cfl_dev) return CLAW_EINVAL;
+    return sharp_dispatch(grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, true,
+                          (hipStream_t)stream);
+}
+
+int claw2d_sharpclaw_stage(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                           int precision, int mode, const void *qin, const void *q0, void *s2, void *out,
+                           const void *aux, const double *dt_dev, double *cfl_dev, void *stream) {
+    (void)ctx;
+    int rc = sharp_validate(grid, rp, bc, precision, aux);
+    if (rc) return rc;
+    if (mode < SM_DQ || mode > SM_SSP33_3) return CLAW_EINVAL;
+    if (!qin || !out || qin == out || !dt_dev || !cfl_dev) return CLAW_EINVAL;
+    if ((mode == SM_SSP104_MID || mode == SM_SSP33_2 || mode == SM_SSP33_3) && !q0) return CLAW_EINVAL;
+    if ((mode == SM_SSP104_MID || mode == SM_SSP104_END) && !s2) return CLAW_EINVAL;
+    if (mode == SM_SSP104_MID && (s2 == qin || s2 == out)) return CLAW_EINVAL;
+    return sharp_dispatch(grid, rp, bc, precision, mode, qin, q0, s2, out, aux, dt_dev, cfl_dev, true,
+                          (hipStream_t)stream);
+}
+
+int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                                 int precision, const void *q, void *qnew, void *s1a, void *s1b, void *s2,
+                                 const void *aux, const double *dt_dev, double *cfl10_dev, void *stream) {
+    (void)ctx;
+    int rc = sharp_validate(grid, rp, bc, precision, aux);
+    if (rc) return rc;
+    if (!q || !qnew || !s1a || !s1b || !s2 || !dt_dev || !cfl10_dev) return CLAW_EINVAL;
+    const void *bufs[5] = {q, qnew, s1a, s1b, s2};
+    for (int a = 0; a < 5; ++a)
+        for (int b = a + 1; b < 5; ++b)
+            if (bufs[a] == bufs[b]) return CLAW_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    // stage k reads `in` and writes `out`; s1 ping-pongs between s1a and s1b
+    void *s1[2] = {s1a, s1b};
+    int k = 0, cur = 0;
+#define STAGE(MODE, IN, OUT)                                                                                   \
+    do {                                                                                                       \
+        rc = sharp_dispatch(grid, rp, bc, precision, MODE, IN, q, s2, OUT, aux, dt_dev, cfl10_dev + k, true, st); \
+        if (rc) return rc;                                                                                     \
+        ++k;                                                                                                   \
+    } while (0)
+    STAGE(SM_SSP104_A, q, s1[cur]);
+    for (int r = 0; r < 3; ++r) { STAGE(SM_SSP104_A, s1[cur], s1[cur ^ 1]); cur ^= 1; }
+    STAGE(SM_SSP104_MID, s1[cur], s1[cur ^ 1]); cur ^= 1;
+    for (int r = 0; r < 4; ++r) { STAGE(SM_SSP104_A, s1[cur], s1[cur ^ 1]); cur ^= 1; }
+    STAGE(SM_SSP104_END, s1[cur], qnew);
+#undef STAGE
+    return CLAW_OK;
+}
+
+}  // extern "C"

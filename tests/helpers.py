@@ -123,3 +123,90 @@ def oracle_classic_step(qbc, auxbc, case, dt, order=2, trans=1, split=False, mth
     c1 = O.step2ds(case['rp'], qold, qn, auxbc, dx, dy, dt, meth, lim, case['nw'], 1)
     c2 = O.step2ds(case['rp'], qn, qn, auxbc, dx, dy, dt, meth, lim, case['nw'], 2)
     return qn[:, 2:-2, 2:-2], max(c1, c2)
+
+
+def _dev_grid(mx, my, dx, dy):
+    from pyclaw_amd import _lib
+    return _lib.ClawGrid(mx, my, NG, 0, mx + 2 * NG, (mx + 2 * NG) * (my + 2 * NG), dx, dy)
+
+
+def _rp_struct(case):
+    from pyclaw_amd import _lib
+    r = _lib.ClawRiemann()
+    r.kind = case['rp']
+    r.efix = 1
+    vals = {O.RP_ACOUSTICS: ['rho', 'bulk', 'cc', 'zz'], O.RP_EULER: ['gamma', 'gamma1'],
+            O.RP_SHALLOW: ['grav'], O.RP_VC_ACOUSTICS: []}[case['rp']]
+    for k, key in enumerate(vals):
+        r.p[k] = case['pd'][key]
+    return r
+
+
+def _bc_struct(lo, hi, alo=(1, 1), ahi=(1, 1)):
+    from pyclaw_amd import _lib
+    bc = _lib.ClawBC()
+    for s in range(2):
+        bc.lower[s], bc.upper[s] = lo[s], hi[s]
+        bc.aux_lower[s], bc.aux_upper[s] = alo[s], ahi[s]
+    return bc
+
+
+def to_device_padded(a, dtype=None):
+    """Reference padded array (ncomp, mx+2*NG, my+2*NG) -> device planar layout."""
+    import torch
+    dtype = dtype or torch.float64
+    return torch.from_numpy(np.ascontiguousarray(a.transpose(0, 2, 1))).to('cuda', dtype)
+
+
+def from_device_interior(t, mx, my):
+    import torch
+    return t[:, NG:NG + my, NG:NG + mx].permute(0, 2, 1).to('cpu', torch.float64).numpy()
+
+
+def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), abc_upper=(1, 1),
+                 dx=0.1, dy=0.07):
+    """claw2d_sharpclaw_dq on reference-layout padded arrays with 3 ghost layers."""
+    import torch
+    from pyclaw_amd import _lib
+    ne, nxp, nyp = qbc3.shape
+    mx, my = nxp - 2 * NG, nyp - 2 * NG
+    q = to_device_padded(qbc3)
+    dq = torch.zeros_like(q)
+    aux = to_device_padded(auxbc3) if auxbc3 is not None else None
+    dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
+    cfl = torch.zeros((), dtype=torch.float64, device='cuda')
+    rc = _lib.lib().claw2d_sharpclaw_dq(None, _dev_grid(mx, my, dx, dy), _rp_struct(case),
+                                        _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper), _lib.F64,
+                                        q.data_ptr(), dq.data_ptr(), aux.data_ptr() if aux is not None else None,
+                                        dtd.data_ptr(), cfl.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc, 'claw2d_sharpclaw_dq')
+    torch.cuda.synchronize()
+    return from_device_interior(dq, mx, my), float(cfl.item())
+
+
+def gpu_sharp_ssp104(q0, aux, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), abc_upper=(1, 1),
+                     dx=0.1, dy=0.07):
+    """One fused SSP104 step; q0/aux interior arrays (BCs synthesised in the kernel)."""
+    import torch
+    from pyclaw_amd import _lib
+    ne, mx, my = q0.shape
+    pad = np.zeros((ne, mx + 2 * NG, my + 2 * NG))
+    pad[:, NG:-NG, NG:-NG] = q0
+    q = to_device_padded(pad)
+    qn, s1a, s1b, s2 = (torch.zeros_like(q) for _ in range(4))
+    auxd = None
+    if aux is not None:
+        apad = np.zeros((aux.shape[0], mx + 2 * NG, my + 2 * NG))
+        apad[:, NG:-NG, NG:-NG] = aux
+        auxd = to_device_padded(apad)
+    dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
+    cfl = torch.zeros(10, dtype=torch.float64, device='cuda')
+    rc = _lib.lib().claw2d_sharpclaw_ssp104_step(None, _dev_grid(mx, my, dx, dy), _rp_struct(case),
+                                                 _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper), _lib.F64,
+                                                 q.data_ptr(), qn.data_ptr(), s1a.data_ptr(), s1b.data_ptr(),
+                                                 s2.data_ptr(), auxd.data_ptr() if auxd is not None else None,
+                                                 dtd.data_ptr(), cfl.data_ptr(),
+                                                 torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc, 'claw2d_sharpclaw_ssp104_step')
+    torch.cuda.synchronize()
+    return from_device_interior(qn, mx, my), cfl.cpu().numpy()
