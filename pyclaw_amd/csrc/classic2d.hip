@@ -192,12 +192,17 @@ __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParam
 #pragma unroll
             for (int m = 0; m < NE; ++m) { bmA[m] = bpA[m] = bmP[m] = bpP[m] = R(0); }
             if (trans > 0) {
+                // trans_cor splits A-dq + cqxx and A+dq - cqxx (flux2.f90:170-176);
+                // qadd keeps the unmodified fluctuations (flux2.f90:108-111)
+                R tA[NE], tP[NE];
+#pragma unroll
+                for (int m = 0; m < NE; ++m) { tA[m] = amdq[m]; tP[m] = apdq[m]; }
                 if (order == 2 && trans == 2) {
 #pragma unroll
-                    for (int m = 0; m < NE; ++m) { amdq[m] = amdq[m] + cq[m]; apdq[m] = apdq[m] - cq[m]; }
+                    for (int m = 0; m < NE; ++m) { tA[m] = tA[m] + cq[m]; tP[m] = tP[m] - cq[m]; }
                 }
-                RP::template rpt<0>(P, 1, amdq, alm2, alo, al0, bmA, bpA, cx);
-                RP::template rpt<0>(P, 2, apdq, a_m2, a_m1, a_0, bmP, bpP, cx);
+                RP::template rpt<0>(P, 1, tA, alm2, alo, al0, bmA, bpA, cx);
+                RP::template rpt<0>(P, 2, tP, a_m2, a_m1, a_0, bmP, bpP, cx);
             }
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
@@ -266,9 +271,12 @@ __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParam
 #pragma unroll
             for (int m = 0; m < NE; ++m) { bmA[m] = bpA[m] = bmP[m] = bpP[m] = R(0); }
             if (trans > 0) {
+                R tA[NE], tP[NE];
+#pragma unroll
+                for (int m = 0; m < NE; ++m) { tA[m] = amY[m]; tP[m] = apY[m]; }
                 if (order == 2 && trans == 2) {
 #pragma unroll
-                    for (int m = 0; m < NE; ++m) { amY[m] = amY[m] + cqY[m]; apY[m] = apY[m] - cqY[m]; }
+                    for (int m = 0; m < NE; ++m) { tA[m] = tA[m] + cqY[m]; tP[m] = tP[m] - cqY[m]; }
                 }
                 R am2l[NA], am2r[NA], am1l[NA], am1r[NA];
                 if constexpr (HAS_AUX) {
@@ -279,8 +287,8 @@ __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParam
                         if (l0) { am2l[a] = al_m2[a]; am1l[a] = al_m1[a]; }
                     }
                 }
-                RP::template rpt<1>(P, 1, amY, am2l, a_m2, am2r, bmA, bpA, cY);
-                RP::template rpt<1>(P, 2, apY, am1l, a_m1, am1r, bmP, bpP, cY);
+                RP::template rpt<1>(P, 1, tA, am2l, a_m2, am2r, bmA, bpA, cY);
+                RP::template rpt<1>(P, 2, tP, am1l, a_m1, am1r, bmP, bpP, cY);
             }
             if (L >= R0 + 2) {
                 const int row = L - 2;

@@ -201,12 +201,10 @@ class Solver(object):
             if self.custom_bc_static:
                 if any(b == BC.custom for b in (self.bc_lower[1], self.bc_upper[1])):
                     raise ValueError("custom_bc_static supports custom x boundaries only")
-                key = id(state._q.buf)
-                if key not in self._static_filled:
-                    self._fill_custom_sides(state, state._q.buf)
-                    self._fill_custom_sides(state, self._qalt)
-                    self._static_filled.add(key)
-                    self._static_filled.add(id(self._qalt))
+                for b in self._static_targets(state):
+                    if id(b) not in self._static_filled:
+                        self._fill_custom_sides(state, b)
+                        self._static_filled.add(id(b))
                 for s in range(2):
                     bc.lower[s] = _lib.BC_STORED if self.bc_lower[s] == BC.custom else self.bc_lower[s]
                     bc.upper[s] = _lib.BC_STORED if self.bc_upper[s] == BC.custom else self.bc_upper[s]
@@ -235,6 +233,10 @@ class Solver(object):
             for s in range(2):
                 bc.aux_lower[s] = bc.aux_upper[s] = _lib.BC_EXTRAP
         return bc
+
+    def _static_targets(self, state):
+        """Buffers whose ghost cells hold time-independent user boundary values."""
+        return [state._q.buf, self._qalt]
 
     def _qbc_view(self, state, buf):
         ng = self.num_ghost
