@@ -49,9 +49,13 @@ def padded_with_bc(interior, ng, bc_lower, bc_upper, negate, ghost_rng=None):
     qbc = np.zeros((ne, mx + 2 * ng, my + 2 * ng), order='F')
     qbc[:, ng:-ng, ng:-ng] = interior
     if ghost_rng is not None:
+        # arbitrary but physically valid ghost states ("custom" boundary data):
+        # cell states drawn from the interior, shuffled
         mask = np.ones(qbc.shape[1:], bool)
         mask[ng:-ng, ng:-ng] = False
-        qbc[:, mask] = interior.mean() + 0.1 * ghost_rng.random((ne, mask.sum()))
+        cells = interior.reshape(ne, -1)
+        pick = ghost_rng.integers(0, cells.shape[1], mask.sum())
+        qbc[:, mask] = cells[:, pick]
     # custom sides: the stored values stay; other sides are filled in reference order
     O.fill_bc(qbc, ng, [c if c != 0 else 99 for c in bc_lower], [c if c != 0 else 99 for c in bc_upper],
               negate)
