@@ -1,0 +1,154 @@
+"""Host-side logic (no GPU): the C ABI library, geometry, ghost fill, State,
+solver options, frame IO."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import pyclaw_amd as pc
+from pyclaw_amd import _lib
+from pyclaw_amd.solver import fill_ghosts
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, 'include', 'clawhip.h')).read()
+    names = set(re.findall(r'^int\s+(claw\w+)\s*\(', hdr, flags=re.M))
+    assert len(names) >= 8
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+
+
+def test_abi_struct_layout():
+    # the C structs as declared in include/clawhip.h (natural alignment)
+    assert ctypes.sizeof(_lib.ClawGrid) == 4 * 4 + 8 * 2 + 8 * 2
+    assert ctypes.sizeof(_lib.ClawMethod) == 4 * 5 + 4 * 8
+    assert ctypes.sizeof(_lib.ClawRiemann) == 8 + 8 * 8
+    assert ctypes.sizeof(_lib.ClawBC) == 4 * 8
+
+
+def test_invalid_arguments_return_einval_without_gpu():
+    L = _lib.lib()
+    ctx = _lib.Context(0)
+    g = _lib.ClawGrid(1, 1, 2, 0, 5, 25, 0.1, 0.1)         # too small a grid
+    m = _lib.ClawMethod(); m.order = 2; m.num_waves = 5
+    r = _lib.ClawRiemann(); r.kind = _lib.RP_EULER
+    bc = _lib.ClawBC()
+    rc = L.claw2d_classic_step(ctx.handle, g, m, r, bc, _lib.F64, ctypes.c_void_p(16), ctypes.c_void_p(32),
+                               None, ctypes.c_void_p(48), ctypes.c_void_p(64), None)
+    assert rc == _lib.CLAW_EINVAL
+    g = _lib.ClawGrid(8, 8, 3, 0, 14, 196, 0.1, 0.1)
+    bc.lower[0], bc.upper[0] = _lib.BC_PERIODIC, _lib.BC_EXTRAP   # periodic on one side only
+    rc = L.claw2d_classic_step(ctx.handle, g, m, r, bc, _lib.F64, ctypes.c_void_p(16), ctypes.c_void_p(32),
+                               None, ctypes.c_void_p(48), ctypes.c_void_p(64), None)
+    assert rc == _lib.CLAW_EINVAL
+
+
+def test_dimension_and_domain_match_reference_doctests():
+    # geometry.py:397-421
+    x = pc.Dimension('x', 0., 1., 100)
+    assert x.num_cells == 100 and x.delta == 0.01
+    assert x.edges[0] == 0.0 and x.edges[1] == 0.01 and x.edges[-1] == 1.0
+    assert x.centers[-1] == 0.995 and len(x.centers) == 100 and len(x.edges) == 101
+    d = pc.Domain((0., -1.), (1., 1.), (10, 25))
+    assert d.grid.num_cells == [10, 25] and d.grid.lower == [0.0, -1.0]
+    assert d.grid.delta == [0.1, 0.08]
+    assert d.patch.x.name == 'x' and d.patch.y.name == 'y'
+
+
+@pytest.mark.parametrize("lo,hi", [((1, 1), (1, 1)), ((3, 3), (1, 1)), ((2, 2), (2, 2)), ((3, 1), (1, 3)),
+                                   ((2, 3), (2, 3))])
+@pytest.mark.parametrize("ng", [2, 3])
+def test_host_ghost_fill_matches_oracle(lo, hi, ng):
+    rng = np.random.default_rng(5)
+    q = rng.standard_normal((3, 9, 7))
+    ref = np.zeros((3, 9 + 2 * ng, 7 + 2 * ng), order='F')
+    ref[:, ng:-ng, ng:-ng] = q
+    O.fill_bc(ref, ng, lo, hi, True)
+    t = torch.zeros((3, 9 + 2 * ng, 7 + 2 * ng), dtype=torch.float64)
+    t[:, ng:-ng, ng:-ng] = torch.from_numpy(q)
+    fill_ghosts(t, ng, lo, hi, True)
+    assert np.array_equal(t.numpy(), ref)
+
+
+def test_custom_bc_callback_sees_reference_layout():
+    seen = {}
+
+    def cb(state, dim, t, qbc, ng):
+        seen['shape'] = tuple(qbc.shape)
+        for i in range(ng):
+            qbc[0, i, ...] = 7.0
+    t = torch.zeros((3, 6 + 4, 5 + 4), dtype=torch.float64)
+    fill_ghosts(t, 2, (0, 1), (1, 1), True, user_lower=cb)
+    assert seen['shape'] == (3, 10, 9)
+    # x-lower ghosts hold the user value except the corners, refilled by the y rules
+    assert np.all(t[0, :2, 2:-2].numpy() == 7.0)
+
+
+def test_state_host_mirror_and_device_buffer():
+    x = pc.Dimension('x', 0., 1., 6)
+    y = pc.Dimension('y', 0., 2., 4)
+    dom = pc.Domain([x, y])
+    s = pc.State(dom, 3, 2, device='cpu')
+    assert s.q.shape == (3, 6, 4) and s.aux.shape == (2, 6, 4)
+    q = np.arange(72.).reshape(3, 6, 4)
+    s.q[...] = q
+    s.sync_to_device()
+    ng = 3
+    buf = s._q.buf
+    assert np.array_equal(buf[:, ng:ng + 4, ng:ng + 6].permute(0, 2, 1).numpy(), q)
+    # device-side change is seen by the next host read
+    s.q_device[1, 2, 3] = -1.0
+    assert s.q[1, 2, 3] == -1.0
+    assert np.array_equal(s.get_q_global()[0], q[0])
+
+
+def test_solver_rejects_unknown_attributes_and_checks_problem_data():
+    s = pc.ClawSolver2D(pc.riemann.rp2_euler_5wave)
+    with pytest.raises(TypeError):
+        s.no_such_option = 1
+    assert s.num_waves == 5 and s.dimensional_split and s.transverse_waves == 1
+    assert s.cfl_max == 1.0 and s.cfl_desired == 0.9 and s.dt_initial == 0.1
+    with pytest.raises(Exception):
+        pc.riemann.rp2_euler_5wave.params({'gamma': 1.4})
+    sh = pc.SharpClawSolver2D(pc.riemann.rp2_shallow_roe_with_efix)
+    assert sh.cfl_max == 2.5 and sh.cfl_desired == 2.45 and sh.time_integrator == 'SSP104'
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error path")
+def test_solver_fails_loudly_without_gpu():
+    x = pc.Dimension('x', 0., 1., 8)
+    y = pc.Dimension('y', 0., 1., 8)
+    dom = pc.Domain([x, y])
+    st = pc.State(dom, 3)
+    st.problem_data.update(rho=1., bulk=1., cc=1., zz=1.)
+    sol = pc.Solution(st, dom)
+    s = pc.ClawSolver2D(pc.riemann.rp2_acoustics)
+    s.bc_lower = [1, 1]; s.bc_upper = [1, 1]
+    with pytest.raises(_lib.ClawError):
+        s.setup(sol)
+
+
+def test_ascii_frame_roundtrip(tmp_path):
+    x = pc.Dimension('x', -1., 1., 5)
+    y = pc.Dimension('y', 0., 1., 3)
+    dom = pc.Domain([x, y])
+    st = pc.State(dom, 3, 2, device='cpu')
+    rng = np.random.default_rng(0)
+    st.q[...] = rng.standard_normal((3, 5, 3))
+    st.aux[...] = rng.random((2, 5, 3))
+    st.t = 0.25
+    sol = pc.Solution(st, dom)
+    sol.write(4, str(tmp_path), write_aux=True)
+    back = pc.Solution()
+    back.read(4, str(tmp_path), read_aux=True)
+    assert back.state.t == 0.25
+    assert np.allclose(back.state.q, st.q, rtol=1e-7, atol=1e-8)
+    assert np.allclose(back.state.aux, st.aux, rtol=1e-7, atol=1e-8)
+    assert back.domain.grid.num_cells == [5, 3]
