@@ -38,6 +38,10 @@ def parse():
     ap.add_argument('--precision', default='f64', choices=['f64', 'f32'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
+    ap.add_argument('--workload', default='euler',
+                    choices=['euler', 'batched', 'sharpclaw', 'acoustics'],
+                    help="euler: the headline line; the others measure the other configurations")
+    ap.add_argument('--instances', type=int, default=512)
     return ap.parse_args()
 
 
@@ -68,6 +72,8 @@ def main():
         print("bench.py measures one GPU; multi-GPU runs are not part of this build "
               "(see DESIGN.md for how the grid would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != 'euler':
+        return OTHER[args.workload](args)
     import torch
     from pyclaw_amd import _lib
     from pyclaw_amd.problems import euler_max_speed, shock_state, shockbubble
@@ -173,6 +179,183 @@ def main():
                   "hbm_pct": 100.0 * achieved / HBM_PEAK_GBS},
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_batched(args):
+    """512 independent layered vc-acoustics grids (sizes from {64,...,512}^2),
+    200 steps each with its own CFL-0.9 dt: one hipGraph-captured sequence of
+    persistent work-queue launches vs the same grids stepped one at a time
+    through the plain single-grid entry point."""
+    import torch
+    import pyclaw_amd as pc
+    from pyclaw_amd.batch import BatchedClawSolver2D, run_one_by_one
+    from pyclaw_amd.problems import batched_sizes, layered_vc_acoustics
+    rng = np.random.default_rng(2024)
+    sizes = batched_sizes(rng, args.instances)
+    inst = []
+    for (mx, my) in sizes:
+        q, aux = layered_vc_acoustics(rng, mx, my)
+        dx, dy = 1.0 / mx, 1.0 / my
+        inst.append(dict(q=q, aux=aux, dx=dx, dy=dy, dt_initial=0.9 * min(dx, dy) / aux[1].max()))
+    nsteps = args.steps
+    cells = sum(mx * my for mx, my in sizes)
+    rp = pc.riemann.rp2_vc_acoustics
+
+    b = BatchedClawSolver2D(rp, {}, inst)
+    b.run(nsteps, use_graph=True)          # graph capture + first replay (warm-up)
+    torch.cuda.synchronize()
+    for k in range(b.n):                   # back to the initial data, then the timed replay
+        b._put(k, inst[k]['q'], None)
+    b.dt.copy_(torch.tensor([i['dt_initial'] for i in inst], dtype=torch.float64))
+    b.t.zero_(); b.parity.zero_(); b.naccepted.zero_(); b.nrejected.zero_(); b.cflmax.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(nsteps, use_graph=True)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    b3 = b
+    acc = b3.naccepted.cpu().numpy()
+    t0 = time.perf_counter()
+    ref = run_one_by_one(rp, {}, inst[:8], nsteps)      # equality check on a few instances
+    same = all(np.array_equal(b3.q(k), ref[k]) for k in range(8))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_one_by_one(rp, {}, inst, nsteps, keep=False)
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    out = {"metric": "batched vs plain speedup", "value": t_loop / t_batch, "unit": "x", "n_gpus": 1,
+           "steps": nsteps, "warmup": nsteps, "ms_per_step": 1e3 * t_batch / nsteps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded layered media)",
+           "config": {"workload": "%d vc-acoustics instances, sizes {64..512}^2, %d steps each, classic unsplit "
+                                  "MC trans_inc, CFL 0.9 per instance" % (args.instances, nsteps),
+                      "global_batch": args.instances, "parallelism": "single GPU"},
+           "extra": {"batched_s": t_batch, "one_by_one_s": t_loop, "cells_per_step": cells,
+                     "batched_gcell_updates_per_s": cells * nsteps / t_batch / 1e9,
+                     "bitwise_equal_first8": same, "all_accepted": bool((acc == nsteps).all()),
+                     "work_items_per_step": b3.work_items}}
+    print(json.dumps(out), flush=True)
+
+
+def _grid_bufs(q0, ng, dtype, dev):
+    import torch
+    ne, mx, my = q0.shape
+    buf = torch.zeros((ne, my + 2 * ng, mx + 2 * ng), dtype=torch.float64)
+    buf[:, ng:ng + my, ng:ng + mx] = torch.from_numpy(np.ascontiguousarray(q0.transpose(0, 2, 1)))
+    return buf.to(dev, dtype)
+
+
+def bench_sharpclaw(args):
+    """SharpClaw shallow water 1024x1024 periodic, WENO5 + SSP104, FP64: one
+    step = 10 fused stage launches (one C call)."""
+    import torch
+    from pyclaw_amd import _lib
+    from pyclaw_amd.problems import shallow_hump
+    mx = my = 1024
+    rng = np.random.default_rng(7)
+    x0 = 0.25 + 0.5 * rng.random(2)
+    q0 = shallow_hump(mx, my, x0)
+    dx = dy = 1.0 / mx
+    smax = float((np.sqrt(q0[0])).max())
+    dt = 0.5 * dx / smax
+    dev = torch.device('cuda', 0)
+    ng = 3
+    q = _grid_bufs(q0, ng, torch.float64, dev)
+    qn, s1a, s1b, s2 = (torch.zeros_like(q) for _ in range(4))
+    g = _lib.ClawGrid(mx, my, ng, 0, mx + 2 * ng, (mx + 2 * ng) * (my + 2 * ng), dx, dy)
+    r = _lib.ClawRiemann(); r.kind = _lib.RP_SHALLOW; r.efix = 1; r.p[0] = 1.0
+    bc = _lib.ClawBC()
+    for s_ in range(2):
+        bc.lower[s_] = bc.upper[s_] = _lib.BC_PERIODIC
+        bc.aux_lower[s_] = bc.aux_upper[s_] = _lib.BC_PERIODIC
+    L = _lib.lib()
+    dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
+    cfl = torch.zeros(10, dtype=torch.float64, device=dev)
+    sh = torch.cuda.current_stream().cuda_stream
+    pair = [q, qn]
+    tot0 = q0.reshape(3, -1).sum(axis=1)
+
+    def step(k):
+        rc = L.claw2d_sharpclaw_ssp104_step(None, g, r, bc, _lib.F64, pair[k % 2].data_ptr(), pair[(k + 1) % 2].data_ptr(),
+                                            s1a.data_ptr(), s1b.data_ptr(), s2.data_ptr(), None, dtd.data_ptr(),
+                                            cfl.data_ptr(), sh)
+        _lib.check(rc, 'ssp104')
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(args.steps):
+        step(args.warmup + k)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    fin = pair[(args.warmup + args.steps) % 2][:, ng:ng + my, ng:ng + mx].to('cpu').numpy()
+    tot1 = np.array([np.sum(fin[m], dtype=np.float64) for m in range(3)])
+    h_rel = abs(tot1[0] - tot0[0]) / tot0[0]
+    mom_rel = np.abs(tot1[1:] - tot0[1:]) / tot0[0]
+    bytes_step = 552
+    achieved = bytes_step * mx * my / (ms * 1e-3) / 1e9
+    out = {"metric": "Gcell-updates/s", "value": mx * my / (ms * 1e-3) / 1e9, "unit": "Gcell-updates/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded Gaussian hump)",
+           "config": {"workload": "SharpClaw shallow water 1024x1024 periodic, WENO5 comp-wise + SSP104 (10 fused "
+                                  "stage launches per step), CFL 0.5", "global_batch": 1, "parallelism": "single GPU"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": bytes_step},
+           "extra": {"mass_rel_change": float(h_rel), "momentum_rel_change_over_mass": mom_rel.tolist(),
+                     "cfl_last_stage": float(cfl[9].item())}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_acoustics(args):
+    """Config-1 homogeneous acoustics 200x200 (cache resident): microseconds per
+    unsplit step, trans_cor, MC, walls low / extrapolation high."""
+    import torch
+    from pyclaw_amd import _lib
+    from pyclaw_amd.problems import acoustics_pulse
+    mx = my = 200
+    rng = np.random.default_rng(1)
+    c0 = tuple(-0.1 + 0.2 * rng.random(2))
+    q0 = acoustics_pulse(mx, my, center=c0)
+    dx = dy = 2.0 / mx
+    dt = 0.9 * dx / 2.0
+    dev = torch.device('cuda', 0)
+    ng = 3
+    qa = _grid_bufs(q0, ng, torch.float64, dev); qb = qa.clone()
+    g = _lib.ClawGrid(mx, my, ng, 0, mx + 2 * ng, (mx + 2 * ng) * (my + 2 * ng), dx, dy)
+    m = _lib.ClawMethod(); m.order, m.transverse, m.split, m.fwave, m.num_waves = 2, 2, 0, 0, 2
+    m.mthlim[0] = m.mthlim[1] = 4
+    r = _lib.ClawRiemann(); r.kind = _lib.RP_ACOUSTICS; r.p[0], r.p[1], r.p[2], r.p[3] = 1.0, 4.0, 2.0, 2.0
+    bc = _lib.ClawBC(); bc.lower[0] = bc.lower[1] = _lib.BC_WALL; bc.upper[0] = bc.upper[1] = _lib.BC_EXTRAP
+    L = _lib.lib(); ctx = _lib.Context(0)
+    dtd = torch.tensor(dt, dtype=torch.float64, device=dev); cfl = torch.zeros((), dtype=torch.float64, device=dev)
+    sh = torch.cuda.current_stream().cuda_stream
+    pair = [qa, qb]
+
+    def step(k):
+        _lib.check(L.claw2d_classic_step_ex(ctx.handle, g, m, r, bc, _lib.F64, pair[k % 2].data_ptr(),
+                                            pair[(k + 1) % 2].data_ptr(), None, dtd.data_ptr(), cfl.data_ptr(),
+                                            _lib.STEP_ACCUMULATE_CFL, sh), 'step')
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(args.steps):
+        step(k)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    out = {"metric": "us per step", "value": ms * 1e3, "unit": "us/step", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "homogeneous acoustics 200x200, unsplit MC trans_cor (cache resident)",
+                      "global_batch": 1, "parallelism": "single GPU"},
+           "extra": {"gcell_updates_per_s": mx * my / (ms * 1e-3) / 1e9}}
+    print(json.dumps(out), flush=True)
+
+
+OTHER = {'batched': bench_batched, 'sharpclaw': bench_sharpclaw, 'acoustics': bench_acoustics}
 
 
 if __name__ == '__main__':

@@ -145,6 +145,45 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
                                  void *s1a, void *s1b, void *s2, const void *aux,
                                  const double *dt_dev, double *cfl10_dev, void *stream);
 
+/* ---- Batched runs of many independent grids ---------------------------------
+   No counterpart in the reference (which runs one grid per process); this is
+   the batching of the build's plan: instances of different sizes advance
+   together, one persistent work-queue launch per step, with the reference's
+   per-grid accept/reject and dt rule (solver.py:651-687) applied on the device.
+   All instances share the method, Riemann solver and boundary codes. */
+typedef struct {
+    int32_t mx, my;
+    int64_t q_offset[2];    /* element offsets in q_pool of the instance's two state buffers */
+    int64_t aux_offset;     /* element offset in aux_pool, or -1 without aux                */
+    int64_t ldq, plane;     /* planar padded layout of this instance (see above)           */
+    double dx, dy;
+} ClawBatchInstance;
+
+/* Builds the work list (instances sorted largest first, one item per
+   61-column strip x 32-row band) and device descriptors. */
+int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, int ng,
+                             const ClawMethod *method, const ClawRiemann *rp, const ClawBC *bc,
+                             int precision, void **plan);
+int claw2d_batch_plan_destroy(void *plan);
+int claw2d_batch_work_items(void *plan);
+
+/* nsteps unsplit steps of every instance.  Per-instance device arrays (length
+   n): dt (in/out), t (in/out), cflmax (in/out), parity (0: state in buffer 0,
+   1: buffer 1; flipped on each accepted step), naccepted, nrejected (counts).
+   use_graph != 0 captures the whole sequence into a hipGraph (cached for
+   repeated calls with the same arguments) and launches it on `stream`. */
+int claw2d_batch_steps(void *plan, void *q_pool, const void *aux_pool, int nsteps,
+                       double cfl_max, double cfl_desired, double dt_max, double *dt, double *t,
+                       double *cflmax, int32_t *parity, int32_t *naccepted, int32_t *nrejected,
+                       int use_graph, void *stream);
+
+/* The same per-step bookkeeping for n grids stepped by other entry points:
+   reads cfl[k] (then zeroes it), accepts if cfl <= cfl_max (t += dt, parity
+   flips, naccepted++), else nrejected++; dt = min(dt_max, dt*cfl_desired/cfl). */
+int claw2d_step_update(int n, double cfl_max, double cfl_desired, double dt_max, double *dt,
+                       double *cfl, double *t, double *cflmax, int32_t *parity,
+                       int32_t *naccepted, int32_t *nrejected, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <utility>
+#include <vector>
 
 #include "common.hpp"
 #include "riemann.hpp"
@@ -66,19 +68,17 @@ __device__ __forceinline__ int xcd_block() {
     return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
 }
 
+// One work item of the unsplit step: column strip sx, row band sy (whole
+// wavefront, uniform arguments).  Used by the single-grid kernel below and by
+// the batched work-queue kernel.
 template <class RP, typename R>
-__global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
+__device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy) {
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
     constexpr int NA = HAS_AUX ? RP::NA : 1;
     using Cache = typename RP::Cache;
 
     const int lane = threadIdx.x & (WAVE - 1);
-    const int vb = xcd_block();
-    const int sx = vb / p.nsyb;
-    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-    if (sx >= p.nsx || sy >= p.nsy) return;
-
     const GridDev g = p.g;
     const int c0 = sx * XS_USEFUL;
     const int i = c0 - 1 + lane;
@@ -333,6 +333,15 @@ __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParam
 
     double c = wave_max((double)cfl);
     if (lane == 0) atomic_max_nonneg(p.cfl, c);
+}
+
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
+    const int vb = xcd_block();
+    const int sx = vb / p.nsyb;
+    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+    if (sx >= p.nsx || sy >= p.nsy) return;
+    unsplit_band<RP, R>(p, sx, sy);
 }
 
 // ---------------------------------------------------------------------------
@@ -599,6 +608,100 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
 __global__ void zero_scalar_kernel(double *p) { *p = 0.0; }
 
 // ---------------------------------------------------------------------------
+// Batched runs: many independent grids of different sizes advanced together.
+// One launch per step: a persistent grid of wavefronts pulls (instance, strip,
+// band) work items from a queue (atomic counter) and runs unsplit_band on each;
+// a small update kernel then applies the reference's accept/reject and dt
+// rule per instance (solver.py:651-687) on the device, flips the instance's
+// buffer parity on acceptance, and rearms the queue and the Courant slots, so
+// a whole sequence of steps can be captured in one hipGraph without host
+// synchronisation or memset nodes.
+// ---------------------------------------------------------------------------
+struct BatchInstDev {
+    int mx, my, nsx, nsy, H, pad_;
+    long long q0, q1, aux, ldq, plane;
+    double dx, dy;
+};
+
+template <class RP, typename R>
+struct BatchStepParams {
+    R *qpool;
+    const R *auxpool;
+    const BatchInstDev *inst;
+    const int *item_inst;
+    const int *item_local;
+    int nitems;
+    int *counter;
+    const double *dt;
+    double *cfl;
+    const int *parity;
+    BCDev bq, ba;
+    typename RP::Params P;
+    int order, trans, ng;
+    int mthlim[8];
+};
+
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<RP, R> b) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(b.counter, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, WAVE));
+        if (item >= b.nitems) break;
+        const int k = b.item_inst[item];
+        const int local = b.item_local[item];
+        const BatchInstDev I = b.inst[k];
+        const int par = b.parity[k];
+        ClassicParams<RP, R> p;
+        p.qold = b.qpool + (par ? I.q1 : I.q0);
+        p.qnew = b.qpool + (par ? I.q0 : I.q1);
+        p.aux = b.auxpool ? b.auxpool + I.aux : nullptr;
+        p.g = GridDev{I.mx, I.my, b.ng, I.ldq, I.plane};
+        p.bq = b.bq; p.ba = b.ba; p.P = b.P;
+        p.dt = b.dt + k; p.cfl = b.cfl + k;
+        p.dx = I.dx; p.dy = I.dy;
+        p.order = b.order; p.trans = b.trans;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) p.mthlim[q] = b.mthlim[q];
+        p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0;
+        unsplit_band<RP, R>(p, local / I.nsy, local % I.nsy);
+    }
+}
+
+// evolve_to_time's per-step bookkeeping, one thread per instance.
+__global__ void batch_update_kernel(int n, double cfl_max, double cfl_desired, double dt_max, double *dt,
+                                    double *cfl, double *t, double *cflmax, int *parity, int *nacc, int *nrej,
+                                    int *counter) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0 && counter) *counter = 0;
+    if (k >= n) return;
+    const double c = cfl[k];
+    if (c <= cfl_max) {
+        cflmax[k] = c > cflmax[k] ? c : cflmax[k];
+        t[k] = t[k] + dt[k];
+        parity[k] ^= 1;
+        nacc[k] += 1;
+    } else {
+        nrej[k] += 1;
+    }
+    if (c > 0.0) {
+        const double d = dt[k] * cfl_desired / c;
+        dt[k] = dt_max <= d ? dt_max : d;
+    } else {
+        dt[k] = dt_max;
+    }
+    cfl[k] = 0.0;
+}
+
+__global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *counter = 0;
+    if (k < n) cfl[k] = 0.0;
+}
+
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 struct Ctx {
@@ -699,6 +802,90 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
     return check(hipGetLastError());
 }
 
+
+// ---- batched runs: host side -----------------------------------------------
+struct BatchPlan {
+    Ctx *ctx = nullptr;
+    int n = 0, ng = 0, nitems = 0, precision = 0, blocks = 0;
+    ClawMethod meth;
+    ClawRiemann rp;
+    ClawBC bc;
+    BatchInstDev *d_inst = nullptr;
+    int *d_item_inst = nullptr, *d_item_local = nullptr, *d_counter = nullptr;
+    double *d_cfl = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    const void *key[10] = {};
+    int key_steps = -1;
+    double key_par[3] = {0, 0, 0};
+};
+
+template <class RP, typename R>
+static int batch_step_launch(BatchPlan *pl, void *qpool, const void *auxpool, const double *dt, const int *parity,
+                             hipStream_t st) {
+    BatchStepParams<RP, R> b;
+    b.qpool = (R *)qpool; b.auxpool = (const R *)auxpool;
+    b.inst = pl->d_inst; b.item_inst = pl->d_item_inst; b.item_local = pl->d_item_local;
+    b.nitems = pl->nitems; b.counter = pl->d_counter;
+    b.dt = dt; b.cfl = pl->d_cfl; b.parity = parity;
+    b.bq = BCDev{{pl->bc.lower[0], pl->bc.lower[1]}, {pl->bc.upper[0], pl->bc.upper[1]}};
+    b.ba = BCDev{{pl->bc.aux_lower[0], pl->bc.aux_lower[1]}, {pl->bc.aux_upper[0], pl->bc.aux_upper[1]}};
+    b.P = make_params<RP, R>(&pl->rp);
+    b.order = pl->meth.order; b.trans = pl->meth.transverse; b.ng = pl->ng;
+    for (int k = 0; k < 8; ++k) b.mthlim[k] = pl->meth.mthlim[k];
+    if (pl->blocks == 0) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<RP, R>, 256, 0) != hipSuccess)
+            per_cu = 1;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+        pl->blocks = std::max(1, per_cu) * std::max(1, cus);
+        // no more waves than work items
+        pl->blocks = std::min(pl->blocks, (pl->nitems + 3) / 4);
+    }
+    hipLaunchKernelGGL((batch_step_kernel<RP, R>), dim3(pl->blocks), dim3(256), 0, st, b);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+static int batch_step_dispatch(BatchPlan *pl, void *qpool, const void *auxpool, const double *dt, const int *parity,
+                               hipStream_t st) {
+    const bool f32 = pl->precision == CLAW_F32;
+#define BDISPATCH(RPT) \
+    return f32 ? batch_step_launch<RPT<float>, float>(pl, qpool, auxpool, dt, parity, st) \
+               : batch_step_launch<RPT<double>, double>(pl, qpool, auxpool, dt, parity, st)
+    switch (pl->rp.kind) {
+    case CLAW_RP_ACOUSTICS: BDISPATCH(Acoustics);
+    case CLAW_RP_VC_ACOUSTICS: BDISPATCH(VCAcoustics);
+    case CLAW_RP_EULER: BDISPATCH(Euler5);
+    default: BDISPATCH(ShallowRoe);
+    }
+#undef BDISPATCH
+}
+
+static int batch_sequence(BatchPlan *pl, void *qpool, const void *auxpool, int nsteps, double cfl_max,
+                          double cfl_desired, double dt_max, double *dt, double *t, double *cflmax, int *parity,
+                          int *nacc, int *nrej, hipStream_t st) {
+    const int tb = 256, nb = (pl->n + tb - 1) / tb;
+    hipLaunchKernelGGL(batch_arm_kernel, dim3(nb), dim3(tb), 0, st, pl->n, pl->d_cfl, pl->d_counter);
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = batch_step_dispatch(pl, qpool, auxpool, dt, parity, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(batch_update_kernel, dim3(nb), dim3(tb), 0, st, pl->n, cfl_max, cfl_desired, dt_max, dt,
+                           pl->d_cfl, t, cflmax, parity, nacc, nrej, pl->d_counter);
+    }
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+static void batch_free(BatchPlan *pl) {
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->cap) (void)hipStreamDestroy(pl->cap);
+    (void)hipFree(pl->d_inst); (void)hipFree(pl->d_item_inst); (void)hipFree(pl->d_item_local);
+    (void)hipFree(pl->d_counter); (void)hipFree(pl->d_cfl);
+    delete pl;
+}
 }  // namespace clawhip
 
 using namespace clawhip;
@@ -770,6 +957,110 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
     default: DISPATCH(ShallowRoe);
     }
 #undef DISPATCH
+}
+
+int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, int ng, const ClawMethod *meth,
+                             const ClawRiemann *rp, const ClawBC *bc, int precision, void **plan_out) {
+    if (!ctx || !inst || n <= 0 || !meth || !rp || !bc || !plan_out) return CLAW_EINVAL;
+    if (meth->split || meth->fwave || (meth->order != 1 && meth->order != 2)) return CLAW_EINVAL;
+    if (meth->transverse < 0 || meth->transverse > 2) return CLAW_EINVAL;
+    if (precision != CLAW_F64 && precision != CLAW_F32) return CLAW_EINVAL;
+    if (ng < 2 || rp->kind < 0 || rp->kind > 3) return CLAW_EINVAL;
+    for (int s = 0; s < 2; ++s)
+        if ((bc->lower[s] == CLAW_BC_PERIODIC) != (bc->upper[s] == CLAW_BC_PERIODIC)) return CLAW_EINVAL;
+    std::vector<BatchInstDev> hi(n);
+    std::vector<std::pair<long long, int>> order;
+    for (int k = 0; k < n; ++k) {
+        const ClawBatchInstance &I = inst[k];
+        if (I.mx < 2 || I.my < 2 || I.ldq < I.mx + 2 * ng || I.plane < I.ldq * (I.my + 2 * ng)) return CLAW_EINVAL;
+        if (rp->kind == CLAW_RP_VC_ACOUSTICS && I.aux_offset < 0) return CLAW_EINVAL;
+        BatchInstDev d;
+        d.mx = I.mx; d.my = I.my;
+        d.nsx = (I.mx + XS_USEFUL - 1) / XS_USEFUL;
+        d.H = std::min(I.my, 32);
+        d.nsy = (I.my + d.H - 1) / d.H;
+        d.pad_ = 0;
+        d.q0 = I.q_offset[0]; d.q1 = I.q_offset[1]; d.aux = I.aux_offset < 0 ? 0 : I.aux_offset;
+        d.ldq = I.ldq; d.plane = I.plane; d.dx = I.dx; d.dy = I.dy;
+        hi[k] = d;
+        order.push_back({-(long long)I.mx * I.my, k});
+    }
+    // work items of the largest grids first (longest-processing-time order)
+    std::stable_sort(order.begin(), order.end());
+    std::vector<int> item_inst, item_local;
+    for (auto &o : order) {
+        const BatchInstDev &d = hi[o.second];
+        for (int l = 0; l < d.nsx * d.nsy; ++l) { item_inst.push_back(o.second); item_local.push_back(l); }
+    }
+    BatchPlan *pl = new (std::nothrow) BatchPlan();
+    if (!pl) return CLAW_ENOMEM;
+    pl->ctx = (Ctx *)ctx; pl->n = n; pl->ng = ng; pl->nitems = (int)item_inst.size(); pl->precision = precision;
+    pl->meth = *meth; pl->rp = *rp; pl->bc = *bc;
+    bool ok = hipMalloc(&pl->d_inst, sizeof(BatchInstDev) * n) == hipSuccess &&
+              hipMalloc(&pl->d_item_inst, sizeof(int) * pl->nitems) == hipSuccess &&
+              hipMalloc(&pl->d_item_local, sizeof(int) * pl->nitems) == hipSuccess &&
+              hipMalloc(&pl->d_counter, sizeof(int)) == hipSuccess &&
+              hipMalloc(&pl->d_cfl, sizeof(double) * n) == hipSuccess;
+    if (!ok) { batch_free(pl); return CLAW_ENOMEM; }
+    ok = hipMemcpy(pl->d_inst, hi.data(), sizeof(BatchInstDev) * n, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(pl->d_item_inst, item_inst.data(), sizeof(int) * pl->nitems, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(pl->d_item_local, item_local.data(), sizeof(int) * pl->nitems, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(pl->d_counter, 0, sizeof(int)) == hipSuccess &&
+         hipMemset(pl->d_cfl, 0, sizeof(double) * n) == hipSuccess &&
+         hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) { batch_free(pl); return CLAW_EDEVICE; }
+    *plan_out = pl;
+    return CLAW_OK;
+}
+
+int claw2d_batch_plan_destroy(void *plan) {
+    if (!plan) return CLAW_EINVAL;
+    batch_free((BatchPlan *)plan);
+    return CLAW_OK;
+}
+
+int claw2d_batch_work_items(void *plan) { return plan ? ((BatchPlan *)plan)->nitems : -1; }
+
+int claw2d_batch_steps(void *plan, void *q_pool, const void *aux_pool, int nsteps, double cfl_max,
+                       double cfl_desired, double dt_max, double *dt, double *t, double *cflmax, int *parity,
+                       int *naccepted, int *nrejected, int use_graph, void *stream) {
+    BatchPlan *pl = (BatchPlan *)plan;
+    if (!pl || !q_pool || nsteps < 0 || !dt || !t || !cflmax || !parity || !naccepted || !nrejected)
+        return CLAW_EINVAL;
+    if (pl->rp.kind == CLAW_RP_VC_ACOUSTICS && !aux_pool) return CLAW_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (!use_graph)
+        return batch_sequence(pl, q_pool, aux_pool, nsteps, cfl_max, cfl_desired, dt_max, dt, t, cflmax, parity,
+                              naccepted, nrejected, st);
+    const void *key[10] = {q_pool, aux_pool, dt, t, cflmax, parity, naccepted, nrejected, nullptr, nullptr};
+    bool same = pl->exec && pl->key_steps == nsteps && pl->key_par[0] == cfl_max && pl->key_par[1] == cfl_desired &&
+                pl->key_par[2] == dt_max;
+    for (int k = 0; k < 10 && same; ++k) same = (pl->key[k] == key[k]);
+    if (!same) {
+        if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
+        if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+        if (hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) return CLAW_EDEVICE;
+        int rc = batch_sequence(pl, q_pool, aux_pool, nsteps, cfl_max, cfl_desired, dt_max, dt, t, cflmax, parity,
+                                naccepted, nrejected, pl->cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(pl->cap, &g);
+        if (rc || e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return rc ? rc : CLAW_EDEVICE; }
+        pl->graph = g;
+        if (hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0) != hipSuccess) { pl->exec = nullptr; return CLAW_EDEVICE; }
+        for (int k = 0; k < 10; ++k) pl->key[k] = key[k];
+        pl->key_steps = nsteps;
+        pl->key_par[0] = cfl_max; pl->key_par[1] = cfl_desired; pl->key_par[2] = dt_max;
+    }
+    return hipGraphLaunch(pl->exec, st) == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+int claw2d_step_update(int n, double cfl_max, double cfl_desired, double dt_max, double *dt, double *cfl, double *t,
+                       double *cflmax, int *parity, int *naccepted, int *nrejected, void *stream) {
+    if (n <= 0 || !dt || !cfl || !t || !cflmax || !parity || !naccepted || !nrejected) return CLAW_EINVAL;
+    const int tb = 256, nb = (n + tb - 1) / tb;
+    hipLaunchKernelGGL(batch_update_kernel, dim3(nb), dim3(tb), 0, (hipStream_t)stream, n, cfl_max, cfl_desired,
+                       dt_max, dt, cfl, t, cflmax, parity, naccepted, nrejected, (int *)nullptr);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 
 }  // extern "C"

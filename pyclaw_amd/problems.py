@@ -97,3 +97,30 @@ def shallow_hump(mx, my, x0, lower=(0., 0.), upper=(1., 1.), amp=0.5, width2=0.0
 
 def sound_speed_acoustics(bulk, rho):
     return math.sqrt(bulk / rho)
+
+
+def layered_vc_acoustics(rng, nx, ny, nlayers=None, amp=1.0, width=0.05):
+    """One instance of the batched variable-coefficient acoustics workload on
+    [0,1]^2: 2-6 horizontal layers (interfaces uniform in y), per-layer
+    rho ~ U[1,4] and c ~ U[0.5,2]; pressure pulse amp*exp(-|x-x0|^2/width^2)
+    at a uniform random x0; u = v = 0.  Returns q (3,nx,ny), aux (2,nx,ny)."""
+    if nlayers is None:
+        nlayers = int(rng.integers(2, 7))
+    cuts = np.sort(rng.random(nlayers - 1))
+    rho_l = 1.0 + 3.0 * rng.random(nlayers)
+    c_l = 0.5 + 1.5 * rng.random(nlayers)
+    x0 = rng.random(2)
+    x, _ = centers(0.0, 1.0, nx)
+    y, _ = centers(0.0, 1.0, ny)
+    layer = np.searchsorted(cuts, y)
+    aux = np.zeros((2, nx, ny), order='F')
+    aux[0] = rho_l[layer][None, :]
+    aux[1] = c_l[layer][None, :]
+    Y, X = np.meshgrid(y, x)
+    q = np.zeros((3, nx, ny), order='F')
+    q[0] = amp * np.exp(-((X - x0[0]) ** 2 + (Y - x0[1]) ** 2) / width ** 2)
+    return q, aux
+
+
+def batched_sizes(rng, n, choices=tuple(range(64, 513, 32))):
+    return [(int(rng.choice(choices)), int(rng.choice(choices))) for _ in range(n)]
