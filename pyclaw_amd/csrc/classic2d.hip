@@ -331,8 +331,9 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         }
     }
 
-    double c = wave_max((double)cfl);
-    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+    // max with 0 leaves the slot as it is: the Courant slots hold
+    // non-negative values only
+    atomic_max_nonneg_all_lanes(p.cfl, wave_max((double)cfl));
 }
 
 template <class RP, typename R>
@@ -645,9 +646,11 @@ template <class RP, typename R>
 __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<RP, R> b) {
     const int lane = threadIdx.x & (WAVE - 1);
     for (;;) {
-        int item = 0;
-        if (lane == 0) item = atomicAdd(b.counter, 1);
-        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, WAVE));
+        // one ticket per wavefront: every lane takes part in the atomic (lane 0
+        // adds 1, the others 0) so the loop head holds no lane-dependent branch;
+        // lane 0's old value is the ticket
+        const int mine = atomicAdd(b.counter, lane == 0 ? 1 : 0);
+        const int item = __builtin_amdgcn_readfirstlane(__shfl(mine, 0, WAVE));
         if (item >= b.nitems) break;
         const int k = b.item_inst[item];
         const int local = b.item_local[item];

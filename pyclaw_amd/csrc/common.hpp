@@ -122,6 +122,17 @@ __device__ __forceinline__ void atomic_max_nonneg(double *p, double v) {
     atomicMax(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v));
 }
 
+// The same, issued by every lane of a wavefront that holds the same v (a
+// wave_max result) without a lane-dependent branch.  Code that runs inside a
+// work-queue loop must not guard its per-wave atomics with `lane == 0`: two
+// such guards in a row (this one at the end of an item, the queue fetch at the
+// start of the next) let the compiler thread the branch so that lanes other
+// than 0 skip the fetch and loop on a stale item that only lane 0 could end.
+__device__ __forceinline__ void atomic_max_nonneg_all_lanes(double *p, double v) {
+    const unsigned long long bits = v > 0.0 ? (unsigned long long)__double_as_longlong(v) : 0ull;
+    atomicMax(reinterpret_cast<unsigned long long *>(p), bits);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
