@@ -39,9 +39,10 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
-                    choices=['euler', 'batched', 'sharpclaw', 'acoustics'],
+                    choices=['euler', 'batched', 'sharpclaw', 'acoustics', 'adjoint'],
                     help="euler: the headline line; the others measure the other configurations")
     ap.add_argument('--instances', type=int, default=512)
+    ap.add_argument('--adjoint-steps', type=int, default=2000)
     return ap.parse_args()
 
 
@@ -355,7 +356,86 @@ def bench_acoustics(args):
     print(json.dumps(out), flush=True)
 
 
-OTHER = {'batched': bench_batched, 'sharpclaw': bench_sharpclaw, 'acoustics': bench_acoustics}
+def bench_adjoint(args):
+    """Gradient of the receiver misfit of 2D vc acoustics at 512x512 (2000 steps
+    at dt = 0.8 dx / max c, MC limiter, trans_inc, extrapolation boundaries;
+    K in [1,4], rho in [1,2] smooth random fields; 64 receivers on y = 0.95;
+    observed traces from K scaled by 1.05 in a centred disc) with respect to
+    K, rho and the initial state, checkpointing every 45 steps.  Reports the
+    gradient time (forward with checkpoints + reverse sweep) over the time of
+    the plain forward run, and the adjoint step kernels against the forward
+    step kernel."""
+    import torch
+    from pyclaw_amd.adjoint import MisfitProblem, receivers_on_row
+    from pyclaw_amd.problems import adjoint_medium
+    mx = my = 512
+    N = args.adjoint_steps
+    rng = np.random.default_rng(2025)
+    q0, K, rho, K_true = adjoint_medium(rng, mx, my)
+    dx = dy = 1.0 / mx
+    dt = 0.8 * dx / float(np.sqrt(K / rho).max())
+    rec = receivers_on_row(mx, my, 0.95, 64)
+    P = MisfitProblem(mx, my, dx, dy, dt, N, rec, order=2, transverse=1, limiters=4, every=45)
+    aux_true = P.aux_from(K_true, rho)
+    obs, _, _ = P.forward(P.padded(q0), aux_true, keep_checkpoints=False)
+    P.p_obs = obs
+    aux = P.aux_from(K, rho)
+    q0p = P.padded(q0)
+    P.forward(q0p, aux, keep_checkpoints=False)        # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P.forward(q0p, aux, keep_checkpoints=False)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    traces, cps, _ = P.forward(q0p, aux)
+    torch.cuda.synchronize()
+    t_fwd_cp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lam, gaux = P.backward(traces, cps, aux)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    J = float(P.objective(traces))
+    finite = bool(torch.isfinite(lam).all()) and bool(torch.isfinite(gaux).all())
+    # per-kernel averages on the launch stream: forward step vs adjoint step
+    st = torch.cuda.current_stream()
+    lam2 = torch.zeros_like(lam)
+    reps = 50
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    a, b = q0p.clone(), torch.empty_like(q0p)
+    e[0].record(st)
+    for k in range(reps):
+        P._step(a, b, aux)
+        a, b = b, a
+    e[1].record(st)
+    e[2].record(st)
+    for k in range(reps):
+        P._adjoint_step(a, aux, lam, lam2, gaux)
+    e[3].record(st)
+    torch.cuda.synchronize()
+    us_fwd = e[0].elapsed_time(e[1]) * 1e3 / reps
+    us_adj = e[2].elapsed_time(e[3]) * 1e3 / reps
+    cells = mx * my
+    adj_bytes = 120.0 * cells          # read q 24 + aux 16 + lam 24, write lam 24, gaux r/w 32
+    ratio = (t_fwd_cp + t_bwd) / t_fwd
+    out = {"metric": "adjoint/fwd time", "value": ratio, "unit": "x", "n_gpus": 1, "steps": N, "warmup": N,
+           "ms_per_step": 1e3 * (t_fwd_cp + t_bwd) / N, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded smooth random media)",
+           "config": {"workload": "vc acoustics 512x512, %d steps, misfit gradient w.r.t. K, rho, q0; MC, "
+                                  "trans_inc, checkpoint every 45" % N, "global_batch": 1,
+                      "parallelism": "single GPU"},
+           "roofline": {"bound": "hbm", "achieved": adj_bytes / (us_adj * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": adj_bytes / (us_adj * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": None, "bytes_per_cell_update": 120, "kernel": "adjoint step (2 launches)"},
+           "extra": {"forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_cp, "reverse_s": t_bwd,
+                     "reverse_over_forward": t_bwd / t_fwd, "forward_step_us": us_fwd, "adjoint_step_us": us_adj,
+                     "adjoint_over_forward_step": us_adj / us_fwd, "J": J, "finite": finite,
+                     "checkpoints": len(cps)}}
+    print(json.dumps(out), flush=True)
+
+
+OTHER = {'batched': bench_batched, 'sharpclaw': bench_sharpclaw, 'acoustics': bench_acoustics,
+         'adjoint': bench_adjoint}
 
 
 if __name__ == '__main__':

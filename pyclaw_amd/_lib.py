@@ -75,6 +75,8 @@ def lib():
             fn = getattr(L, name, None)
             if fn is not None:
                 fn.argtypes = argtypes
+        if hasattr(L, "claw2d_vc_adjoint_workspace_bytes"):
+            L.claw2d_vc_adjoint_workspace_bytes.restype = ctypes.c_longlong
         _lib = L
     return _lib
 
@@ -90,6 +92,11 @@ def _optional_symbols():
         "claw2d_sharpclaw_ssp104_step": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann),
                                          ctypes.POINTER(ClawBC), ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, vp,
                                          vp],
+        "claw2d_vc_adjoint_workspace_bytes": [ctypes.POINTER(ClawGrid)],
+        "claw2d_vc_adjoint_step": [ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod), ctypes.POINTER(ClawBC),
+                                   vp, vp, vp, vp, vp, ctypes.c_double, vp, ctypes.c_longlong, vp],
+        "claw2d_gather": [vp, vp, ctypes.c_int, vp, vp],
+        "claw2d_misfit_source": [vp, vp, ctypes.c_int, vp, vp, vp],
     }
 
 

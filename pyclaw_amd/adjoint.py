@@ -1,0 +1,221 @@
+"""Gradients of a receiver-misfit objective through many classic steps of 2D
+variable-coefficient acoustics, by the discrete adjoint on the GPU.
+
+The reference has no adjoint; the forward steps it differentiates are the
+reference's unsplit step2 (classic/step2.f90:92-248) for rp2_vc_acoustics
+with aux = (rho, c) per cell (apps/acoustics_2d_variable/acoustics.py:25-26,
+46-61), at a fixed dt.
+
+    J(q0, K, rho) = 0.5 * sum_{n=1..N} sum_r (p_n(r) - p_obs[n-1, r])^2,
+    c = sqrt(K / rho)
+
+Forward: N steps through claw2d_classic_step, keeping every `every`-th state
+as a checkpoint and the pressure at the receiver cells after each step
+(claw2d_gather).  Reverse: for each checkpoint segment, last first, the
+segment's states are recomputed from its checkpoint and the adjoint steps
+(claw2d_vc_adjoint_step, hand-written HIP kernels) run backwards through it;
+each receiver residual enters lam at its cell (claw2d_misfit_source).
+Memory: ceil(N/every) checkpoints plus one segment of `every` states.
+
+VCAcousticsMisfit.apply(q0, K, rho) is the torch.autograd.Function form.
+"""
+import torch
+
+from . import _lib
+from .state import NG_STORE
+
+
+class MisfitProblem(object):
+    """Grid, method, boundary codes, receivers and observed traces of one
+    misfit objective.  receivers: sequence of (i, j) interior cells.
+    order/transverse/limiters as in ClawSolver2D (order 2 with limiters 0 is
+    unlimited Lax-Wendroff, the exactly linear case).  bc: q and aux boundary
+    codes per side, extrap (1), periodic (2) or wall (3)."""
+
+    def __init__(self, mx, my, dx, dy, dt, nsteps, receivers, p_obs=None, order=2, transverse=1, limiters=4,
+                 bc_lower=(1, 1), bc_upper=(1, 1), aux_bc_lower=(1, 1), aux_bc_upper=(1, 1), every=45,
+                 device=None):
+        if not torch.cuda.is_available():
+            raise _lib.ClawError("the adjoint runs on the GPU; no device is available")
+        self.L = _lib.lib()
+        for name in ("claw2d_vc_adjoint_step", "claw2d_gather", "claw2d_misfit_source"):
+            if not hasattr(self.L, name):
+                raise _lib.ClawError("libclawhip.so lacks %s: rebuild it" % name)
+        self.dev = torch.device(device) if device is not None else torch.device('cuda', 0)
+        self.mx, self.my, self.dx, self.dy, self.dt = int(mx), int(my), float(dx), float(dy), float(dt)
+        self.nsteps, self.every = int(nsteps), max(1, int(every))
+        ng = NG_STORE
+        self.ng, self.ldq = ng, self.mx + 2 * ng
+        self.rows = self.my + 2 * ng
+        self.plane = self.ldq * self.rows
+        self.grid = _lib.ClawGrid(self.mx, self.my, ng, 0, self.ldq, self.plane, self.dx, self.dy)
+        m = _lib.ClawMethod()
+        m.order, m.transverse, m.split, m.fwave, m.num_waves = int(order), int(transverse), 0, 0, 2
+        lim = limiters if isinstance(limiters, (list, tuple)) else [limiters] * 2
+        for k, v in enumerate(lim):
+            m.mthlim[k] = int(v)
+        self.method = m
+        self.rp = _lib.ClawRiemann()
+        self.rp.kind, self.rp.efix = _lib.RP_VC_ACOUSTICS, 1
+        bc = _lib.ClawBC()
+        for s in range(2):
+            bc.lower[s], bc.upper[s] = bc_lower[s], bc_upper[s]
+            bc.aux_lower[s], bc.aux_upper[s] = aux_bc_lower[s], aux_bc_upper[s]
+        self.bc = bc
+        rec = [(int(i), int(j)) for (i, j) in receivers]
+        for (i, j) in rec:
+            if not (0 <= i < self.mx and 0 <= j < self.my):
+                raise ValueError("receiver (%d, %d) outside the grid" % (i, j))
+        self.nrec = len(rec)
+        self.offsets = torch.tensor([(j + ng) * self.ldq + (i + ng) for (i, j) in rec], dtype=torch.int64,
+                                    device=self.dev)
+        self.p_obs = None if p_obs is None else torch.as_tensor(p_obs, dtype=torch.float64, device=self.dev)
+        self._ctx = _lib.Context(self.dev.index or 0)
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self._dt = torch.tensor([self.dt], **f64)
+        self._cfl = torch.zeros(1, **f64)
+        self._ws = torch.empty(int(self.L.claw2d_vc_adjoint_workspace_bytes(self.grid)) // 8, **f64)
+
+    # ---- layout helpers ----------------------------------------------------
+    def padded(self, a):
+        """(ncomp, mx, my) interior -> (ncomp, my+2ng, ldq) padded device tensor."""
+        a = torch.as_tensor(a, dtype=torch.float64, device=self.dev)
+        out = torch.zeros((a.shape[0], self.rows, self.ldq), dtype=torch.float64, device=self.dev)
+        out[:, self.ng:self.ng + self.my, self.ng:self.ng + self.mx] = a.permute(0, 2, 1)
+        return out
+
+    def interior(self, buf):
+        ng = self.ng
+        return buf[:, ng:ng + self.my, ng:ng + self.mx].permute(0, 2, 1)
+
+    def aux_from(self, K, rho):
+        K = torch.as_tensor(K, dtype=torch.float64, device=self.dev)
+        rho = torch.as_tensor(rho, dtype=torch.float64, device=self.dev)
+        return self.padded(torch.stack([rho, torch.sqrt(K / rho)]))
+
+    # ---- device calls ------------------------------------------------------
+    def _stream(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _step(self, qold, qnew, aux):
+        rc = self.L.claw2d_classic_step_ex(self._ctx.handle, self.grid, self.method, self.rp, self.bc, _lib.F64,
+                                           qold.data_ptr(), qnew.data_ptr(), aux.data_ptr(), self._dt.data_ptr(),
+                                           self._cfl.data_ptr(), _lib.STEP_ACCUMULATE_CFL, self._stream())
+        _lib.check(rc, "claw2d_classic_step_ex")
+
+    def _record(self, q, out):
+        _lib.check(self.L.claw2d_gather(q.data_ptr(), self.offsets.data_ptr(), self.nrec, out.data_ptr(),
+                                        self._stream()), "claw2d_gather")
+
+    def _source(self, lam, traces_n, obs_n):
+        _lib.check(self.L.claw2d_misfit_source(lam.data_ptr(), self.offsets.data_ptr(), self.nrec,
+                                               traces_n.data_ptr(), obs_n.data_ptr(), self._stream()),
+                   "claw2d_misfit_source")
+
+    def _adjoint_step(self, q, aux, lam_in, lam_out, gaux):
+        rc = self.L.claw2d_vc_adjoint_step(self.grid, self.method, self.bc, q.data_ptr(), aux.data_ptr(),
+                                           lam_in.data_ptr(), lam_out.data_ptr(), gaux.data_ptr(), self.dt,
+                                           self._ws.data_ptr(), self._ws.numel() * 8, self._stream())
+        _lib.check(rc, "claw2d_vc_adjoint_step")
+
+    # ---- passes ------------------------------------------------------------
+    def forward(self, q0p, aux, keep_checkpoints=True):
+        """N steps from the padded state q0p; returns (traces (N, nrec), checkpoints, final state)."""
+        N = self.nsteps
+        traces = torch.empty((N, self.nrec), dtype=torch.float64, device=self.dev)
+        cps = []
+        a, b = q0p.clone(), torch.empty_like(q0p)
+        self._cfl.zero_()
+        for n in range(N):
+            if keep_checkpoints and n % self.every == 0:
+                cps.append(a.clone())
+            self._step(a, b, aux)
+            self._record(b, traces[n])
+            a, b = b, a
+        return traces, cps, a
+
+    def backward(self, traces, cps, aux, p_obs=None):
+        """Adjoint sweep; returns (dJ/dq0 padded, dJ/d(rho, c) padded)."""
+        N, E = self.nsteps, self.every
+        obs = self.p_obs if p_obs is None else torch.as_tensor(p_obs, dtype=torch.float64, device=self.dev)
+        if obs is None:
+            raise ValueError("observed traces are needed")
+        lam = torch.zeros((3, self.rows, self.ldq), dtype=torch.float64, device=self.dev)
+        lam2 = torch.zeros_like(lam)
+        gaux = torch.zeros((2, self.rows, self.ldq), dtype=torch.float64, device=self.dev)
+        seg = torch.empty((E,) + tuple(lam.shape), dtype=torch.float64, device=self.dev)
+        self._source(lam, traces[N - 1], obs[N - 1])
+        nseg = (N + E - 1) // E
+        for k in range(nseg - 1, -1, -1):
+            s0, s1 = k * E, min((k + 1) * E, N)
+            seg[0].copy_(cps[k])
+            for n in range(s0, s1 - 1):
+                self._step(seg[n - s0], seg[n - s0 + 1], aux)
+            for n in range(s1 - 1, s0 - 1, -1):
+                self._adjoint_step(seg[n - s0], aux, lam, lam2, gaux)
+                lam, lam2 = lam2, lam
+                if n >= 1:
+                    self._source(lam, traces[n - 1], obs[n - 1])
+        return lam, gaux
+
+    def objective(self, traces, p_obs=None):
+        obs = self.p_obs if p_obs is None else torch.as_tensor(p_obs, dtype=torch.float64, device=self.dev)
+        return 0.5 * ((traces - obs) ** 2).sum()
+
+    def gradient(self, q0, K, rho, p_obs=None):
+        """(J, dJ/dq0 (3, mx, my), dJ/dK (mx, my), dJ/drho (mx, my)) at the given state and medium."""
+        aux = self.aux_from(K, rho)
+        traces, cps, _ = self.forward(self.padded(q0), aux)
+        J = self.objective(traces, p_obs)
+        lam, gaux = self.backward(traces, cps, aux, p_obs)
+        gq0 = self.interior(lam)
+        grho_a, gc = self.interior(gaux)[0], self.interior(gaux)[1]
+        K = torch.as_tensor(K, dtype=torch.float64, device=self.dev)
+        rho = torch.as_tensor(rho, dtype=torch.float64, device=self.dev)
+        c = torch.sqrt(K / rho)
+        # c = sqrt(K / rho): dc/dK = c / (2K), dc/drho = -c / (2 rho)
+        gK = gc * (c / (2.0 * K))
+        grho = grho_a - gc * (c / (2.0 * rho))
+        return J, gq0, gK, grho
+
+
+class VCAcousticsMisfit(torch.autograd.Function):
+    """J = VCAcousticsMisfit.apply(problem, q0, K, rho): the misfit of
+    MisfitProblem `problem` (which holds p_obs), differentiable in q0, K, rho."""
+
+    @staticmethod
+    def forward(ctx, problem, q0, K, rho):
+        aux = problem.aux_from(K.detach(), rho.detach())
+        traces, cps, _ = problem.forward(problem.padded(q0.detach()), aux)
+        ctx.problem = problem
+        ctx.saved = (traces, cps, aux)
+        ctx.save_for_backward(K, rho)
+        return problem.objective(traces)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        problem = ctx.problem
+        traces, cps, aux = ctx.saved
+        K, rho = ctx.saved_tensors
+        lam, gaux = problem.backward(traces, cps, aux)
+        gq0 = problem.interior(lam)
+        gaux_i = problem.interior(gaux)
+        Kd = K.detach().to(problem.dev, torch.float64)
+        rd = rho.detach().to(problem.dev, torch.float64)
+        c = torch.sqrt(Kd / rd)
+        gK = gaux_i[1] * (c / (2.0 * Kd))
+        grho = gaux_i[0] - gaux_i[1] * (c / (2.0 * rd))
+        ctx.saved = None
+        return None, (gJ * gq0).to(K.dtype), (gJ * gK).to(K.dtype), (gJ * grho).to(rho.dtype)
+
+
+def receivers_on_row(mx, my, y, nrec, lower=0.0, upper=1.0):
+    """nrec receiver cells evenly spread in x on the row whose centre is nearest y."""
+    dy = (upper - lower) / my
+    j = int(round((y - lower) / dy - 0.5))
+    j = min(max(j, 0), my - 1)
+    return [((2 * k + 1) * mx // (2 * nrec), j) for k in range(nrec)]
+
+
+def fixed_dt(dx, cmax, cfl=0.8):
+    return cfl * dx / cmax

@@ -1,0 +1,582 @@
+// Discrete adjoint of the unsplit classic step for variable-coefficient
+// acoustics (q = (p, u, v), aux = (rho, c) per cell), gfx950.
+//
+// Forward map (the reference's step2, classic/step2.f90:92-248, with flux2,
+// classic/flux2.f90:100-206, the limiter, classic/limiter.f90:31-61 and
+// philim.f90:18-55, and the vc-acoustics normal and transverse solvers; dt is
+// fixed): q_{n+1} = q_n + sum over interfaces of their contributions.  For an
+// x-interface (I, J) between cells lo = (I-1, J) and hi = (I, J), with
+// dtdn = dt/dx, gam = dt/dy * dt/(2 dx):
+//     lo: -dtdn A-dq - dtdn f          hi: -dtdn A+dq + dtdn f      (f = cq/2)
+//     column of lo (I-1) at rows J-1, J, J+1:  +gam bmA, gam (bpA - bmA), -gam bpA
+//     column of hi (I)   likewise with bmP, bpP
+// where A-dq = s1 W1, A+dq = s2 W2 (raw waves), cq = sum_p |s_p|(1 - |s_p| dtdn) W~_p
+// (limited waves), (bmA, bpA) = rpt(A-dq [+cq]) with the aux of column I-1 and
+// (bmP, bpP) = rpt(A+dq [-cq]) with the aux of column I.  y-interfaces are the
+// same with x and y exchanged.  Ghost values are functions of interior values
+// through the boundary codes (copies, negated normal velocity at walls).
+//
+// Reverse sweep, written by hand: given lam = dJ/dq_{n+1} (zero outside the
+// interior, since ghost cells of q_{n+1} are discarded), each interface turns
+// lam into the adjoints of its outputs, then goes back through the transverse
+// solver (transposed, plus its derivative with respect to the three aux cells),
+// the correction flux, the limiter (phi and its derivative with respect to the
+// ratio theta = W.W_upwind / W.W, which also reaches the upwind neighbour's raw
+// wave), the fluctuations and the normal solver (transposed, plus its aux
+// derivative).  The results are gathered per cell instead of scattered:
+// a thread owns one position, recomputes every interface whose inputs include
+// that position and sums their contributions in a fixed order, so the sweep is
+// deterministic and needs no atomics.  Ghost positions are handled by a first
+// launch that writes their raw contributions to a workspace; the cell launch
+// adds each ghost's contribution to the interior cell it was copied from.
+//
+// Requirements: c > 0 in every cell (so wave 1 moves left and is limited with
+// the interface to its right, wave 2 with the interface to its left); boundary
+// codes extrap, periodic or wall (stored ghost cells are treated as constants).
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace clawhip {
+
+namespace adj {
+
+constexpr int ANG = 2;   // ghost width of the classic step
+
+struct Params {
+    const double *q;      // q_n, padded planar layout
+    const double *aux;    // (rho, c), padded planar layout
+    const double *lam;    // lam_{n+1}, interior cells read only
+    double *lam_out;      // lam_n
+    double *gaux;         // accumulated dJ/d(rho, c)
+    double *frame;        // workspace: 5 planes, ghost positions only
+    GridDev g;
+    BCDev bq, ba;
+    double dtdx, dtdy;
+    int order, trans;
+    int mth[2];
+};
+
+struct CellQ { double p, un, ut; };          // pressure, normal and transverse velocity
+struct CellA { double rho, c; };
+
+// (n, t) -> (i, j): DIR 0 normal = x, DIR 1 normal = y
+template <int DIR> __host__ __device__ __forceinline__ int ci(int n, int t) { return DIR == 0 ? n : t; }
+template <int DIR> __host__ __device__ __forceinline__ int cj(int n, int t) { return DIR == 0 ? t : n; }
+
+template <int DIR>
+__host__ __device__ __forceinline__ CellQ getq(const Params &P, int n, int t) {
+    const int i = ci<DIR>(n, t), j = cj<DIR>(n, t);
+    SrcCell s = bc_source(i, j, P.g, P.bq);
+    double u = P.q[1 * P.g.plane + s.off], v = P.q[2 * P.g.plane + s.off];
+    if (s.fx) u = -u;
+    if (s.fy) v = -v;
+    CellQ r;
+    r.p = P.q[s.off];
+    r.un = DIR == 0 ? u : v;
+    r.ut = DIR == 0 ? v : u;
+    return r;
+}
+
+template <int DIR>
+__host__ __device__ __forceinline__ CellA geta(const Params &P, int n, int t) {
+    const int i = ci<DIR>(n, t), j = cj<DIR>(n, t);
+    SrcCell s = bc_source(i, j, P.g, P.ba);
+    CellA r;
+    r.rho = P.aux[s.off];
+    r.c = P.aux[P.g.plane + s.off];
+    return r;
+}
+
+// lam_{n+1} in (p, normal, transverse) components; zero outside the interior
+template <int DIR>
+__host__ __device__ __forceinline__ CellQ getl(const Params &P, int n, int t) {
+    const int i = ci<DIR>(n, t), j = cj<DIR>(n, t);
+    CellQ r{0.0, 0.0, 0.0};
+    if (i < 0 || j < 0 || i >= P.g.mx || j >= P.g.my) return r;
+    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
+    const double u = P.lam[P.g.plane + off], v = P.lam[2 * P.g.plane + off];
+    r.p = P.lam[off];
+    r.un = DIR == 0 ? u : v;
+    r.ut = DIR == 0 ? v : u;
+    return r;
+}
+
+// raw waves of interface N (between cells N-1 and N) in transverse line T
+struct Raw {
+    double w1p, w1u, w2p, w2u;   // W1 = (w1p, w1u, 0), W2 = (w2p, w2u, 0)
+    double s1, s2;               // -c(lo), c(hi)
+    double a1, a2, d1, D;        // wave strengths, normal-velocity jump, zlo + zhi
+    double zlo, zhi;
+    CellA alo, ahi;
+};
+
+template <int DIR>
+__host__ __device__ __forceinline__ Raw raw(const Params &P, int N, int T) {
+    const CellQ ql = getq<DIR>(P, N - 1, T), qh = getq<DIR>(P, N, T);
+    Raw r;
+    r.alo = geta<DIR>(P, N - 1, T);
+    r.ahi = geta<DIR>(P, N, T);
+    const double d0 = qh.p - ql.p;
+    const double d1 = qh.un - ql.un;
+    r.zhi = r.ahi.rho * r.ahi.c;
+    r.zlo = r.alo.rho * r.alo.c;
+    r.D = r.zlo + r.zhi;
+    r.a1 = (-d0 + r.zhi * d1) / r.D;
+    r.a2 = (d0 + r.zlo * d1) / r.D;
+    r.d1 = d1;
+    r.w1p = -r.a1 * r.zlo; r.w1u = r.a1;
+    r.w2p = r.a2 * r.zhi;  r.w2u = r.a2;
+    r.s1 = -r.alo.c; r.s2 = r.ahi.c;
+    return r;
+}
+
+// phi(theta) and dphi/dtheta with the forward's branch choices (philim.f90:18-55)
+__host__ __device__ __forceinline__ void philim_d(double r, int meth, double &phi, double &dphi) {
+    switch (meth) {
+    case 2: {   // superbee: max(max(0, min(1, 2r)), min(2, r))
+        const double A = 1.0 < 2.0 * r ? 1.0 : 2.0 * r, dA = 1.0 < 2.0 * r ? 0.0 : 2.0;
+        const double B = 2.0 < r ? 2.0 : r, dB = 2.0 < r ? 0.0 : 1.0;
+        const double t = 0.0 > A ? 0.0 : A, dt = 0.0 > A ? 0.0 : dA;
+        phi = t > B ? t : B; dphi = t > B ? dt : dB;
+        return;
+    }
+    case 3: {   // van Leer: (r + |r|) / (1 + |r|)
+        const double ar = __builtin_fabs(r), sg = r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : 0.0);
+        phi = (r + ar) / (1.0 + ar);
+        dphi = ((1.0 + sg) * (1.0 + ar) - (r + ar) * sg) / ((1.0 + ar) * (1.0 + ar));
+        return;
+    }
+    case 4: {   // MC: max(0, min(min((1+r)/2, 2), 2r))
+        const double c = (1.0 + r) / 2.0;
+        const double m1 = c < 2.0 ? c : 2.0, dm1 = c < 2.0 ? 0.5 : 0.0;
+        const double m2 = m1 < 2.0 * r ? m1 : 2.0 * r, dm2 = m1 < 2.0 * r ? dm1 : 2.0;
+        phi = 0.0 > m2 ? 0.0 : m2; dphi = 0.0 > m2 ? 0.0 : dm2;
+        return;
+    }
+    case 5: phi = r; dphi = 1.0; return;
+    default: {  // minmod (ids outside 1..5 as well): max(0, min(1, r))
+        const double m = 1.0 < r ? 1.0 : r, dm = 1.0 < r ? 0.0 : 1.0;
+        phi = 0.0 > m ? 0.0 : m; dphi = 0.0 > m ? 0.0 : dm;
+        return;
+    }
+    }
+}
+
+// limiter state of one wave (wp, wu) with its upwind neighbour (np, nu)
+struct Lim { double phi, dphi, inv, th; bool on; double wp, wu, np, nu; };
+
+__host__ __device__ __forceinline__ Lim limit(double wp, double wu, double np, double nu, int meth) {
+    Lim L;
+    L.wp = wp; L.wu = wu; L.np = np; L.nu = nu;
+    const double wn2 = wp * wp + wu * wu;
+    L.on = (meth != 0) && (wn2 != 0.0);
+    L.phi = 1.0; L.dphi = 0.0; L.inv = 0.0; L.th = 0.0;
+    if (L.on) {
+        const double dot = wp * np + wu * nu;
+        L.th = dot / wn2;
+        philim_d(L.th, meth, L.phi, L.dphi);
+        L.inv = 1.0 / wn2;
+    }
+    return L;
+}
+
+// transposed vc-acoustics transverse solve.  Forward (asdq t = (t0, ., tt)):
+//   a1 = (-t0 + tt zz)/(zm + zz),  a2 = (t0 + tt zz)/(zz + zp)
+//   bm = (cm a1 zm, 0, -cm a1),    bp = (cp a2 zp, 0, cp a2)
+// In: adjoints of bm, bp (p and transverse components).  Out: adjoint of t
+// (p and transverse components) and of (rho, c) of the three cells.
+struct RptRev { double gt0, gtt; double gm_rho, gm_c, g0_rho, g0_c, gp_rho, gp_c; };
+
+__host__ __device__ __forceinline__ RptRev rpt_rev(double t0, double tt, CellA am, CellA a0, CellA ap,
+                                          double gbm0, double gbmt, double gbp0, double gbpt) {
+    const double zm = am.rho * am.c, zz = a0.rho * a0.c, zp = ap.rho * ap.c;
+    const double D1 = zm + zz, D2 = zz + zp;
+    const double a1 = (-t0 + tt * zz) / D1, a2 = (t0 + tt * zz) / D2;
+    const double cm = am.c, cp = ap.c;
+    const double ga1 = cm * zm * gbm0 - cm * gbmt;
+    const double gcm = a1 * zm * gbm0 - a1 * gbmt;
+    double gzm = cm * a1 * gbm0;
+    const double ga2 = cp * zp * gbp0 + cp * gbpt;
+    const double gcp = a2 * zp * gbp0 + a2 * gbpt;
+    double gzp = cp * a2 * gbp0;
+    const double i1 = ga1 / D1, i2 = ga2 / D2;
+    RptRev r;
+    r.gt0 = -i1 + i2;
+    r.gtt = zz * (i1 + i2);
+    const double gzz = i1 * (tt - a1) + i2 * (tt - a2);
+    gzm -= i1 * a1;
+    gzp -= i2 * a2;
+    r.gm_rho = gzm * am.c; r.gm_c = gcm + gzm * am.rho;
+    r.g0_rho = gzz * a0.c; r.g0_c = gzz * a0.rho;
+    r.gp_rho = gzp * ap.c; r.gp_c = gcp + gzp * ap.rho;
+    return r;
+}
+
+// Everything one interface sends backwards, given its raw waves, its two
+// limiter states and lam around it.
+struct Own {
+    double gw1p, gw1u, gw2p, gw2u;     // own part of the adjoint of the raw waves
+    double push1p, push1u;             // to the raw wave 1 of interface N+1
+    double push2p, push2u;             // to the raw wave 2 of interface N-1
+    double gclo, gchi;                 // adjoints of c(lo), c(hi) through the speeds
+    RptRev rA, rP;                     // transverse solves (column N-1, column N)
+};
+
+template <int DIR>
+__host__ __device__ __forceinline__ Own own(const Params &P, int N, int T, const Raw &F, const Lim &L1, const Lim &L2) {
+    const double dtdn = DIR == 0 ? P.dtdx : P.dtdy;
+    const double dtdt = DIR == 0 ? P.dtdy : P.dtdx;
+    const CellQ lc = getl<DIR>(P, N - 1, T), hc = getl<DIR>(P, N, T);
+    // adjoints of A-dq, A+dq (p, normal, transverse) and of cq
+    double gAm0 = -dtdn * lc.p, gAmu = -dtdn * lc.un, gAmt = -dtdn * lc.ut;
+    double gAp0 = -dtdn * hc.p, gApu = -dtdn * hc.un, gApt = -dtdn * hc.ut;
+    double gq0 = 0.0, gqu = 0.0;
+    if (P.order == 2) { gq0 = 0.5 * dtdn * (hc.p - lc.p); gqu = 0.5 * dtdn * (hc.un - lc.un); }
+    // limited waves and cq (needed by trans_cor's transverse inputs)
+    const double k1 = -F.s1 * (1.0 - (-F.s1) * dtdn), k2 = F.s2 * (1.0 - F.s2 * dtdn);
+    const double l1p = L1.phi * F.w1p, l1u = L1.phi * F.w1u, l2p = L2.phi * F.w2p, l2u = L2.phi * F.w2u;
+    Own o;
+    o.rA = RptRev{0, 0, 0, 0, 0, 0, 0, 0};
+    o.rP = RptRev{0, 0, 0, 0, 0, 0, 0, 0};
+    if (P.trans > 0) {
+        const double gam = dtdt * (0.5 * dtdn);
+        const CellQ lm = getl<DIR>(P, N - 1, T - 1), lp = getl<DIR>(P, N - 1, T + 1);
+        const CellQ hm = getl<DIR>(P, N, T - 1), hp = getl<DIR>(P, N, T + 1);
+        double tA0 = F.s1 * F.w1p, tP0 = F.s2 * F.w2p;
+        if (P.order == 2 && P.trans == 2) {
+            const double cq0 = k1 * l1p + k2 * l2p;
+            tA0 = tA0 + cq0; tP0 = tP0 - cq0;
+        }
+        o.rA = rpt_rev(tA0, 0.0, geta<DIR>(P, N - 1, T - 1), F.alo, geta<DIR>(P, N - 1, T + 1),
+                       gam * (lm.p - lc.p), gam * (lm.ut - lc.ut), gam * (lc.p - lp.p), gam * (lc.ut - lp.ut));
+        o.rP = rpt_rev(tP0, 0.0, geta<DIR>(P, N, T - 1), F.ahi, geta<DIR>(P, N, T + 1),
+                       gam * (hm.p - hc.p), gam * (hm.ut - hc.ut), gam * (hc.p - hp.p), gam * (hc.ut - hp.ut));
+        gAm0 += o.rA.gt0; gAmt += o.rA.gtt;
+        gAp0 += o.rP.gt0; gApt += o.rP.gtt;
+        if (P.order == 2 && P.trans == 2) { gq0 += o.rA.gt0 - o.rP.gt0; }
+    }
+    (void)gAmt; (void)gApt;   // the waves have no transverse component
+    // fluctuations: A-dq = s1 W1, A+dq = s2 W2
+    o.gw1p = F.s1 * gAm0; o.gw1u = F.s1 * gAmu;
+    o.gw2p = F.s2 * gAp0; o.gw2u = F.s2 * gApu;
+    o.gclo = -(gAm0 * F.w1p + gAmu * F.w1u);
+    o.gchi = gAp0 * F.w2p + gApu * F.w2u;
+    o.push1p = o.push1u = o.push2p = o.push2u = 0.0;
+    if (P.order == 2) {
+        // cq = k1 W~1 + k2 W~2, k = |s| (1 - |s| dtdn)
+        const double g1p = k1 * gq0, g1u = k1 * gqu, g2p = k2 * gq0, g2u = k2 * gqu;
+        o.gclo += (1.0 - 2.0 * (-F.s1) * dtdn) * (gq0 * l1p + gqu * l1u);
+        o.gchi += (1.0 - 2.0 * F.s2 * dtdn) * (gq0 * l2p + gqu * l2u);
+        // W~ = phi(theta) W, theta = W.Wn / W.W
+        o.gw1p += L1.phi * g1p; o.gw1u += L1.phi * g1u;
+        o.gw2p += L2.phi * g2p; o.gw2u += L2.phi * g2u;
+        if (L1.on) {
+            const double e = (g1p * L1.wp + g1u * L1.wu) * L1.dphi * L1.inv;
+            o.gw1p += e * (L1.np - 2.0 * L1.th * L1.wp); o.gw1u += e * (L1.nu - 2.0 * L1.th * L1.wu);
+            o.push1p = e * L1.wp; o.push1u = e * L1.wu;
+        }
+        if (L2.on) {
+            const double e = (g2p * L2.wp + g2u * L2.wu) * L2.dphi * L2.inv;
+            o.gw2p += e * (L2.np - 2.0 * L2.th * L2.wp); o.gw2u += e * (L2.nu - 2.0 * L2.th * L2.wu);
+            o.push2p = e * L2.wp; o.push2u = e * L2.wu;
+        }
+    }
+    return o;
+}
+
+// transposed normal solve: adjoints of (d0, d1) and of zlo, zhi from the
+// adjoint of the raw waves
+struct RpnRev { double gd0, gd1, gzlo, gzhi; };
+
+__host__ __device__ __forceinline__ RpnRev rpn_rev(const Raw &F, double g1p, double g1u, double g2p, double g2u) {
+    const double ga1 = -F.zlo * g1p + g1u;
+    const double ga2 = F.zhi * g2p + g2u;
+    const double i1 = ga1 / F.D, i2 = ga2 / F.D;
+    RpnRev r;
+    r.gd0 = -i1 + i2;
+    r.gd1 = F.zhi * i1 + F.zlo * i2;
+    r.gzhi = F.a2 * g2p + i1 * (F.d1 - F.a1) - i2 * F.a2;
+    r.gzlo = -F.a1 * g1p - i1 * F.a1 + i2 * (F.d1 - F.a2);
+    return r;
+}
+
+// The limiter push of one wave of interface N to its upwind neighbour, as
+// the scalar e with push = e * W (wave 0: wave 1, else wave 2):
+// e = (adjoint of the limited wave . W) phi'(theta) / W.W.
+template <int DIR>
+__host__ __device__ __forceinline__ double cq_adjoint(const Params &P, int N, int T, const Raw &F, const Lim &L1,
+                                             const Lim &L2, int wave) {
+    const double dtdn = DIR == 0 ? P.dtdx : P.dtdy;
+    const CellQ lc = getl<DIR>(P, N - 1, T), hc = getl<DIR>(P, N, T);
+    double gq0 = 0.5 * dtdn * (hc.p - lc.p);
+    const double gqu = 0.5 * dtdn * (hc.un - lc.un);
+    const double k1 = -F.s1 * (1.0 - (-F.s1) * dtdn), k2 = F.s2 * (1.0 - F.s2 * dtdn);
+    if (P.trans == 2) {
+        // trans_cor: the transverse solves take A-dq + cq and A+dq - cq
+        const double dtdt = DIR == 0 ? P.dtdy : P.dtdx;
+        const double gam = dtdt * (0.5 * dtdn);
+        const CellQ lm = getl<DIR>(P, N - 1, T - 1), lp = getl<DIR>(P, N - 1, T + 1);
+        const CellQ hm = getl<DIR>(P, N, T - 1), hp = getl<DIR>(P, N, T + 1);
+        const double cq0 = k1 * (L1.phi * F.w1p) + k2 * (L2.phi * F.w2p);
+        const RptRev rA = rpt_rev(F.s1 * F.w1p + cq0, 0.0, geta<DIR>(P, N - 1, T - 1), F.alo,
+                                  geta<DIR>(P, N - 1, T + 1), gam * (lm.p - lc.p), gam * (lm.ut - lc.ut),
+                                  gam * (lc.p - lp.p), gam * (lc.ut - lp.ut));
+        const RptRev rP = rpt_rev(F.s2 * F.w2p - cq0, 0.0, geta<DIR>(P, N, T - 1), F.ahi, geta<DIR>(P, N, T + 1),
+                                  gam * (hm.p - hc.p), gam * (hm.ut - hc.ut), gam * (hc.p - hp.p),
+                                  gam * (hc.ut - hp.ut));
+        gq0 += rA.gt0 - rP.gt0;
+    }
+    const Lim &L = wave == 0 ? L1 : L2;
+    const double k = wave == 0 ? k1 : k2;
+    return (k * gq0 * L.wp + k * gqu * L.wu) * L.dphi * L.inv;
+}
+
+// Transverse-solve aux partial at position (gn, gt) from interfaces of the
+// lines gt-1 and gt+1: rptP of interface (gn, T) and rptA of (gn+1, T) use
+// the column gn, with (gn, gt) as the cell above (T = gt-1) or below (T = gt+1).
+template <int DIR>
+__host__ __device__ __forceinline__ void rpt_side(const Params &P, int gn, int gt, double &grho, double &gc) {
+    const double dtdn = DIR == 0 ? P.dtdx : P.dtdy;
+    const double dtdt = DIR == 0 ? P.dtdy : P.dtdx;
+    const double gam = dtdt * (0.5 * dtdn);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int T = side == 0 ? gt - 1 : gt + 1;
+        const CellQ lm = getl<DIR>(P, gn, T - 1), lc = getl<DIR>(P, gn, T), lp = getl<DIR>(P, gn, T + 1);
+        const double gbm0 = gam * (lm.p - lc.p), gbmt = gam * (lm.ut - lc.ut);
+        const double gbp0 = gam * (lc.p - lp.p), gbpt = gam * (lc.ut - lp.ut);
+        const CellA am = geta<DIR>(P, gn, T - 1), a0 = geta<DIR>(P, gn, T), ap = geta<DIR>(P, gn, T + 1);
+        // interface (gn, T): A+dq enters column gn; interface (gn+1, T): A-dq enters column gn
+#pragma unroll
+        for (int which = 0; which < 2; ++which) {
+            const int N = which == 0 ? gn : gn + 1;
+            const Raw F = raw<DIR>(P, N, T);
+            double t0 = which == 0 ? F.s2 * F.w2p : F.s1 * F.w1p;
+            if (P.order == 2 && P.trans == 2) {
+                const Raw Fm = raw<DIR>(P, N - 1, T), Fp = raw<DIR>(P, N + 1, T);
+                const Lim L1 = limit(F.w1p, F.w1u, Fp.w1p, Fp.w1u, P.mth[0]);
+                const Lim L2 = limit(F.w2p, F.w2u, Fm.w2p, Fm.w2u, P.mth[1]);
+                const double k1 = -F.s1 * (1.0 - (-F.s1) * dtdn), k2 = F.s2 * (1.0 - F.s2 * dtdn);
+                const double cq0 = k1 * (L1.phi * F.w1p) + k2 * (L2.phi * F.w2p);
+                t0 = which == 0 ? t0 - cq0 : t0 + cq0;
+            }
+            const RptRev r = rpt_rev(t0, 0.0, am, a0, ap, gbm0, gbmt, gbp0, gbpt);
+            if (side == 0) { grho += r.gp_rho; gc += r.gp_c; }
+            else { grho += r.gm_rho; gc += r.gm_c; }
+        }
+    }
+}
+
+// Raw adjoint contributions at position (i, j) (interior or ghost) from every
+// interface normal to DIR whose inputs include that position.
+template <int DIR>
+__host__ __device__ __forceinline__ void position_dir(const Params &P, int i, int j, double (&gq)[3], double &grho,
+                                             double &gc) {
+    const int gn = DIR == 0 ? i : j, gt = DIR == 0 ? j : i;
+    // interfaces gn-1 .. gn+2 along the line gt
+    const Raw R0 = raw<DIR>(P, gn - 1, gt), R1 = raw<DIR>(P, gn, gt);
+    const Raw R2 = raw<DIR>(P, gn + 1, gt), R3 = raw<DIR>(P, gn + 2, gt);
+    // wave 1 is limited with the interface to its right, wave 2 with the one to its left
+    const Lim L1_1 = limit(R1.w1p, R1.w1u, R2.w1p, R2.w1u, P.mth[0]);
+    const Lim L2_1 = limit(R1.w2p, R1.w2u, R0.w2p, R0.w2u, P.mth[1]);
+    const Lim L1_2 = limit(R2.w1p, R2.w1u, R3.w1p, R3.w1u, P.mth[0]);
+    const Lim L2_2 = limit(R2.w2p, R2.w2u, R1.w2p, R1.w2u, P.mth[1]);
+    const Own O1 = own<DIR>(P, gn, gt, R1, L1_1, L2_1);
+    const Own O2 = own<DIR>(P, gn + 1, gt, R2, L1_2, L2_2);
+    double pin1p = 0.0, pin1u = 0.0, pin2p = 0.0, pin2u = 0.0;
+    if (P.order == 2) {
+        // wave 1 of interface gn-1 limited with gn; wave 2 of gn+2 with gn+1
+        const Raw Rm = raw<DIR>(P, gn - 2, gt), Rp = raw<DIR>(P, gn + 3, gt);
+        const Lim L1_0 = limit(R0.w1p, R0.w1u, R1.w1p, R1.w1u, P.mth[0]);
+        const Lim L2_0 = limit(R0.w2p, R0.w2u, Rm.w2p, Rm.w2u, P.mth[1]);
+        const Lim L1_3 = limit(R3.w1p, R3.w1u, Rp.w1p, Rp.w1u, P.mth[0]);
+        const Lim L2_3 = limit(R3.w2p, R3.w2u, R2.w2p, R2.w2u, P.mth[1]);
+        if (L1_0.on) {
+            const double g = cq_adjoint<DIR>(P, gn - 1, gt, R0, L1_0, L2_0, 0);
+            pin1p = g * L1_0.wp; pin1u = g * L1_0.wu;
+        }
+        if (L2_3.on) {
+            const double g = cq_adjoint<DIR>(P, gn + 2, gt, R3, L1_3, L2_3, 1);
+            pin2p = g * L2_3.wp; pin2u = g * L2_3.wu;
+        }
+    }
+    // interface gn: the position is its upper/right cell
+    {
+        const RpnRev r = rpn_rev(R1, O1.gw1p + pin1p, O1.gw1u + pin1u, O1.gw2p + O2.push2p, O1.gw2u + O2.push2u);
+        gq[0] += r.gd0;
+        gq[DIR == 0 ? 1 : 2] += r.gd1;
+        grho += r.gzhi * R1.ahi.c;
+        gc += r.gzhi * R1.ahi.rho + O1.gchi + O1.rP.g0_c;
+        grho += O1.rP.g0_rho;
+    }
+    // interface gn+1: the position is its lower/left cell
+    {
+        const RpnRev r = rpn_rev(R2, O2.gw1p + O1.push1p, O2.gw1u + O1.push1u, O2.gw2p + pin2p, O2.gw2u + pin2u);
+        gq[0] -= r.gd0;
+        gq[DIR == 0 ? 1 : 2] -= r.gd1;
+        grho += r.gzlo * R2.alo.c;
+        gc += r.gzlo * R2.alo.rho + O2.gclo + O2.rA.g0_c;
+        grho += O2.rA.g0_rho;
+    }
+    if (P.trans > 0) rpt_side<DIR>(P, gn, gt, grho, gc);
+}
+
+__host__ __device__ __forceinline__ void position_raw(const Params &P, int i, int j, double (&gq)[3], double &grho,
+                                             double &gc) {
+    gq[0] = gq[1] = gq[2] = 0.0;
+    grho = 0.0; gc = 0.0;
+    position_dir<0>(P, i, j, gq, grho, gc);
+    position_dir<1>(P, i, j, gq, grho, gc);
+}
+
+// ghost positions of the 2-wide frame: (mx+4)(my+4) - mx*my of them
+__host__ __device__ __forceinline__ long long frame_count(const Params &P) {
+    const long long W = P.g.mx + 2 * ANG;
+    return W * (P.g.my + 2 * ANG) - (long long)P.g.mx * P.g.my;
+}
+
+__host__ __device__ __forceinline__ void frame_one(const Params &P, long long k) {
+    const int mx = P.g.mx, my = P.g.my;
+    const int W = mx + 2 * ANG;
+    int i, j;
+    // rows -2, -1 and my, my+1 in full width, then the side columns of the interior rows
+    const long long band = 2LL * ANG * W;
+    if (k < band) {
+        const int r = (int)(k / W), c = (int)(k % W);
+        j = r < ANG ? r - ANG : my + (r - ANG);
+        i = c - ANG;
+    } else {
+        const long long kk = k - band;
+        const int r = (int)(kk / (2 * ANG)), c = (int)(kk % (2 * ANG));
+        j = r;
+        i = c < ANG ? c - ANG : mx + (c - ANG);
+    }
+    double gq[3], grho, gc;
+    position_raw(P, i, j, gq, grho, gc);
+    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
+    P.frame[off] = gq[0];
+    P.frame[P.g.plane + off] = gq[1];
+    P.frame[2 * P.g.plane + off] = gq[2];
+    P.frame[3 * P.g.plane + off] = grho;
+    P.frame[4 * P.g.plane + off] = gc;
+}
+
+__host__ __device__ __forceinline__ void cell_one(const Params &P, int i, int j) {
+    const int mx = P.g.mx, my = P.g.my;
+    double gq[3], grho, gc;
+    position_raw(P, i, j, gq, grho, gc);
+    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
+    // ghosts copied from this cell: their raw contributions fold back here
+    if (i < ANG || j < ANG || i >= mx - ANG || j >= my - ANG) {
+        const int gis[5] = {i, -2, -1, mx, mx + 1};
+        const int gjs[5] = {j, -2, -1, my, my + 1};
+        for (int b = 0; b < 5; ++b) {
+            for (int a = 0; a < 5; ++a) {
+                if (a == 0 && b == 0) continue;
+                const int gi = gis[a], gj = gjs[b];
+                const long long goff = (long long)(gj + P.g.ng) * P.g.ldq + (gi + P.g.ng);
+                const SrcCell sq = bc_source(gi, gj, P.g, P.bq);
+                if (sq.off == off) {
+                    gq[0] += P.frame[goff];
+                    gq[1] += sq.fx ? -P.frame[P.g.plane + goff] : P.frame[P.g.plane + goff];
+                    gq[2] += sq.fy ? -P.frame[2 * P.g.plane + goff] : P.frame[2 * P.g.plane + goff];
+                }
+                const SrcCell sa = bc_source(gi, gj, P.g, P.ba);
+                if (sa.off == off) {
+                    grho += P.frame[3 * P.g.plane + goff];
+                    gc += P.frame[4 * P.g.plane + goff];
+                }
+            }
+        }
+    }
+    for (int m = 0; m < 3; ++m) P.lam_out[m * P.g.plane + off] = P.lam[m * P.g.plane + off] + gq[m];
+    P.gaux[off] += grho;
+    P.gaux[P.g.plane + off] += gc;
+}
+
+__global__ __launch_bounds__(256) void adjoint_frame_kernel(const Params P) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < frame_count(P)) frame_one(P, k);
+}
+
+__global__ __launch_bounds__(256) void adjoint_cell_kernel(const Params P) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i < P.g.mx && j < P.g.my) cell_one(P, i, j);
+}
+
+}  // namespace adj
+
+// Receiver traces and their adjoint sources: out[k] = a[off[k]]; a[off[k]] += v[k] - w[k].
+__global__ void gather_kernel(const double *a, const long long *off, int n, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = a[off[k]];
+}
+
+__global__ void misfit_source_kernel(double *a, const long long *off, int n, const double *v, const double *w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[off[k]] += v[k] - w[k];
+}
+
+}  // namespace clawhip
+
+using namespace clawhip;
+
+extern "C" {
+
+long long claw2d_vc_adjoint_workspace_bytes(const ClawGrid *grid) {
+    if (!grid) return -1;
+    return 5LL * grid->plane * (long long)sizeof(double);
+}
+
+int claw2d_vc_adjoint_step(const ClawGrid *grid, const ClawMethod *meth, const ClawBC *bc, const double *q,
+                           const double *aux, const double *lam_in, double *lam_out, double *gaux, double dt,
+                           void *workspace, long long workspace_bytes, void *stream) {
+    if (!grid || !meth || !bc || !q || !aux || !lam_in || !lam_out || !gaux || !workspace) return CLAW_EINVAL;
+    if (lam_in == lam_out) return CLAW_EINVAL;
+    if (grid->mx < 2 || grid->my < 2 || grid->ng < adj::ANG) return CLAW_EINVAL;
+    if (grid->ldq < grid->mx + 2 * grid->ng || grid->plane < grid->ldq * (grid->my + 2 * grid->ng)) return CLAW_EINVAL;
+    if (workspace_bytes < claw2d_vc_adjoint_workspace_bytes(grid)) return CLAW_EINVAL;
+    if (meth->split || meth->fwave || (meth->order != 1 && meth->order != 2)) return CLAW_EINVAL;
+    if (meth->transverse < 0 || meth->transverse > 2 || meth->num_waves != 2) return CLAW_EINVAL;
+    for (int s = 0; s < 2; ++s) {
+        const int codes[4] = {bc->lower[s], bc->upper[s], bc->aux_lower[s], bc->aux_upper[s]};
+        for (int c : codes)
+            if (c < CLAW_BC_EXTRAP || c > CLAW_BC_WALL) return CLAW_EINVAL;   // stored ghosts are not differentiated
+        if ((bc->lower[s] == CLAW_BC_PERIODIC) != (bc->upper[s] == CLAW_BC_PERIODIC)) return CLAW_EINVAL;
+        if ((bc->aux_lower[s] == CLAW_BC_PERIODIC) != (bc->aux_upper[s] == CLAW_BC_PERIODIC)) return CLAW_EINVAL;
+    }
+    adj::Params P;
+    P.q = q; P.aux = aux; P.lam = lam_in; P.lam_out = lam_out; P.gaux = gaux; P.frame = (double *)workspace;
+    P.g = GridDev{grid->mx, grid->my, grid->ng, grid->ldq, grid->plane};
+    P.bq = BCDev{{bc->lower[0], bc->lower[1]}, {bc->upper[0], bc->upper[1]}};
+    P.ba = BCDev{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
+    P.dtdx = dt / grid->dx; P.dtdy = dt / grid->dy;
+    P.order = meth->order; P.trans = meth->transverse;
+    P.mth[0] = meth->mthlim[0]; P.mth[1] = meth->mthlim[1];
+    hipStream_t st = (hipStream_t)stream;
+    const long long W = grid->mx + 2 * adj::ANG;
+    const long long nframe = W * (grid->my + 2 * adj::ANG) - (long long)grid->mx * grid->my;
+    hipLaunchKernelGGL(adj::adjoint_frame_kernel, dim3((unsigned)((nframe + 255) / 256)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(adj::adjoint_cell_kernel, dim3((grid->mx + 63) / 64, (grid->my + 3) / 4), dim3(256), 0, st, P);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+int claw2d_gather(const double *a, const long long *offsets, int n, double *out, void *stream) {
+    if (!a || !offsets || !out || n < 0) return CLAW_EINVAL;
+    if (n == 0) return CLAW_OK;
+    hipLaunchKernelGGL(gather_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, a, offsets, n, out);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+int claw2d_misfit_source(double *a, const long long *offsets, int n, const double *traces, const double *observed,
+                         void *stream) {
+    if (!a || !offsets || !traces || !observed || n < 0) return CLAW_EINVAL;
+    if (n == 0) return CLAW_OK;
+    hipLaunchKernelGGL(misfit_source_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, a, offsets,
+                       n, traces, observed);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+}  // extern "C"

@@ -30,11 +30,11 @@ struct BCDev {
     int lo[2], hi[2];
 };
 
-__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__host__ __device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Map a 1-D ghost index to its source (solver.py:386-434 for one side).
 // Returns the source index in [0, n-1]; sets flip for wall.
-__device__ __forceinline__ int bc_map1(int k, int n, int code, bool &flip) {
+__host__ __device__ __forceinline__ int bc_map1(int k, int n, int code, bool &flip) {
     int s;
     if (k < 0) {
         s = code == CLAW_BC_PERIODIC ? k + n : (code == CLAW_BC_WALL ? -1 - k : 0);
@@ -51,7 +51,7 @@ __device__ __forceinline__ int bc_map1(int k, int n, int code, bool &flip) {
 // clamped into the array (their results are never used).
 struct SrcCell { long long off; bool fx, fy; };
 
-__device__ __forceinline__ SrcCell bc_source(int i, int j, const GridDev &g, const BCDev &bc) {
+__host__ __device__ __forceinline__ SrcCell bc_source(int i, int j, const GridDev &g, const BCDev &bc) {
     SrcCell r;
     r.fx = false; r.fy = false;
     int ii = i, jj = j;

@@ -124,3 +124,34 @@ def layered_vc_acoustics(rng, nx, ny, nlayers=None, amp=1.0, width=0.05):
 
 def batched_sizes(rng, n, choices=tuple(range(64, 513, 32))):
     return [(int(rng.choice(choices)), int(rng.choice(choices))) for _ in range(n)]
+
+
+def smooth_random_field(rng, nx, ny, lo, hi, nmodes=16, kmax=4):
+    """Sum of nmodes random Fourier modes on [0,1]^2 (integer wave numbers
+    0..kmax per axis, uniform phases and amplitudes), rescaled to [lo, hi]."""
+    x, _ = centers(0.0, 1.0, nx)
+    y, _ = centers(0.0, 1.0, ny)
+    Y, X = np.meshgrid(y, x)
+    f = np.zeros((nx, ny))
+    for _ in range(nmodes):
+        kx, ky = rng.integers(0, kmax + 1, size=2)
+        f += rng.random() * np.cos(2.0 * np.pi * (kx * X + ky * Y) + 2.0 * np.pi * rng.random())
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-300)
+    return lo + (hi - lo) * f
+
+
+def adjoint_medium(rng, nx, ny, width=0.02, disc=0.15, scale=1.05):
+    """The adjoint workload's data on [0,1]^2: K in [1,4] and rho in [1,2]
+    (smooth random fields), a Gaussian pressure pulse exp(-r^2/width^2) at the
+    centre, and the perturbed K (times `scale` inside a centred disc of radius
+    `disc`) that produces the observed traces.  Returns q0, K, rho, K_true."""
+    K = smooth_random_field(rng, nx, ny, 1.0, 4.0)
+    rho = smooth_random_field(rng, nx, ny, 1.0, 2.0)
+    x, _ = centers(0.0, 1.0, nx)
+    y, _ = centers(0.0, 1.0, ny)
+    Y, X = np.meshgrid(y, x)
+    r2 = (X - 0.5) ** 2 + (Y - 0.5) ** 2
+    q0 = np.zeros((3, nx, ny))
+    q0[0] = np.exp(-r2 / width ** 2)
+    K_true = K * np.where(r2 <= disc ** 2, scale, 1.0)
+    return q0, K, rho, K_true
