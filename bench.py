@@ -13,6 +13,7 @@ Prints one JSON line (see the keys below).  --gpus N>1 is not supported: this
 build targets one GPU.
 """
 import argparse
+import ctypes
 import json
 import os
 import subprocess
@@ -37,6 +38,7 @@ def parse():
     ap.add_argument('--my', type=int, default=1024)
     ap.add_argument('--precision', default='f64', choices=['f64', 'f32'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-autotune', action='store_true', help="keep the default row-band height")
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
                     choices=['euler', 'batched', 'sharpclaw', 'acoustics', 'adjoint'],
@@ -122,6 +124,14 @@ def main():
                                       _lib.STEP_ACCUMULATE_CFL, sh)
         _lib.check(rc, 'claw2d_classic_step_ex')
 
+    band = None
+    if not args.no_autotune:
+        # pick the kernel's row-band height for this grid (answers do not depend on it)
+        hb = ctypes.c_int(0)
+        _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, prec, qa.data_ptr(), qb.data_ptr(), None,
+                                             dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
+                   'claw2d_classic_autotune')
+        band = hb.value
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
@@ -177,7 +187,7 @@ def main():
                      "kernel": "classic_unsplit_kernel<Euler5<%s>>" % ('double' if args.precision == 'f64' else 'float')},
         "cpu_baseline": cpu,
         "extra": {"cfl_max_over_timed_steps": cfl_max, "finite": finite, "dt": dt, "wall_s": wall,
-                  "hbm_pct": 100.0 * achieved / HBM_PEAK_GBS},
+                  "hbm_pct": 100.0 * achieved / HBM_PEAK_GBS, "band_rows": band},
     }
     print(json.dumps(out), flush=True)
 

@@ -106,6 +106,22 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
                            const void *qold, void *qnew, const void *aux,
                            const double *dt_dev, double *cfl_dev, int flags, void *stream);
 
+/* Autotuner of the unsplit step's launch shape: times the fused kernel for a
+   set of row-band heights (the rows one wavefront marches through; more
+   bands mean more wavefronts in flight, fewer mean fewer halo rows) and keeps
+   the fastest for this context and problem shape (Riemann solver, order,
+   transverse mode, precision, mx, my); later classic steps of that shape use
+   it.  Results do not depend on the band height (every cell is computed the
+   same way and the Courant maximum is order-free), so tuning never changes
+   answers.  Runs real steps from qold into qnew and max-combines into
+   *cfl_dev.  Blocks until the timing launches finish.  *band_out (optional)
+   receives the chosen height, 0 for the split step (not tuned).
+   No counterpart in the reference (its step has no launch shape). */
+int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *method,
+                            const ClawRiemann *rp, const ClawBC *bc, int precision,
+                            const void *qold, void *qnew, const void *aux,
+                            const double *dt_dev, double *cfl_dev, void *stream, int *band_out);
+
 /* Number of kernel launches one classic step issues (for timing). */
 int claw2d_classic_launches(const ClawMethod *method);
 

@@ -19,6 +19,8 @@ Memory: ceil(N/every) checkpoints plus one segment of `every` states.
 
 VCAcousticsMisfit.apply(q0, K, rho) is the torch.autograd.Function form.
 """
+import ctypes
+
 import torch
 
 from . import _lib
@@ -75,6 +77,7 @@ class MisfitProblem(object):
         self._dt = torch.tensor([self.dt], **f64)
         self._cfl = torch.zeros(1, **f64)
         self._ws = torch.empty(int(self.L.claw2d_vc_adjoint_workspace_bytes(self.grid)) // 8, **f64)
+        self._tuned = False
 
     # ---- layout helpers ----------------------------------------------------
     def padded(self, a):
@@ -119,9 +122,22 @@ class MisfitProblem(object):
         _lib.check(rc, "claw2d_vc_adjoint_step")
 
     # ---- passes ------------------------------------------------------------
+    def tune(self, q0p, aux):
+        """Pick the forward kernel's row-band height for this grid (claw2d_classic_autotune)."""
+        b = torch.empty_like(q0p)
+        band = ctypes.c_int(0)
+        _lib.check(self.L.claw2d_classic_autotune(self._ctx.handle, self.grid, self.method, self.rp, self.bc,
+                                                  _lib.F64, q0p.data_ptr(), b.data_ptr(), aux.data_ptr(),
+                                                  self._dt.data_ptr(), self._cfl.data_ptr(), self._stream(),
+                                                  ctypes.byref(band)), "claw2d_classic_autotune")
+        self._tuned = True
+        return band.value
+
     def forward(self, q0p, aux, keep_checkpoints=True):
         """N steps from the padded state q0p; returns (traces (N, nrec), checkpoints, final state)."""
         N = self.nsteps
+        if not self._tuned:
+            self.tune(q0p, aux)
         traces = torch.empty((N, self.nrec), dtype=torch.float64, device=self.dev)
         cps = []
         a, b = q0p.clone(), torch.empty_like(q0p)

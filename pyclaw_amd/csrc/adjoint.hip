@@ -163,20 +163,32 @@ __host__ __device__ __forceinline__ void philim_d(double r, int meth, double &ph
     }
 }
 
-// limiter state of one wave (wp, wu) with its upwind neighbour (np, nu)
-struct Lim { double phi, dphi, inv, th; bool on; double wp, wu, np, nu; };
+// limiter state of one wave (wp, wu) with its upwind neighbour (np, nu).
+// The derivative terms are kept in scaled form, W = s u with s = max|W_k|,
+// so that 1/|W|^2 is never formed: tiny wave tails (|W|^2 subnormal) would
+// otherwise make it overflow.  With e_s = (g . u) phi' / |u|^2:
+//   dtheta/dW contribution  e_s (Wn/s - 2 theta u),   push to Wn  e_s u.
+struct Lim { double phi, dphi, th; bool on, d; double up, uu, nps, nus, un2; };
 
 __host__ __device__ __forceinline__ Lim limit(double wp, double wu, double np, double nu, int meth) {
     Lim L;
-    L.wp = wp; L.wu = wu; L.np = np; L.nu = nu;
     const double wn2 = wp * wp + wu * wu;
     L.on = (meth != 0) && (wn2 != 0.0);
-    L.phi = 1.0; L.dphi = 0.0; L.inv = 0.0; L.th = 0.0;
+    L.d = false;
+    L.phi = 1.0; L.dphi = 0.0; L.th = 0.0;
+    L.up = L.uu = L.nps = L.nus = 0.0; L.un2 = 1.0;
     if (L.on) {
         const double dot = wp * np + wu * nu;
         L.th = dot / wn2;
         philim_d(L.th, meth, L.phi, L.dphi);
-        L.inv = 1.0 / wn2;
+        L.d = L.dphi != 0.0;
+        if (L.d) {
+            const double aw = __builtin_fabs(wp), bw = __builtin_fabs(wu);
+            const double is = 1.0 / (aw > bw ? aw : bw);
+            L.up = wp * is; L.uu = wu * is;
+            L.nps = np * is; L.nus = nu * is;
+            L.un2 = L.up * L.up + L.uu * L.uu;
+        }
     }
     return L;
 }
@@ -271,15 +283,15 @@ __host__ __device__ __forceinline__ Own own(const Params &P, int N, int T, const
         // W~ = phi(theta) W, theta = W.Wn / W.W
         o.gw1p += L1.phi * g1p; o.gw1u += L1.phi * g1u;
         o.gw2p += L2.phi * g2p; o.gw2u += L2.phi * g2u;
-        if (L1.on) {
-            const double e = (g1p * L1.wp + g1u * L1.wu) * L1.dphi * L1.inv;
-            o.gw1p += e * (L1.np - 2.0 * L1.th * L1.wp); o.gw1u += e * (L1.nu - 2.0 * L1.th * L1.wu);
-            o.push1p = e * L1.wp; o.push1u = e * L1.wu;
+        if (L1.d) {
+            const double e = (g1p * L1.up + g1u * L1.uu) * L1.dphi / L1.un2;
+            o.gw1p += e * (L1.nps - 2.0 * L1.th * L1.up); o.gw1u += e * (L1.nus - 2.0 * L1.th * L1.uu);
+            o.push1p = e * L1.up; o.push1u = e * L1.uu;
         }
-        if (L2.on) {
-            const double e = (g2p * L2.wp + g2u * L2.wu) * L2.dphi * L2.inv;
-            o.gw2p += e * (L2.np - 2.0 * L2.th * L2.wp); o.gw2u += e * (L2.nu - 2.0 * L2.th * L2.wu);
-            o.push2p = e * L2.wp; o.push2u = e * L2.wu;
+        if (L2.d) {
+            const double e = (g2p * L2.up + g2u * L2.uu) * L2.dphi / L2.un2;
+            o.gw2p += e * (L2.nps - 2.0 * L2.th * L2.up); o.gw2u += e * (L2.nus - 2.0 * L2.th * L2.uu);
+            o.push2p = e * L2.up; o.push2u = e * L2.uu;
         }
     }
     return o;
@@ -302,8 +314,8 @@ __host__ __device__ __forceinline__ RpnRev rpn_rev(const Raw &F, double g1p, dou
 }
 
 // The limiter push of one wave of interface N to its upwind neighbour, as
-// the scalar e with push = e * W (wave 0: wave 1, else wave 2):
-// e = (adjoint of the limited wave . W) phi'(theta) / W.W.
+// the scalar e_s with push = e_s * u, W = s u (wave 0: wave 1, else wave 2):
+// e_s = (adjoint of the limited wave . u) phi'(theta) / u.u.
 template <int DIR>
 __host__ __device__ __forceinline__ double cq_adjoint(const Params &P, int N, int T, const Raw &F, const Lim &L1,
                                              const Lim &L2, int wave) {
@@ -329,7 +341,7 @@ __host__ __device__ __forceinline__ double cq_adjoint(const Params &P, int N, in
     }
     const Lim &L = wave == 0 ? L1 : L2;
     const double k = wave == 0 ? k1 : k2;
-    return (k * gq0 * L.wp + k * gqu * L.wu) * L.dphi * L.inv;
+    return (k * gq0 * L.up + k * gqu * L.uu) * L.dphi / L.un2;
 }
 
 // Transverse-solve aux partial at position (gn, gt) from interfaces of the
@@ -392,13 +404,13 @@ __host__ __device__ __forceinline__ void position_dir(const Params &P, int i, in
         const Lim L2_0 = limit(R0.w2p, R0.w2u, Rm.w2p, Rm.w2u, P.mth[1]);
         const Lim L1_3 = limit(R3.w1p, R3.w1u, Rp.w1p, Rp.w1u, P.mth[0]);
         const Lim L2_3 = limit(R3.w2p, R3.w2u, R2.w2p, R2.w2u, P.mth[1]);
-        if (L1_0.on) {
+        if (L1_0.d) {
             const double g = cq_adjoint<DIR>(P, gn - 1, gt, R0, L1_0, L2_0, 0);
-            pin1p = g * L1_0.wp; pin1u = g * L1_0.wu;
+            pin1p = g * L1_0.up; pin1u = g * L1_0.uu;
         }
-        if (L2_3.on) {
+        if (L2_3.d) {
             const double g = cq_adjoint<DIR>(P, gn + 2, gt, R3, L1_3, L2_3, 1);
-            pin2p = g * L2_3.wp; pin2u = g * L2_3.wu;
+            pin2p = g * L2_3.up; pin2u = g * L2_3.uu;
         }
     }
     // interface gn: the position is its upper/right cell

@@ -29,7 +29,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -707,17 +709,33 @@ __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Band heights picked by claw2d_classic_autotune, per problem shape.
+struct TuneKey {
+    int kind, order, trans, precision, mx, my;
+    bool operator<(const TuneKey &o) const {
+        const int a[6] = {kind, order, trans, precision, mx, my}, b[6] = {o.kind, o.order, o.trans, o.precision, o.mx, o.my};
+        return std::lexicographical_compare(a, a + 6, b, b + 6);
+    }
+};
+
 struct Ctx {
     int device;
     void *scratch = nullptr;   // split intermediate state
     size_t scratch_bytes = 0;
+    std::map<TuneKey, int> band;   // tuned unsplit band heights
+    int force_band = 0;            // set while the tuner times a candidate
 };
 
 static int band_height(int rows, int strips_x) {
-    // about 2048 wavefronts (two per SIMD) with bands of at least 16 rows
+    // about 2048 wavefronts (two per SIMD) with bands of at least 4 rows; small
+    // grids trade the 3 extra rows each band marches through for more waves
     long long target = 2048;
-    int h = (int)std::max<long long>(16, ((long long)rows * strips_x + target - 1) / target);
+    int h = (int)std::max<long long>(4, ((long long)rows * strips_x + target - 1) / target);
     return std::min(h, std::max(rows, 1));
+}
+
+static TuneKey tune_key(const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, int precision) {
+    return TuneKey{rp->kind, meth->order, meth->transverse, precision, grid->mx, grid->my};
 }
 
 template <class RP, typename R>
@@ -761,6 +779,12 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         for (int k = 0; k < 8; ++k) p.mthlim[k] = meth->mthlim[k];
         p.nsx = (grid->mx + XS_USEFUL - 1) / XS_USEFUL;
         p.H = band_height(grid->my, p.nsx);
+        if (ctx->force_band > 0) {
+            p.H = std::min(ctx->force_band, grid->my);
+        } else {
+            auto it = ctx->band.find(tune_key(grid, meth, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_F64));
+            if (it != ctx->band.end()) p.H = std::min(it->second, grid->my);
+        }
         p.nsy = (grid->my + p.H - 1) / p.H;
         p.wpb = wpb;
         p.nsyb = (p.nsy + wpb - 1) / wpb;
@@ -960,6 +984,51 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
     default: DISPATCH(ShallowRoe);
     }
 #undef DISPATCH
+}
+
+int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                            const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
+                            const double *dt_dev, double *cfl_dev, void *stream, int *band_out) {
+    if (!ctx || !grid || !meth) return CLAW_EINVAL;
+    Ctx *c = (Ctx *)ctx;
+    if (meth->split) {   // the split passes keep the default bands
+        if (band_out) *band_out = 0;
+        return CLAW_OK;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return CLAW_EDEVICE;
+    }
+    const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
+    int best = 0;
+    float best_ms = 0.f;
+    int rc = CLAW_OK;
+    for (int h : cand) {
+        if (h > grid->my && h != cand[0]) break;
+        c->force_band = h;
+        rc = claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev,
+                                    CLAW_STEP_ACCUMULATE_CFL, stream);
+        if (rc) break;
+        const int reps = 5;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev,
+                                        CLAW_STEP_ACCUMULATE_CFL, stream);
+        (void)hipEventRecord(e1, st);
+        if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (best == 0 || ms < best_ms) { best = h; best_ms = ms; }
+    }
+    c->force_band = 0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    c->band[tune_key(grid, meth, rp, precision)] = best;
+    if (band_out) *band_out = best;
+    return CLAW_OK;
 }
 
 int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, int ng, const ClawMethod *meth,

@@ -179,3 +179,19 @@ def test_misfit_gradient_matches_central_differences():
     Jf.backward()
     assert float(Jf) == float(J)
     assert torch.equal(Kt.grad, gK) and torch.equal(rt.grad, gr) and torch.equal(q0t.grad, gq)
+
+
+@pytest.mark.parametrize("scale", [1e-155, 1e-160])
+def test_adjoint_step_is_scale_invariant_down_to_subnormal_wave_norms(scale):
+    """The limited step is positively homogeneous in q (theta and phi depend on
+    ratios of waves), so lam_n does not change when q is scaled and the aux
+    gradient scales with q.  At these scales |W|^2 is subnormal: the limiter
+    derivative must still be finite and the same."""
+    mx, my = 21, 13
+    cfg = (2, 1, (4, 4), (1, 1), (1, 1))
+    q, aux, lam = _data(mx, my, 77)
+    g1, a1 = _gpu_vjp(mx, my, cfg, q, aux, lam)
+    g2, a2 = _gpu_vjp(mx, my, cfg, scale * q, aux, lam)
+    assert np.isfinite(g2).all() and np.isfinite(a2).all()
+    assert np.abs(g2 - g1).max() <= 1e-12 * np.abs(g1).max()
+    assert np.abs(a2 / scale - a1).max() <= 1e-12 * np.abs(a1).max()

@@ -99,3 +99,59 @@ def test_f32_error_vs_f64():
     print("f32 rel Linf", rel)
     assert rel < 1e-5
     assert abs(cgot - cref) / cref < 1e-5
+
+
+def test_autotuned_band_heights_give_identical_results():
+    """The row-band height only changes the launch shape: after autotuning
+    (which times heights 4..64) the step equals the default-shape step and the
+    oracle bit for bit, and the Courant number is the same."""
+    import ctypes
+    import torch
+    from pyclaw_amd import _lib
+    from pyclaw_amd.problems import shockbubble, euler_max_speed
+    mx, my = 130, 70
+    q0, dx, dy = shockbubble(mx, my)
+    dt = 0.8 * min(dx, dy) / euler_max_speed(q0)
+    dev = torch.device('cuda', 0)
+    ng = 3
+    buf = torch.zeros((5, my + 2 * ng, mx + 2 * ng), dtype=torch.float64)
+    buf[:, ng:ng + my, ng:ng + mx] = torch.from_numpy(np.ascontiguousarray(q0.transpose(0, 2, 1)))
+    qa = buf.to(dev)
+    grid = _lib.ClawGrid(mx, my, ng, 0, mx + 2 * ng, (mx + 2 * ng) * (my + 2 * ng), dx, dy)
+    m = _lib.ClawMethod()
+    m.order, m.transverse, m.split, m.fwave, m.num_waves = 2, 2, 0, 0, 5
+    for k in range(5):
+        m.mthlim[k] = 4
+    r = _lib.ClawRiemann()
+    r.kind, r.efix = _lib.RP_EULER, 1
+    r.p[0], r.p[1] = 1.4, 0.4
+    bc = _lib.ClawBC()
+    bc.lower[0], bc.lower[1], bc.upper[0], bc.upper[1] = 1, 3, 1, 1
+    L = _lib.lib()
+    dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
+    sh = torch.cuda.current_stream().cuda_stream
+    outs, cfls = [], []
+    for tuned in (False, True):
+        ctx = _lib.Context(0)
+        cfl = torch.zeros((), dtype=torch.float64, device=dev)
+        qb = torch.zeros_like(qa)
+        if tuned:
+            band = ctypes.c_int(0)
+            _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, _lib.F64, qa.data_ptr(), qb.data_ptr(),
+                                                 None, dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(band)),
+                       "autotune")
+            assert band.value in (4, 6, 8, 12, 16, 24, 32, 48, 64)
+            qb.zero_()
+        _lib.check(L.claw2d_classic_step(ctx.handle, grid, m, r, bc, _lib.F64, qa.data_ptr(), qb.data_ptr(), None,
+                                         dtd.data_ptr(), cfl.data_ptr(), sh), "step")
+        torch.cuda.synchronize()
+        outs.append(qb[:, ng:ng + my, ng:ng + mx].cpu().numpy())
+        cfls.append(float(cfl))
+    assert np.array_equal(outs[0], outs[1]) and cfls[0] == cfls[1]
+    O.set_params(O.RP_EULER, dict(gamma=1.4, gamma1=0.4))
+    qbc = np.zeros((5, mx + 4, my + 4), order='F')
+    qbc[:, 2:-2, 2:-2] = q0
+    O.fill_bc(qbc, 2, [1, 3], [1, 1], True)
+    ref, rcfl = O.step2(O.RP_EULER, qbc, None, dx, dy, dt, O.method_array(2, 2, False), O.mthlim_array(4, 5), 5)
+    assert np.array_equal(outs[1], np.ascontiguousarray(ref[:, 2:-2, 2:-2].transpose(0, 2, 1)))
+    assert cfls[1] == rcfl
