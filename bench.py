@@ -41,7 +41,7 @@ def parse():
     ap.add_argument('--no-autotune', action='store_true', help="keep the default row-band height")
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
-                    choices=['euler', 'batched', 'sharpclaw', 'acoustics', 'adjoint'],
+                    choices=['euler', 'batched', 'sharpclaw', 'acoustics', 'adjoint', 'copy'],
                     help="euler: the headline line; the others measure the other configurations")
     ap.add_argument('--instances', type=int, default=512)
     ap.add_argument('--adjoint-steps', type=int, default=2000)
@@ -131,7 +131,7 @@ def main():
         _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, prec, qa.data_ptr(), qb.data_ptr(), None,
                                              dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
                    'claw2d_classic_autotune')
-        band = hb.value
+        band = {"rows": abs(hb.value), "waves_per_simd": 2 if hb.value < 0 else 1}
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
@@ -155,6 +155,22 @@ def main():
     cfl_max = float(cfl.item())
     qfin = bufs[(args.warmup + args.steps) % 2]
     finite = bool(torch.isfinite(qfin).all().item())
+
+    f32_err = None
+    if args.precision == 'f32':
+        # error of the FP32 variant against the FP64 kernels over the same steps (untimed)
+        ref = [buf.to(dev, torch.float64), None]
+        ref[1] = ref[0].clone()
+        for k in range(args.warmup + args.steps):
+            _lib.check(L.claw2d_classic_step_ex(ctx.handle, grid, m, r, bc, _lib.F64, ref[k % 2].data_ptr(),
+                                                ref[(k + 1) % 2].data_ptr(), None, dtd.data_ptr(), cfl.data_ptr(),
+                                                _lib.STEP_ACCUMULATE_CFL, sh), 'f64 reference step')
+        r64 = ref[(args.warmup + args.steps) % 2][:, ng:ng + my, ng:ng + mx]
+        r32 = qfin[:, ng:ng + my, ng:ng + mx].to(torch.float64)
+        # per field, relative to the field's scale (both momenta share the momentum scale)
+        scale = [r64[0].abs().max(), r64[1:3].abs().max(), r64[1:3].abs().max(), r64[3].abs().max(),
+                 max(float(r64[4].abs().max()), 1e-300)]
+        f32_err = [float(((r32[c] - r64[c]).abs().max() / scale[c]).item()) for c in range(5)]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -187,7 +203,8 @@ def main():
                      "kernel": "classic_unsplit_kernel<Euler5<%s>>" % ('double' if args.precision == 'f64' else 'float')},
         "cpu_baseline": cpu,
         "extra": {"cfl_max_over_timed_steps": cfl_max, "finite": finite, "dt": dt, "wall_s": wall,
-                  "hbm_pct": 100.0 * achieved / HBM_PEAK_GBS, "band_rows": band},
+                  "hbm_pct": 100.0 * achieved / HBM_PEAK_GBS, "launch_shape": band,
+                  "f32_rel_linf_vs_f64_per_field": f32_err},
     }
     print(json.dumps(out), flush=True)
 
@@ -285,10 +302,19 @@ def bench_sharpclaw(args):
     pair = [q, qn]
     tot0 = q0.reshape(3, -1).sum(axis=1)
 
+    ctx = _lib.Context(0)
+    band = None
+    if not args.no_autotune:
+        hb = ctypes.c_int(0)
+        _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, _lib.F64, q.data_ptr(), qn.data_ptr(), None,
+                                               dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
+                   'claw2d_sharpclaw_autotune')
+        band = hb.value
+
     def step(k):
-        rc = L.claw2d_sharpclaw_ssp104_step(None, g, r, bc, _lib.F64, pair[k % 2].data_ptr(), pair[(k + 1) % 2].data_ptr(),
-                                            s1a.data_ptr(), s1b.data_ptr(), s2.data_ptr(), None, dtd.data_ptr(),
-                                            cfl.data_ptr(), sh)
+        rc = L.claw2d_sharpclaw_ssp104_step(ctx.handle, g, r, bc, _lib.F64, pair[k % 2].data_ptr(),
+                                            pair[(k + 1) % 2].data_ptr(), s1a.data_ptr(), s1b.data_ptr(),
+                                            s2.data_ptr(), None, dtd.data_ptr(), cfl.data_ptr(), sh)
         _lib.check(rc, 'ssp104')
     for k in range(args.warmup):
         step(k)
@@ -314,7 +340,7 @@ def bench_sharpclaw(args):
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": bytes_step},
            "extra": {"mass_rel_change": float(h_rel), "momentum_rel_change_over_mass": mom_rel.tolist(),
-                     "cfl_last_stage": float(cfl[9].item())}}
+                     "cfl_last_stage": float(cfl[9].item()), "band_rows": band}}
     print(json.dumps(out), flush=True)
 
 
@@ -444,8 +470,32 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_copy(args):
+    """Calibration for the HBM byte counters: a plain device-to-device copy of
+    512 MiB, 5 times (FETCH_SIZE and WRITE_SIZE of that kernel should each
+    read 524288 KiB per dispatch)."""
+    import torch
+    n = 512 * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"metric": "copy GB/s", "value": 2 * n * 8 / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                      "n_gpus": 1, "steps": 5, "warmup": 0, "ms_per_step": ms, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "device copy 512 MiB", "global_batch": 1,
+                                 "parallelism": "single GPU"}}), flush=True)
+
+
 OTHER = {'batched': bench_batched, 'sharpclaw': bench_sharpclaw, 'acoustics': bench_acoustics,
-         'adjoint': bench_adjoint}
+         'adjoint': bench_adjoint, 'copy': bench_copy}
 
 
 if __name__ == '__main__':

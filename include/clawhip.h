@@ -108,19 +108,28 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
 
 /* Autotuner of the unsplit step's launch shape: times the fused kernel for a
    set of row-band heights (the rows one wavefront marches through; more
-   bands mean more wavefronts in flight, fewer mean fewer halo rows) and keeps
-   the fastest for this context and problem shape (Riemann solver, order,
-   transverse mode, precision, mx, my); later classic steps of that shape use
-   it.  Results do not depend on the band height (every cell is computed the
-   same way and the Courant maximum is order-free), so tuning never changes
-   answers.  Runs real steps from qold into qnew and max-combines into
-   *cfl_dev.  Blocks until the timing launches finish.  *band_out (optional)
-   receives the chosen height, 0 for the split step (not tuned).
+   bands mean more wavefronts in flight, fewer mean fewer halo rows), each
+   with the kernel compiled for one and for two wavefronts per SIMD, and
+   keeps the fastest for this context and problem shape (Riemann solver,
+   order, transverse mode, precision, mx, my); later classic steps of that
+   shape use it.  Results do not depend on the launch shape (every cell is
+   computed the same way and the Courant maximum is order-free), so tuning
+   never changes answers.  Runs real steps from qold into qnew and
+   max-combines into *cfl_dev.  Blocks until the timing launches finish.
+   *band_out (optional) receives the chosen height, negated when the
+   two-waves-per-SIMD build won; 0 for the split step (not tuned).
    No counterpart in the reference (its step has no launch shape). */
 int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *method,
                             const ClawRiemann *rp, const ClawBC *bc, int precision,
                             const void *qold, void *qnew, const void *aux,
                             const double *dt_dev, double *cfl_dev, void *stream, int *band_out);
+
+/* Sets the launch shape the autotuner would store: row-band height `band`
+   (>= 1) and the build for `waves_per_simd` (1 or 2) wavefronts per SIMD,
+   for classic steps of this problem shape on this context.  For callers
+   that tune once and reuse the shape.  No counterpart in the reference. */
+int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod *method,
+                              const ClawRiemann *rp, int precision, int band, int waves_per_simd);
 
 /* Number of kernel launches one classic step issues (for timing). */
 int claw2d_classic_launches(const ClawMethod *method);
@@ -160,6 +169,16 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
                                  const ClawBC *bc, int precision, const void *q, void *qnew,
                                  void *s1a, void *s1b, void *s2, const void *aux,
                                  const double *dt_dev, double *cfl10_dev, void *stream);
+
+/* Autotuner of the SharpClaw stage kernel's row-band height (the same idea as
+   claw2d_classic_autotune): times dq evaluations of q into dq for heights
+   4..64 and keeps the fastest for this context and shape; every later
+   SharpClaw call of that shape (dq, stage, SSP104 step) uses it.  Results do
+   not depend on it.  Overwrites dq and *cfl_dev.  No counterpart in the
+   reference. */
+int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                              int precision, const void *q, void *dq, const void *aux,
+                              const double *dt_dev, double *cfl_dev, void *stream, int *band_out);
 
 /* ---- Batched runs of many independent grids ---------------------------------
    No counterpart in the reference (which runs one grid per process); this is

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "riemann.hpp"
 
 namespace clawhip {
@@ -340,6 +341,19 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 
 template <class RP, typename R>
 __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
+    const int vb = xcd_block();
+    const int sx = vb / p.nsyb;
+    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+    if (sx >= p.nsx || sy >= p.nsy) return;
+    unsplit_band<RP, R>(p, sx, sy);
+}
+
+// The same kernel compiled for two wavefronts per SIMD: the register budget
+// halves (the FP64 Euler instantiation then spills to scratch), latency
+// hiding doubles.  The autotuner times both.
+template <class RP, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void classic_unsplit_kernel_w2(const ClassicParams<RP, R> p) {
     const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
@@ -709,23 +723,6 @@ __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-// Band heights picked by claw2d_classic_autotune, per problem shape.
-struct TuneKey {
-    int kind, order, trans, precision, mx, my;
-    bool operator<(const TuneKey &o) const {
-        const int a[6] = {kind, order, trans, precision, mx, my}, b[6] = {o.kind, o.order, o.trans, o.precision, o.mx, o.my};
-        return std::lexicographical_compare(a, a + 6, b, b + 6);
-    }
-};
-
-struct Ctx {
-    int device;
-    void *scratch = nullptr;   // split intermediate state
-    size_t scratch_bytes = 0;
-    std::map<TuneKey, int> band;   // tuned unsplit band heights
-    int force_band = 0;            // set while the tuner times a candidate
-};
-
 static int band_height(int rows, int strips_x) {
     // about 2048 wavefronts (two per SIMD) with bands of at least 4 rows; small
     // grids trade the 3 extra rows each band marches through for more waves
@@ -779,17 +776,22 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         for (int k = 0; k < 8; ++k) p.mthlim[k] = meth->mthlim[k];
         p.nsx = (grid->mx + XS_USEFUL - 1) / XS_USEFUL;
         p.H = band_height(grid->my, p.nsx);
+        int w2 = 0;
         if (ctx->force_band > 0) {
             p.H = std::min(ctx->force_band, grid->my);
+            w2 = ctx->force_w == 2;
         } else {
             auto it = ctx->band.find(tune_key(grid, meth, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_F64));
-            if (it != ctx->band.end()) p.H = std::min(it->second, grid->my);
+            if (it != ctx->band.end()) { p.H = std::min(it->second.first, grid->my); w2 = it->second.second == 2; }
         }
         p.nsy = (grid->my + p.H - 1) / p.H;
         p.wpb = wpb;
         p.nsyb = (p.nsy + wpb - 1) / wpb;
         int nblocks = p.nsx * p.nsyb;
-        hipLaunchKernelGGL((classic_unsplit_kernel<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
+        if (w2)
+            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
+        else
+            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
         return check(hipGetLastError());
     }
     // split: x pass into scratch (padded layout, ghost rows included), then y pass
@@ -986,6 +988,14 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
 #undef DISPATCH
 }
 
+int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                              int precision, int band, int waves_per_simd) {
+    if (!ctx || !grid || !meth || !rp) return CLAW_EINVAL;
+    if (band < 1 || (waves_per_simd != 1 && waves_per_simd != 2)) return CLAW_EINVAL;
+    ((Ctx *)ctx)->band[tune_key(grid, meth, rp, precision)] = std::make_pair(band, waves_per_simd);
+    return CLAW_OK;
+}
+
 int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
                             const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
                             const double *dt_dev, double *cfl_dev, void *stream, int *band_out) {
@@ -1002,12 +1012,14 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         return CLAW_EDEVICE;
     }
     const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
-    int best = 0;
+    int best = 0, best_w = 1;
     float best_ms = 0.f;
     int rc = CLAW_OK;
-    for (int h : cand) {
-        if (h > grid->my && h != cand[0]) break;
+    for (int k = 0; k < 18 && !rc; ++k) {
+        const int h = cand[k % 9], w = k < 9 ? 1 : 2;
+        if (h > grid->my && h != cand[0]) continue;
         c->force_band = h;
+        c->force_w = w;
         rc = claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev,
                                     CLAW_STEP_ACCUMULATE_CFL, stream);
         if (rc) break;
@@ -1020,14 +1032,15 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (best == 0 || ms < best_ms) { best = h; best_ms = ms; }
+        if (best == 0 || ms < best_ms) { best = h; best_w = w; best_ms = ms; }
     }
     c->force_band = 0;
+    c->force_w = 0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    c->band[tune_key(grid, meth, rp, precision)] = best;
-    if (band_out) *band_out = best;
+    c->band[tune_key(grid, meth, rp, precision)] = std::make_pair(best, best_w);
+    if (band_out) *band_out = best_w == 2 ? -best : best;
     return CLAW_OK;
 }
 

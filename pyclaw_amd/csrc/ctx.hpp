@@ -1,0 +1,32 @@
+// Library context shared by the kernel files: scratch buffers and the launch
+// shapes picked by the autotuners, per problem shape.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <utility>
+
+namespace clawhip {
+
+// Launch shapes picked by the autotuners, per problem shape (SharpClaw keys
+// use order = SHARP_KEY).
+struct TuneKey {
+    int kind, order, trans, precision, mx, my;
+    bool operator<(const TuneKey &o) const {
+        const int a[6] = {kind, order, trans, precision, mx, my}, b[6] = {o.kind, o.order, o.trans, o.precision, o.mx, o.my};
+        return std::lexicographical_compare(a, a + 6, b, b + 6);
+    }
+};
+
+struct Ctx {
+    int device;
+    void *scratch = nullptr;   // split intermediate state
+    size_t scratch_bytes = 0;
+    std::map<TuneKey, std::pair<int, int>> band;   // tuned (band height, waves per SIMD)
+    int force_band = 0, force_w = 0;               // set while the tuner times a candidate
+};
+
+constexpr int SHARP_KEY = 100;
+
+}  // namespace clawhip

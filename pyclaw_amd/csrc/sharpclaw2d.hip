@@ -20,8 +20,10 @@
 
 #include <algorithm>
 #include <new>
+#include <type_traits>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "riemann.hpp"
 
 namespace clawhip {
@@ -282,8 +284,22 @@ static int sharp_band_height(int rows, int strips_x) {
     return std::min(h, std::max(rows, 1));
 }
 
+static TuneKey sharp_key(const ClawGrid *grid, const ClawRiemann *rp, int precision) {
+    return TuneKey{rp->kind, SHARP_KEY, 0, precision, grid->mx, grid->my};
+}
+
+// band height for this call: the tuner's candidate, the tuned value, or the default
+static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int strips) {
+    if (ctx && ctx->force_band > 0) return std::min(ctx->force_band, grid->my);
+    if (ctx) {
+        auto it = ctx->band.find(sharp_key(grid, rp, precision));
+        if (it != ctx->band.end()) return std::min(it->second.first, grid->my);
+    }
+    return sharp_band_height(grid->my, strips);
+}
+
 template <class RP, typename R>
-static int launch_sharp(const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int mode,
+static int launch_sharp(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int mode,
                         const void *qin, const void *q0, void *s2, void *out, const void *aux,
                         const double *dt, double *cfl, bool zero_cfl, hipStream_t st) {
     const int wpb = 4;
@@ -295,7 +311,7 @@ static int launch_sharp(const ClawGrid *grid, const ClawRiemann *rp, const ClawB
     p.P = make_sharp_params<RP, R>(rp);
     p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy; p.mode = mode;
     p.nsx = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
-    p.H = sharp_band_height(grid->my, p.nsx);
+    p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_F64, p.nsx);
     p.nsy = (grid->my + p.H - 1) / p.H;
     p.wpb = wpb;
     p.nsyb = (p.nsy + wpb - 1) / wpb;
@@ -319,13 +335,13 @@ static int sharp_validate(const ClawGrid *grid, const ClawRiemann *rp, const Cla
     return CLAW_OK;
 }
 
-static int sharp_dispatch(const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int precision, int mode,
-                          const void *qin, const void *q0, void *s2, void *out, const void *aux,
+static int sharp_dispatch(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int precision,
+                          int mode, const void *qin, const void *q0, void *s2, void *out, const void *aux,
                           const double *dt, double *cfl, bool zero_cfl, hipStream_t st) {
     const bool f32 = precision == CLAW_F32;
 #define SDISPATCH(RPT)                                                                                        \
-    return f32 ? launch_sharp<RPT<float>, float>(grid, rp, bc, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st) \
-               : launch_sharp<RPT<double>, double>(grid, rp, bc, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st)
+    return f32 ? launch_sharp<RPT<float>, float>(ctx, grid, rp, bc, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st) \
+               : launch_sharp<RPT<double>, double>(ctx, grid, rp, bc, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st)
     switch (rp->kind) {
     case CLAW_RP_ACOUSTICS: SDISPATCH(Acoustics);
     case CLAW_RP_VC_ACOUSTICS: SDISPATCH(VCAcoustics);
@@ -344,18 +360,16 @@ extern "C" {
 int claw2d_sharpclaw_dq(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int precision,
                         const void *q, void *dq, const void *aux, const double *dt_dev, double *cfl_dev,
                         void *stream) {
-    (void)ctx;
     int rc = sharp_validate(grid, rp, bc, precision, aux);
     if (rc) return rc;
     if (!q || !dq || q == dq || !dt_dev || !cfl_dev) return CLAW_EINVAL;
-    return sharp_dispatch(grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, true,
+    return sharp_dispatch((Ctx *)ctx, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, true,
                           (hipStream_t)stream);
 }
 
 int claw2d_sharpclaw_stage(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
                            int precision, int mode, const void *qin, const void *q0, void *s2, void *out,
                            const void *aux, const double *dt_dev, double *cfl_dev, void *stream) {
-    (void)ctx;
     int rc = sharp_validate(grid, rp, bc, precision, aux);
     if (rc) return rc;
     if (mode < SM_DQ || mode > SM_SSP33_3) return CLAW_EINVAL;
@@ -363,14 +377,13 @@ int claw2d_sharpclaw_stage(void *ctx, const ClawGrid *grid, const ClawRiemann *r
     if ((mode == SM_SSP104_MID || mode == SM_SSP33_2 || mode == SM_SSP33_3) && !q0) return CLAW_EINVAL;
     if ((mode == SM_SSP104_MID || mode == SM_SSP104_END) && !s2) return CLAW_EINVAL;
     if (mode == SM_SSP104_MID && (s2 == qin || s2 == out)) return CLAW_EINVAL;
-    return sharp_dispatch(grid, rp, bc, precision, mode, qin, q0, s2, out, aux, dt_dev, cfl_dev, true,
+    return sharp_dispatch((Ctx *)ctx, grid, rp, bc, precision, mode, qin, q0, s2, out, aux, dt_dev, cfl_dev, true,
                           (hipStream_t)stream);
 }
 
 int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
                                  int precision, const void *q, void *qnew, void *s1a, void *s1b, void *s2,
                                  const void *aux, const double *dt_dev, double *cfl10_dev, void *stream) {
-    (void)ctx;
     int rc = sharp_validate(grid, rp, bc, precision, aux);
     if (rc) return rc;
     if (!q || !qnew || !s1a || !s1b || !s2 || !dt_dev || !cfl10_dev) return CLAW_EINVAL;
@@ -384,7 +397,7 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
     int k = 0, cur = 0;
 #define STAGE(MODE, IN, OUT)                                                                                   \
     do {                                                                                                       \
-        rc = sharp_dispatch(grid, rp, bc, precision, MODE, IN, q, s2, OUT, aux, dt_dev, cfl10_dev + k, true, st); \
+        rc = sharp_dispatch((Ctx *)ctx, grid, rp, bc, precision, MODE, IN, q, s2, OUT, aux, dt_dev, cfl10_dev + k, true, st); \
         if (rc) return rc;                                                                                     \
         ++k;                                                                                                   \
     } while (0)
@@ -397,4 +410,47 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
     return CLAW_OK;
 }
 
+int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                              int precision, const void *q, void *dq, const void *aux, const double *dt_dev,
+                              double *cfl_dev, void *stream, int *band_out) {
+    if (!ctx) return CLAW_EINVAL;
+    int rc = sharp_validate(grid, rp, bc, precision, aux);
+    if (rc) return rc;
+    if (!q || !dq || q == dq || !dt_dev || !cfl_dev) return CLAW_EINVAL;
+    Ctx *c = (Ctx *)ctx;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return CLAW_EDEVICE;
+    }
+    const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
+    int best = 0;
+    float best_ms = 0.f;
+    for (int h : cand) {
+        if (h > grid->my && h != cand[0]) break;
+        c->force_band = h;
+        rc = sharp_dispatch(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, false, st);
+        if (rc) break;
+        const int reps = 5;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = sharp_dispatch(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev,
+                                false, st);
+        (void)hipEventRecord(e1, st);
+        if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (best == 0 || ms < best_ms) { best = h; best_ms = ms; }
+    }
+    c->force_band = 0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    c->band[sharp_key(grid, rp, precision)] = std::make_pair(best, 1);
+    if (band_out) *band_out = best;
+    return CLAW_OK;
+}
+
 }  // extern "C"
+

@@ -102,9 +102,10 @@ def test_f32_error_vs_f64():
 
 
 def test_autotuned_band_heights_give_identical_results():
-    """The row-band height only changes the launch shape: after autotuning
-    (which times heights 4..64) the step equals the default-shape step and the
-    oracle bit for bit, and the Courant number is the same."""
+    """The launch shape (row-band height, one or two wavefronts per SIMD) only
+    changes scheduling: after autotuning and for shapes set explicitly the step
+    equals the default-shape step and the oracle bit for bit, and the Courant
+    number is the same."""
     import ctypes
     import torch
     from pyclaw_amd import _lib
@@ -131,23 +132,26 @@ def test_autotuned_band_heights_give_identical_results():
     dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
     sh = torch.cuda.current_stream().cuda_stream
     outs, cfls = [], []
-    for tuned in (False, True):
+    for tuned in (False, True, (5, 1), (5, 2), (70, 2)):
         ctx = _lib.Context(0)
         cfl = torch.zeros((), dtype=torch.float64, device=dev)
         qb = torch.zeros_like(qa)
-        if tuned:
+        if isinstance(tuned, tuple):
+            _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, _lib.F64, tuned[0], tuned[1]), "set_launch")
+        elif tuned:
             band = ctypes.c_int(0)
             _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, _lib.F64, qa.data_ptr(), qb.data_ptr(),
                                                  None, dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(band)),
                        "autotune")
-            assert band.value in (4, 6, 8, 12, 16, 24, 32, 48, 64)
+            assert abs(band.value) in (4, 6, 8, 12, 16, 24, 32, 48, 64)
             qb.zero_()
         _lib.check(L.claw2d_classic_step(ctx.handle, grid, m, r, bc, _lib.F64, qa.data_ptr(), qb.data_ptr(), None,
                                          dtd.data_ptr(), cfl.data_ptr(), sh), "step")
         torch.cuda.synchronize()
         outs.append(qb[:, ng:ng + my, ng:ng + mx].cpu().numpy())
         cfls.append(float(cfl))
-    assert np.array_equal(outs[0], outs[1]) and cfls[0] == cfls[1]
+    for k in range(1, len(outs)):
+        assert np.array_equal(outs[0], outs[k]) and cfls[0] == cfls[k]
     O.set_params(O.RP_EULER, dict(gamma=1.4, gamma1=0.4))
     qbc = np.zeros((5, mx + 4, my + 4), order='F')
     qbc[:, 2:-2, 2:-2] = q0

@@ -60,3 +60,45 @@ def test_sharp_dq_ragged(shape):
     got, cgot = gpu_sharp_dq(qbc, None, case, case['dt'], (3, 1), (1, 3))
     assert np.array_equal(ref, got)
     assert cref == cgot
+
+
+def test_sharp_autotuned_band_gives_identical_results():
+    """SharpClaw dq after claw2d_sharpclaw_autotune (heights 4..64) equals the
+    default-shape dq bit for bit, Courant number included."""
+    import ctypes
+    import torch
+    from pyclaw_amd import _lib
+    from pyclaw_amd.problems import shallow_hump
+    mx, my = 150, 90
+    q0 = shallow_hump(mx, my, (0.4, 0.6))
+    dev = torch.device('cuda', 0)
+    ng = 3
+    buf = torch.zeros((3, my + 2 * ng, mx + 2 * ng), dtype=torch.float64)
+    buf[:, ng:ng + my, ng:ng + mx] = torch.from_numpy(np.ascontiguousarray(q0.transpose(0, 2, 1)))
+    q = buf.to(dev)
+    g = _lib.ClawGrid(mx, my, ng, 0, mx + 2 * ng, (mx + 2 * ng) * (my + 2 * ng), 1.0 / mx, 1.0 / my)
+    r = _lib.ClawRiemann()
+    r.kind, r.efix = _lib.RP_SHALLOW, 1
+    r.p[0] = 1.0
+    bc = _lib.ClawBC()
+    bc.lower[0], bc.lower[1], bc.upper[0], bc.upper[1] = 2, 3, 2, 1
+    L = _lib.lib()
+    dtd = torch.tensor(0.3 / mx, dtype=torch.float64, device=dev)
+    sh = torch.cuda.current_stream().cuda_stream
+    outs, cfls = [], []
+    for tuned in (False, True):
+        ctx = _lib.Context(0)
+        cfl = torch.zeros((), dtype=torch.float64, device=dev)
+        dq = torch.zeros_like(q)
+        if tuned:
+            band = ctypes.c_int(0)
+            _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, _lib.F64, q.data_ptr(), dq.data_ptr(), None,
+                                                   dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(band)), "tune")
+            assert band.value in (4, 6, 8, 12, 16, 24, 32, 48, 64)
+            dq.zero_()
+        _lib.check(L.claw2d_sharpclaw_dq(ctx.handle, g, r, bc, _lib.F64, q.data_ptr(), dq.data_ptr(), None,
+                                         dtd.data_ptr(), cfl.data_ptr(), sh), "dq")
+        torch.cuda.synchronize()
+        outs.append(dq.cpu().numpy())
+        cfls.append(float(cfl))
+    assert np.array_equal(outs[0], outs[1]) and cfls[0] == cfls[1]
