@@ -195,7 +195,7 @@ typedef struct {
 } ClawBatchInstance;
 
 /* Builds the work list (instances sorted largest first, one item per
-   61-column strip x 32-row band) and device descriptors. */
+   60-column strip x 32-row band) and device descriptors. */
 int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, int ng,
                              const ClawMethod *method, const ClawRiemann *rp, const ClawBC *bc,
                              int precision, void **plan);

@@ -26,9 +26,12 @@
 // derivative).  The results are gathered per cell instead of scattered:
 // a thread owns one position, recomputes every interface whose inputs include
 // that position and sums their contributions in a fixed order, so the sweep is
-// deterministic and needs no atomics.  Ghost positions are handled by a first
-// launch that writes their raw contributions to a workspace; the cell launch
-// adds each ghost's contribution to the interior cell it was copied from.
+// deterministic and needs no atomics.  A block stages its tile of positions
+// and the cells within 3 of it (q with the boundary map applied, aux, lam) in
+// LDS first, so the recomputation reads LDS, not global memory.  Tiles cover
+// the ghost frame too; ghost positions write to a workspace, and a second,
+// small launch adds each ghost's contribution to the ring cell it was copied
+// from.
 //
 // Requirements: c > 0 in every cell (so wave 1 moves left and is limited with
 // the interface to its right, wave 2 with the interface to its left); boundary
@@ -64,39 +67,103 @@ struct CellA { double rho, c; };
 template <int DIR> __host__ __device__ __forceinline__ int ci(int n, int t) { return DIR == 0 ? n : t; }
 template <int DIR> __host__ __device__ __forceinline__ int cj(int n, int t) { return DIR == 0 ? t : n; }
 
-template <int DIR>
-__host__ __device__ __forceinline__ CellQ getq(const Params &P, int n, int t) {
-    const int i = ci<DIR>(n, t), j = cj<DIR>(n, t);
-    SrcCell s = bc_source(i, j, P.g, P.bq);
-    double u = P.q[1 * P.g.plane + s.off], v = P.q[2 * P.g.plane + s.off];
-    if (s.fx) u = -u;
-    if (s.fy) v = -v;
+// Cell data come through an accessor (the parameters plus loadq/loada/loadl):
+// GlobalAcc reads the padded arrays through the boundary map, TileAcc reads a
+// block's tile staged in LDS.  Both return the same values.
+struct GlobalAcc : Params {
+    __host__ __device__ __forceinline__ void loadq(int i, int j, double &p, double &u, double &v) const {
+        SrcCell s = bc_source(i, j, g, bq);
+        p = q[s.off];
+        u = q[g.plane + s.off];
+        v = q[2 * g.plane + s.off];
+        if (s.fx) u = -u;
+        if (s.fy) v = -v;
+    }
+    __host__ __device__ __forceinline__ void loada(int i, int j, double &rho, double &c) const {
+        SrcCell s = bc_source(i, j, g, ba);
+        rho = aux[s.off];
+        c = aux[g.plane + s.off];
+    }
+    template <int DIR>
+    __host__ __device__ __forceinline__ void face(int, int, const CellQ &ql, const CellQ &qh, const CellA &alo,
+                                                  const CellA &ahi, double &a1, double &a2, double &iD) const;
+    // 1/(z(n, t-1) + z(n, t)), (n, t) in the normal/transverse coordinates of DIR
+    template <int DIR>
+    __host__ __device__ __forceinline__ double tiD(int n, int t) const {
+        double r0, c0, r1, c1;
+        loada(DIR == 0 ? n : t - 1, DIR == 0 ? t - 1 : n, r0, c0);
+        loada(DIR == 0 ? n : t, DIR == 0 ? t : n, r1, c1);
+        return 1.0 / (r0 * c0 + r1 * c1);
+    }
+    // lam_{n+1}: zero outside the interior (ghost cells of q_{n+1} are discarded)
+    __host__ __device__ __forceinline__ void loadl(int i, int j, double &p, double &u, double &v) const {
+        if (i < 0 || j < 0 || i >= g.mx || j >= g.my) { p = u = v = 0.0; return; }
+        const long long off = (long long)(j + g.ng) * g.ldq + (i + g.ng);
+        p = lam[off];
+        u = lam[g.plane + off];
+        v = lam[2 * g.plane + off];
+    }
+};
+
+// Tile of positions handled by one block, and the cells its interfaces read
+// (3 around every position).
+constexpr int TW = 64, TH = 4, HALO = 3, BOXW = TW + 2 * HALO, BOXH = TH + 2 * HALO, BOX = BOXW * BOXH;
+
+struct TileAcc : Params {
+    const double *s;   // 8 planes of BOX values: q (3, boundary map applied), aux (2), lam (3)
+    const double *f;   // 6 planes of BOX values: (a1, a2, 1/D) of the x-face and of the y-face whose
+                       // upper/right cell is the box cell (the box's first column/row has none)
+    int bx, by;        // cell coordinates of the box's first element
+    template <int DIR>
+    __host__ __device__ __forceinline__ void face(int N, int T, const CellQ &, const CellQ &, const CellA &,
+                                                  const CellA &, double &a1, double &a2, double &iD) const {
+        const int k = at(DIR == 0 ? N : T, DIR == 0 ? T : N) + DIR * 3 * BOX;
+        a1 = f[k]; a2 = f[BOX + k]; iD = f[2 * BOX + k];
+    }
+    // the transverse face's 1/D: a y-face for DIR 0, an x-face for DIR 1
+    template <int DIR>
+    __host__ __device__ __forceinline__ double tiD(int n, int t) const {
+        return DIR == 0 ? f[5 * BOX + at(n, t)] : f[2 * BOX + at(t, n)];
+    }
+    __host__ __device__ __forceinline__ int at(int i, int j) const { return (j - by) * BOXW + (i - bx); }
+    __host__ __device__ __forceinline__ void loadq(int i, int j, double &p, double &u, double &v) const {
+        const int k = at(i, j);
+        p = s[k]; u = s[BOX + k]; v = s[2 * BOX + k];
+    }
+    __host__ __device__ __forceinline__ void loada(int i, int j, double &rho, double &c) const {
+        const int k = at(i, j);
+        rho = s[3 * BOX + k]; c = s[4 * BOX + k];
+    }
+    __host__ __device__ __forceinline__ void loadl(int i, int j, double &p, double &u, double &v) const {
+        const int k = at(i, j);
+        p = s[5 * BOX + k]; u = s[6 * BOX + k]; v = s[7 * BOX + k];
+    }
+};
+
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ CellQ getq(const PT &P, int n, int t) {
+    double p, u, v;
+    P.loadq(ci<DIR>(n, t), cj<DIR>(n, t), p, u, v);
     CellQ r;
-    r.p = P.q[s.off];
+    r.p = p;
     r.un = DIR == 0 ? u : v;
     r.ut = DIR == 0 ? v : u;
     return r;
 }
 
-template <int DIR>
-__host__ __device__ __forceinline__ CellA geta(const Params &P, int n, int t) {
-    const int i = ci<DIR>(n, t), j = cj<DIR>(n, t);
-    SrcCell s = bc_source(i, j, P.g, P.ba);
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ CellA geta(const PT &P, int n, int t) {
     CellA r;
-    r.rho = P.aux[s.off];
-    r.c = P.aux[P.g.plane + s.off];
+    P.loada(ci<DIR>(n, t), cj<DIR>(n, t), r.rho, r.c);
     return r;
 }
 
-// lam_{n+1} in (p, normal, transverse) components; zero outside the interior
-template <int DIR>
-__host__ __device__ __forceinline__ CellQ getl(const Params &P, int n, int t) {
-    const int i = ci<DIR>(n, t), j = cj<DIR>(n, t);
-    CellQ r{0.0, 0.0, 0.0};
-    if (i < 0 || j < 0 || i >= P.g.mx || j >= P.g.my) return r;
-    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
-    const double u = P.lam[P.g.plane + off], v = P.lam[2 * P.g.plane + off];
-    r.p = P.lam[off];
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ CellQ getl(const PT &P, int n, int t) {
+    double p, u, v;
+    P.loadl(ci<DIR>(n, t), cj<DIR>(n, t), p, u, v);
+    CellQ r;
+    r.p = p;
     r.un = DIR == 0 ? u : v;
     r.ut = DIR == 0 ? v : u;
     return r;
@@ -106,25 +173,40 @@ __host__ __device__ __forceinline__ CellQ getl(const Params &P, int n, int t) {
 struct Raw {
     double w1p, w1u, w2p, w2u;   // W1 = (w1p, w1u, 0), W2 = (w2p, w2u, 0)
     double s1, s2;               // -c(lo), c(hi)
-    double a1, a2, d1, D;        // wave strengths, normal-velocity jump, zlo + zhi
+    double a1, a2, d1, iD;       // wave strengths, normal-velocity jump, 1/(zlo + zhi)
     double zlo, zhi;
     CellA alo, ahi;
 };
 
+// the normal solve's wave strengths (as the forward computes them) and 1/D
 template <int DIR>
-__host__ __device__ __forceinline__ Raw raw(const Params &P, int N, int T) {
+__host__ __device__ __forceinline__ void strengths(const CellQ &ql, const CellQ &qh, const CellA &alo,
+                                                   const CellA &ahi, double &a1, double &a2, double &iD) {
+    const double d0 = qh.p - ql.p;
+    const double d1 = qh.un - ql.un;
+    const double zhi = ahi.rho * ahi.c, zlo = alo.rho * alo.c;
+    const double D = zlo + zhi;
+    a1 = (-d0 + zhi * d1) / D;
+    a2 = (d0 + zlo * d1) / D;
+    iD = 1.0 / D;
+}
+
+template <int DIR>
+__host__ __device__ __forceinline__ void GlobalAcc::face(int, int, const CellQ &ql, const CellQ &qh, const CellA &alo,
+                                                         const CellA &ahi, double &a1, double &a2, double &iD) const {
+    strengths<DIR>(ql, qh, alo, ahi, a1, a2, iD);
+}
+
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ Raw raw(const PT &P, int N, int T) {
     const CellQ ql = getq<DIR>(P, N - 1, T), qh = getq<DIR>(P, N, T);
     Raw r;
     r.alo = geta<DIR>(P, N - 1, T);
     r.ahi = geta<DIR>(P, N, T);
-    const double d0 = qh.p - ql.p;
-    const double d1 = qh.un - ql.un;
     r.zhi = r.ahi.rho * r.ahi.c;
     r.zlo = r.alo.rho * r.alo.c;
-    r.D = r.zlo + r.zhi;
-    r.a1 = (-d0 + r.zhi * d1) / r.D;
-    r.a2 = (d0 + r.zlo * d1) / r.D;
-    r.d1 = d1;
+    P.template face<DIR>(N, T, ql, qh, r.alo, r.ahi, r.a1, r.a2, r.iD);
+    r.d1 = qh.un - ql.un;
     r.w1p = -r.a1 * r.zlo; r.w1u = r.a1;
     r.w2p = r.a2 * r.zhi;  r.w2u = r.a2;
     r.s1 = -r.alo.c; r.s2 = r.ahi.c;
@@ -200,11 +282,11 @@ __host__ __device__ __forceinline__ Lim limit(double wp, double wu, double np, d
 // (p and transverse components) and of (rho, c) of the three cells.
 struct RptRev { double gt0, gtt; double gm_rho, gm_c, g0_rho, g0_c, gp_rho, gp_c; };
 
-__host__ __device__ __forceinline__ RptRev rpt_rev(double t0, double tt, CellA am, CellA a0, CellA ap,
-                                          double gbm0, double gbmt, double gbp0, double gbpt) {
+__host__ __device__ __forceinline__ RptRev rpt_rev(double t0, double tt, CellA am, CellA a0, CellA ap, double iD1,
+                                          double iD2, double gbm0, double gbmt, double gbp0, double gbpt) {
+    // iD1 = 1/(zm + zz), iD2 = 1/(zz + zp): the neighbouring faces' values
     const double zm = am.rho * am.c, zz = a0.rho * a0.c, zp = ap.rho * ap.c;
-    const double D1 = zm + zz, D2 = zz + zp;
-    const double a1 = (-t0 + tt * zz) / D1, a2 = (t0 + tt * zz) / D2;
+    const double a1 = (-t0 + tt * zz) * iD1, a2 = (t0 + tt * zz) * iD2;
     const double cm = am.c, cp = ap.c;
     const double ga1 = cm * zm * gbm0 - cm * gbmt;
     const double gcm = a1 * zm * gbm0 - a1 * gbmt;
@@ -212,7 +294,7 @@ __host__ __device__ __forceinline__ RptRev rpt_rev(double t0, double tt, CellA a
     const double ga2 = cp * zp * gbp0 + cp * gbpt;
     const double gcp = a2 * zp * gbp0 + a2 * gbpt;
     double gzp = cp * a2 * gbp0;
-    const double i1 = ga1 / D1, i2 = ga2 / D2;
+    const double i1 = ga1 * iD1, i2 = ga2 * iD2;
     RptRev r;
     r.gt0 = -i1 + i2;
     r.gtt = zz * (i1 + i2);
@@ -235,8 +317,8 @@ struct Own {
     RptRev rA, rP;                     // transverse solves (column N-1, column N)
 };
 
-template <int DIR>
-__host__ __device__ __forceinline__ Own own(const Params &P, int N, int T, const Raw &F, const Lim &L1, const Lim &L2) {
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ Own own(const PT &P, int N, int T, const Raw &F, const Lim &L1, const Lim &L2) {
     const double dtdn = DIR == 0 ? P.dtdx : P.dtdy;
     const double dtdt = DIR == 0 ? P.dtdy : P.dtdx;
     const CellQ lc = getl<DIR>(P, N - 1, T), hc = getl<DIR>(P, N, T);
@@ -261,8 +343,10 @@ __host__ __device__ __forceinline__ Own own(const Params &P, int N, int T, const
             tA0 = tA0 + cq0; tP0 = tP0 - cq0;
         }
         o.rA = rpt_rev(tA0, 0.0, geta<DIR>(P, N - 1, T - 1), F.alo, geta<DIR>(P, N - 1, T + 1),
+                       P.template tiD<DIR>(N - 1, T), P.template tiD<DIR>(N - 1, T + 1),
                        gam * (lm.p - lc.p), gam * (lm.ut - lc.ut), gam * (lc.p - lp.p), gam * (lc.ut - lp.ut));
         o.rP = rpt_rev(tP0, 0.0, geta<DIR>(P, N, T - 1), F.ahi, geta<DIR>(P, N, T + 1),
+                       P.template tiD<DIR>(N, T), P.template tiD<DIR>(N, T + 1),
                        gam * (hm.p - hc.p), gam * (hm.ut - hc.ut), gam * (hc.p - hp.p), gam * (hc.ut - hp.ut));
         gAm0 += o.rA.gt0; gAmt += o.rA.gtt;
         gAp0 += o.rP.gt0; gApt += o.rP.gtt;
@@ -304,7 +388,7 @@ struct RpnRev { double gd0, gd1, gzlo, gzhi; };
 __host__ __device__ __forceinline__ RpnRev rpn_rev(const Raw &F, double g1p, double g1u, double g2p, double g2u) {
     const double ga1 = -F.zlo * g1p + g1u;
     const double ga2 = F.zhi * g2p + g2u;
-    const double i1 = ga1 / F.D, i2 = ga2 / F.D;
+    const double i1 = ga1 * F.iD, i2 = ga2 * F.iD;
     RpnRev r;
     r.gd0 = -i1 + i2;
     r.gd1 = F.zhi * i1 + F.zlo * i2;
@@ -316,8 +400,8 @@ __host__ __device__ __forceinline__ RpnRev rpn_rev(const Raw &F, double g1p, dou
 // The limiter push of one wave of interface N to its upwind neighbour, as
 // the scalar e_s with push = e_s * u, W = s u (wave 0: wave 1, else wave 2):
 // e_s = (adjoint of the limited wave . u) phi'(theta) / u.u.
-template <int DIR>
-__host__ __device__ __forceinline__ double cq_adjoint(const Params &P, int N, int T, const Raw &F, const Lim &L1,
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ double cq_adjoint(const PT &P, int N, int T, const Raw &F, const Lim &L1,
                                              const Lim &L2, int wave) {
     const double dtdn = DIR == 0 ? P.dtdx : P.dtdy;
     const CellQ lc = getl<DIR>(P, N - 1, T), hc = getl<DIR>(P, N, T);
@@ -332,9 +416,11 @@ __host__ __device__ __forceinline__ double cq_adjoint(const Params &P, int N, in
         const CellQ hm = getl<DIR>(P, N, T - 1), hp = getl<DIR>(P, N, T + 1);
         const double cq0 = k1 * (L1.phi * F.w1p) + k2 * (L2.phi * F.w2p);
         const RptRev rA = rpt_rev(F.s1 * F.w1p + cq0, 0.0, geta<DIR>(P, N - 1, T - 1), F.alo,
-                                  geta<DIR>(P, N - 1, T + 1), gam * (lm.p - lc.p), gam * (lm.ut - lc.ut),
+                                  geta<DIR>(P, N - 1, T + 1), P.template tiD<DIR>(N - 1, T),
+                                  P.template tiD<DIR>(N - 1, T + 1), gam * (lm.p - lc.p), gam * (lm.ut - lc.ut),
                                   gam * (lc.p - lp.p), gam * (lc.ut - lp.ut));
         const RptRev rP = rpt_rev(F.s2 * F.w2p - cq0, 0.0, geta<DIR>(P, N, T - 1), F.ahi, geta<DIR>(P, N, T + 1),
+                                  P.template tiD<DIR>(N, T), P.template tiD<DIR>(N, T + 1),
                                   gam * (hm.p - hc.p), gam * (hm.ut - hc.ut), gam * (hc.p - hp.p),
                                   gam * (hc.ut - hp.ut));
         gq0 += rA.gt0 - rP.gt0;
@@ -347,8 +433,8 @@ __host__ __device__ __forceinline__ double cq_adjoint(const Params &P, int N, in
 // Transverse-solve aux partial at position (gn, gt) from interfaces of the
 // lines gt-1 and gt+1: rptP of interface (gn, T) and rptA of (gn+1, T) use
 // the column gn, with (gn, gt) as the cell above (T = gt-1) or below (T = gt+1).
-template <int DIR>
-__host__ __device__ __forceinline__ void rpt_side(const Params &P, int gn, int gt, double &grho, double &gc) {
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ void rpt_side(const PT &P, int gn, int gt, double &grho, double &gc) {
     const double dtdn = DIR == 0 ? P.dtdx : P.dtdy;
     const double dtdt = DIR == 0 ? P.dtdy : P.dtdx;
     const double gam = dtdt * (0.5 * dtdn);
@@ -359,6 +445,7 @@ __host__ __device__ __forceinline__ void rpt_side(const Params &P, int gn, int g
         const double gbm0 = gam * (lm.p - lc.p), gbmt = gam * (lm.ut - lc.ut);
         const double gbp0 = gam * (lc.p - lp.p), gbpt = gam * (lc.ut - lp.ut);
         const CellA am = geta<DIR>(P, gn, T - 1), a0 = geta<DIR>(P, gn, T), ap = geta<DIR>(P, gn, T + 1);
+        const double iD1 = P.template tiD<DIR>(gn, T), iD2 = P.template tiD<DIR>(gn, T + 1);
         // interface (gn, T): A+dq enters column gn; interface (gn+1, T): A-dq enters column gn
 #pragma unroll
         for (int which = 0; which < 2; ++which) {
@@ -373,7 +460,7 @@ __host__ __device__ __forceinline__ void rpt_side(const Params &P, int gn, int g
                 const double cq0 = k1 * (L1.phi * F.w1p) + k2 * (L2.phi * F.w2p);
                 t0 = which == 0 ? t0 - cq0 : t0 + cq0;
             }
-            const RptRev r = rpt_rev(t0, 0.0, am, a0, ap, gbm0, gbmt, gbp0, gbpt);
+            const RptRev r = rpt_rev(t0, 0.0, am, a0, ap, iD1, iD2, gbm0, gbmt, gbp0, gbpt);
             if (side == 0) { grho += r.gp_rho; gc += r.gp_c; }
             else { grho += r.gm_rho; gc += r.gm_c; }
         }
@@ -382,8 +469,8 @@ __host__ __device__ __forceinline__ void rpt_side(const Params &P, int gn, int g
 
 // Raw adjoint contributions at position (i, j) (interior or ghost) from every
 // interface normal to DIR whose inputs include that position.
-template <int DIR>
-__host__ __device__ __forceinline__ void position_dir(const Params &P, int i, int j, double (&gq)[3], double &grho,
+template <int DIR, class PT>
+__host__ __device__ __forceinline__ void position_dir(const PT &P, int i, int j, double (&gq)[3], double &grho,
                                              double &gc) {
     const int gn = DIR == 0 ? i : j, gt = DIR == 0 ? j : i;
     // interfaces gn-1 .. gn+2 along the line gt
@@ -399,11 +486,15 @@ __host__ __device__ __forceinline__ void position_dir(const Params &P, int i, in
     double pin1p = 0.0, pin1u = 0.0, pin2p = 0.0, pin2u = 0.0;
     if (P.order == 2) {
         // wave 1 of interface gn-1 limited with gn; wave 2 of gn+2 with gn+1
-        const Raw Rm = raw<DIR>(P, gn - 2, gt), Rp = raw<DIR>(P, gn + 3, gt);
         const Lim L1_0 = limit(R0.w1p, R0.w1u, R1.w1p, R1.w1u, P.mth[0]);
-        const Lim L2_0 = limit(R0.w2p, R0.w2u, Rm.w2p, Rm.w2u, P.mth[1]);
-        const Lim L1_3 = limit(R3.w1p, R3.w1u, Rp.w1p, Rp.w1u, P.mth[0]);
         const Lim L2_3 = limit(R3.w2p, R3.w2u, R2.w2p, R2.w2u, P.mth[1]);
+        // the other wave of these two interfaces matters only through trans_cor's cq
+        Lim L2_0 = limit(0.0, 0.0, 0.0, 0.0, 0), L1_3 = L2_0;
+        if (P.trans == 2) {
+            const Raw Rm = raw<DIR>(P, gn - 2, gt), Rp = raw<DIR>(P, gn + 3, gt);
+            L2_0 = limit(R0.w2p, R0.w2u, Rm.w2p, Rm.w2u, P.mth[1]);
+            L1_3 = limit(R3.w1p, R3.w1u, Rp.w1p, Rp.w1u, P.mth[0]);
+        }
         if (L1_0.d) {
             const double g = cq_adjoint<DIR>(P, gn - 1, gt, R0, L1_0, L2_0, 0);
             pin1p = g * L1_0.up; pin1u = g * L1_0.uu;
@@ -434,7 +525,8 @@ __host__ __device__ __forceinline__ void position_dir(const Params &P, int i, in
     if (P.trans > 0) rpt_side<DIR>(P, gn, gt, grho, gc);
 }
 
-__host__ __device__ __forceinline__ void position_raw(const Params &P, int i, int j, double (&gq)[3], double &grho,
+template <class PT>
+__host__ __device__ __forceinline__ void position_raw(const PT &P, int i, int j, double (&gq)[3], double &grho,
                                              double &gc) {
     gq[0] = gq[1] = gq[2] = 0.0;
     grho = 0.0; gc = 0.0;
@@ -442,80 +534,146 @@ __host__ __device__ __forceinline__ void position_raw(const Params &P, int i, in
     position_dir<1>(P, i, j, gq, grho, gc);
 }
 
-// ghost positions of the 2-wide frame: (mx+4)(my+4) - mx*my of them
-__host__ __device__ __forceinline__ long long frame_count(const Params &P) {
-    const long long W = P.g.mx + 2 * ANG;
-    return W * (P.g.my + 2 * ANG) - (long long)P.g.mx * P.g.my;
-}
-
-__host__ __device__ __forceinline__ void frame_one(const Params &P, long long k) {
-    const int mx = P.g.mx, my = P.g.my;
-    const int W = mx + 2 * ANG;
-    int i, j;
-    // rows -2, -1 and my, my+1 in full width, then the side columns of the interior rows
-    const long long band = 2LL * ANG * W;
-    if (k < band) {
-        const int r = (int)(k / W), c = (int)(k % W);
-        j = r < ANG ? r - ANG : my + (r - ANG);
-        i = c - ANG;
+// Writes the raw contributions at position (i, j) of the padded region
+// [-ANG, mx+ANG) x [-ANG, my+ANG): interior cells get lam_n = lam_{n+1} + raw
+// and their aux gradient; ghost positions go to the frame workspace, folded
+// back into their source cells by fold_one.
+template <class PT>
+__host__ __device__ __forceinline__ void position_write(const PT &A, int i, int j) {
+    double gq[3], grho, gc;
+    position_raw(A, i, j, gq, grho, gc);
+    const long long off = (long long)(j + A.g.ng) * A.g.ldq + (i + A.g.ng);
+    if (i >= 0 && j >= 0 && i < A.g.mx && j < A.g.my) {
+        for (int m = 0; m < 3; ++m) A.lam_out[m * A.g.plane + off] = A.lam[m * A.g.plane + off] + gq[m];
+        A.gaux[off] += grho;
+        A.gaux[A.g.plane + off] += gc;
     } else {
-        const long long kk = k - band;
-        const int r = (int)(kk / (2 * ANG)), c = (int)(kk % (2 * ANG));
-        j = r;
-        i = c < ANG ? c - ANG : mx + (c - ANG);
+        A.frame[off] = gq[0];
+        A.frame[A.g.plane + off] = gq[1];
+        A.frame[2 * A.g.plane + off] = gq[2];
+        A.frame[3 * A.g.plane + off] = grho;
+        A.frame[4 * A.g.plane + off] = gc;
     }
-    double gq[3], grho, gc;
-    position_raw(P, i, j, gq, grho, gc);
-    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
-    P.frame[off] = gq[0];
-    P.frame[P.g.plane + off] = gq[1];
-    P.frame[2 * P.g.plane + off] = gq[2];
-    P.frame[3 * P.g.plane + off] = grho;
-    P.frame[4 * P.g.plane + off] = gc;
 }
 
-__host__ __device__ __forceinline__ void cell_one(const Params &P, int i, int j) {
+// cells within ANG of the boundary: the only ones ghost values are copied from
+__host__ __device__ __forceinline__ long long ring_count(const Params &P) {
     const int mx = P.g.mx, my = P.g.my;
-    double gq[3], grho, gc;
-    position_raw(P, i, j, gq, grho, gc);
+    const int rb = my < ANG ? my : ANG, rt = (my - rb) < ANG ? (my - rb) : ANG;
+    const int cl = mx < ANG ? mx : ANG, cr = (mx - cl) < ANG ? (mx - cl) : ANG;
+    return (long long)(rb + rt) * mx + (long long)(my - rb - rt) * (cl + cr);
+}
+
+__host__ __device__ __forceinline__ void ring_cell(const Params &P, long long k, int &i, int &j) {
+    const int mx = P.g.mx, my = P.g.my;
+    const int rb = my < ANG ? my : ANG, rt = (my - rb) < ANG ? (my - rb) : ANG;
+    const int cl = mx < ANG ? mx : ANG, cr = (mx - cl) < ANG ? (mx - cl) : ANG;
+    if (k < (long long)rb * mx) {
+        j = (int)(k / mx); i = (int)(k % mx);
+    } else if (k < (long long)(rb + rt) * mx) {
+        const long long kk = k - (long long)rb * mx;
+        j = my - rt + (int)(kk / mx); i = (int)(kk % mx);
+    } else {
+        const long long kk = k - (long long)(rb + rt) * mx;
+        const int c = (int)(kk % (cl + cr));
+        j = rb + (int)(kk / (cl + cr));
+        i = c < cl ? c : mx - cr + (c - cl);
+    }
+}
+
+// adds the frame contributions of every ghost position copied from cell (i, j)
+__host__ __device__ __forceinline__ void fold_one(const Params &P, int i, int j) {
+    const int mx = P.g.mx, my = P.g.my;
     const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
-    // ghosts copied from this cell: their raw contributions fold back here
-    if (i < ANG || j < ANG || i >= mx - ANG || j >= my - ANG) {
-        const int gis[5] = {i, -2, -1, mx, mx + 1};
-        const int gjs[5] = {j, -2, -1, my, my + 1};
-        for (int b = 0; b < 5; ++b) {
-            for (int a = 0; a < 5; ++a) {
-                if (a == 0 && b == 0) continue;
-                const int gi = gis[a], gj = gjs[b];
-                const long long goff = (long long)(gj + P.g.ng) * P.g.ldq + (gi + P.g.ng);
-                const SrcCell sq = bc_source(gi, gj, P.g, P.bq);
-                if (sq.off == off) {
-                    gq[0] += P.frame[goff];
-                    gq[1] += sq.fx ? -P.frame[P.g.plane + goff] : P.frame[P.g.plane + goff];
-                    gq[2] += sq.fy ? -P.frame[2 * P.g.plane + goff] : P.frame[2 * P.g.plane + goff];
-                }
-                const SrcCell sa = bc_source(gi, gj, P.g, P.ba);
-                if (sa.off == off) {
-                    grho += P.frame[3 * P.g.plane + goff];
-                    gc += P.frame[4 * P.g.plane + goff];
-                }
+    double gq[3] = {0.0, 0.0, 0.0}, grho = 0.0, gc = 0.0;
+    const int gis[5] = {i, -2, -1, mx, mx + 1};
+    const int gjs[5] = {j, -2, -1, my, my + 1};
+    for (int b = 0; b < 5; ++b) {
+        for (int a = 0; a < 5; ++a) {
+            if (a == 0 && b == 0) continue;
+            const int gi = gis[a], gj = gjs[b];
+            const long long goff = (long long)(gj + P.g.ng) * P.g.ldq + (gi + P.g.ng);
+            const SrcCell sq = bc_source(gi, gj, P.g, P.bq);
+            if (sq.off == off) {
+                gq[0] += P.frame[goff];
+                gq[1] += sq.fx ? -P.frame[P.g.plane + goff] : P.frame[P.g.plane + goff];
+                gq[2] += sq.fy ? -P.frame[2 * P.g.plane + goff] : P.frame[2 * P.g.plane + goff];
+            }
+            const SrcCell sa = bc_source(gi, gj, P.g, P.ba);
+            if (sa.off == off) {
+                grho += P.frame[3 * P.g.plane + goff];
+                gc += P.frame[4 * P.g.plane + goff];
             }
         }
     }
-    for (int m = 0; m < 3; ++m) P.lam_out[m * P.g.plane + off] = P.lam[m * P.g.plane + off] + gq[m];
+    for (int m = 0; m < 3; ++m) P.lam_out[m * P.g.plane + off] += gq[m];
     P.gaux[off] += grho;
     P.gaux[P.g.plane + off] += gc;
 }
 
-__global__ __launch_bounds__(256) void adjoint_frame_kernel(const Params P) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < frame_count(P)) frame_one(P, k);
+// Wave strengths and 1/D of every face inside the staged box (faces need both
+// cells in the box, so the box's first column has no x-face, its first row no
+// y-face), computed once per tile from the staged cells.
+__host__ __device__ __forceinline__ void tile_faces(const TileAcc &A, int t0, int nt) {
+    double *f = const_cast<double *>(A.f);
+    for (int k = t0; k < BOX; k += nt) {
+        const int bi = k % BOXW, bj = k / BOXW;
+        const int i = A.bx + bi, j = A.by + bj;
+        if (bi > 0) {
+            double a1, a2, iD;
+            strengths<0>(getq<0>(A, i - 1, j), getq<0>(A, i, j), geta<0>(A, i - 1, j), geta<0>(A, i, j), a1, a2, iD);
+            f[k] = a1; f[BOX + k] = a2; f[2 * BOX + k] = iD;
+        }
+        if (bj > 0) {
+            double a1, a2, iD;
+            strengths<1>(getq<1>(A, j - 1, i), getq<1>(A, j, i), geta<1>(A, j - 1, i), geta<1>(A, j, i), a1, a2, iD);
+            f[3 * BOX + k] = a1; f[4 * BOX + k] = a2; f[5 * BOX + k] = iD;
+        }
+    }
 }
 
-__global__ __launch_bounds__(256) void adjoint_cell_kernel(const Params P) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i < P.g.mx && j < P.g.my) cell_one(P, i, j);
+// One block per TW x TH tile of the padded region: stage q (boundary map
+// applied), aux and lam for the tile plus HALO cells around it in LDS, then
+// one thread per position.
+__global__ __launch_bounds__(256) void adjoint_tile_kernel(const Params P) {
+    __shared__ double tile[8 * BOX];
+    __shared__ double faces[6 * BOX];
+    TileAcc A;
+    static_cast<Params &>(A) = P;
+    const int tx0 = blockIdx.x * TW - ANG, ty0 = blockIdx.y * TH - ANG;
+    A.s = tile;
+    A.f = faces;
+    A.bx = tx0 - HALO;
+    A.by = ty0 - HALO;
+    for (int k = threadIdx.x; k < BOX; k += 256) {
+        const int i = A.bx + k % BOXW, j = A.by + k / BOXW;
+        const SrcCell sq = bc_source(i, j, P.g, P.bq);
+        const double u = P.q[P.g.plane + sq.off], v = P.q[2 * P.g.plane + sq.off];
+        tile[k] = P.q[sq.off];
+        tile[BOX + k] = sq.fx ? -u : u;
+        tile[2 * BOX + k] = sq.fy ? -v : v;
+        const SrcCell sa = bc_source(i, j, P.g, P.ba);
+        tile[3 * BOX + k] = P.aux[sa.off];
+        tile[4 * BOX + k] = P.aux[P.g.plane + sa.off];
+        const bool in = i >= 0 && j >= 0 && i < P.g.mx && j < P.g.my;
+        const long long off = in ? (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng) : 0;
+        tile[5 * BOX + k] = in ? P.lam[off] : 0.0;
+        tile[6 * BOX + k] = in ? P.lam[P.g.plane + off] : 0.0;
+        tile[7 * BOX + k] = in ? P.lam[2 * P.g.plane + off] : 0.0;
+    }
+    __syncthreads();
+    tile_faces(A, threadIdx.x, 256);
+    __syncthreads();
+    const int i = tx0 + (threadIdx.x & (TW - 1)), j = ty0 + (threadIdx.x / TW);
+    if (i < P.g.mx + ANG && j < P.g.my + ANG) position_write(A, i, j);
+}
+
+__global__ __launch_bounds__(256) void adjoint_fold_kernel(const Params P) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ring_count(P)) return;
+    int i, j;
+    ring_cell(P, k, i, j);
+    fold_one(P, i, j);
 }
 
 }  // namespace adj
@@ -568,10 +726,10 @@ int claw2d_vc_adjoint_step(const ClawGrid *grid, const ClawMethod *meth, const C
     P.order = meth->order; P.trans = meth->transverse;
     P.mth[0] = meth->mthlim[0]; P.mth[1] = meth->mthlim[1];
     hipStream_t st = (hipStream_t)stream;
-    const long long W = grid->mx + 2 * adj::ANG;
-    const long long nframe = W * (grid->my + 2 * adj::ANG) - (long long)grid->mx * grid->my;
-    hipLaunchKernelGGL(adj::adjoint_frame_kernel, dim3((unsigned)((nframe + 255) / 256)), dim3(256), 0, st, P);
-    hipLaunchKernelGGL(adj::adjoint_cell_kernel, dim3((grid->mx + 63) / 64, (grid->my + 3) / 4), dim3(256), 0, st, P);
+    const dim3 tiles((grid->mx + 2 * adj::ANG + adj::TW - 1) / adj::TW, (grid->my + 2 * adj::ANG + adj::TH - 1) / adj::TH);
+    hipLaunchKernelGGL(adj::adjoint_tile_kernel, tiles, dim3(256), 0, st, P);
+    const long long nring = adj::ring_count(P);
+    hipLaunchKernelGGL(adj::adjoint_fold_kernel, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, st, P);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 

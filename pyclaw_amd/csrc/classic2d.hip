@@ -6,9 +6,9 @@
 // transverse solves per interface, accumulating into qnew slice by slice.
 //
 // Design (one wavefront = one independent worker, no LDS, no barriers):
-//   * A wavefront owns a strip of 64 cell columns (61 of them produced, the
-//     other 3 are the halo needed by the limiter and the transverse terms)
-//     and marches up a band of H rows.  Lane L holds column c0-1+L.
+//   * A wavefront owns a strip of 64 cell columns (60 of them produced, the
+//     other 4 are the halo needed by the limiter and the transverse terms)
+//     and marches up a band of H rows.  Lane L holds column c0-2+L.
 //   * Per row it solves the x-interface to the left of its column and the
 //     y-interface below its cell.  x-neighbour data move between lanes with
 //     ds_bpermute; y-neighbour data stay in the lane's registers (a one-row
@@ -41,7 +41,8 @@
 
 namespace clawhip {
 
-constexpr int XS_USEFUL = WAVE - 3;   // produced columns per wavefront
+constexpr int XS_USEFUL = WAVE - 3;   // produced columns per wavefront, split x pass
+constexpr int XU_USEFUL = WAVE - 4;   // produced columns per wavefront, unsplit step
 constexpr int CNG = 2;               // ghost width of the classic method (classic/solver.py:82)
 
 template <class RP, typename R>
@@ -71,11 +72,36 @@ __device__ __forceinline__ int xcd_block() {
     return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
 }
 
+// Row state carried from one y-interface to the next (the previous
+// interface's A+dq, correction flux, transverse splits and its non-zero wave
+// components) lives in per-lane LDS slots, [slot][lane] per wavefront, instead
+// of registers: it is read once and written once per row, and keeping it out
+// of the register file is what lets the FP64 Euler kernel run two wavefronts
+// per SIMD.  Each lane touches only its own slots, so no barrier is needed.
+template <class RP, int DIR>
+__host__ __device__ constexpr int wnz_slot(int mw, int m) {
+    int k = 0;
+    for (int a = 0; a < RP::NW; ++a)
+        for (int b = 0; b < RP::NE; ++b) {
+            if (a == mw && b == m) return k;
+            if (RP::template WNZ<DIR>(b, a)) ++k;
+        }
+    return k;
+}
+
+template <class RP>
+struct CarrySlots {
+    static constexpr int UAP = 0, UF = RP::NE, UBM = 2 * RP::NE, UBP = 3 * RP::NE, DOTL = 4 * RP::NE;
+    static constexpr int WY = 4 * RP::NE + RP::NW;
+    static constexpr int COUNT = WY + wnz_slot<RP, 1>(RP::NW, 0);
+};
+
 // One work item of the unsplit step: column strip sx, row band sy (whole
 // wavefront, uniform arguments).  Used by the single-grid kernel below and by
-// the batched work-queue kernel.
+// the batched work-queue kernel.  cs: this wavefront's carry slots
+// (CarrySlots<RP>::COUNT x WAVE values in LDS).
 template <class RP, typename R>
-__device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy) {
+__device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy, R *__restrict__ cs) {
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
     constexpr int NA = HAS_AUX ? RP::NA : 1;
@@ -83,14 +109,16 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 
     const int lane = threadIdx.x & (WAVE - 1);
     const GridDev g = p.g;
-    const int c0 = sx * XS_USEFUL;
-    const int i = c0 - 1 + lane;
+    const int c0 = sx * XU_USEFUL;
+    const int i = c0 - 2 + lane;
     const int R0 = sy * p.H;
     const int R1 = min(R0 + p.H, g.my);
     const int up = ((lane + WAVE - 1) & (WAVE - 1)) << 2;
     const int dn = ((lane + 1) & (WAVE - 1)) << 2;
-    const bool l0 = (lane == 0);
-    const bool produce = lane >= 1 && lane <= XS_USEFUL && i < g.mx;
+    // lanes 0, 1 and 62, 63 are halo: lane 0's own x-interface would need the
+    // column left of the strip and is never used (it is not a limiter neighbour
+    // of a produced cell's interface), lane 1's is
+    const bool produce = lane >= 2 && lane <= XU_USEFUL + 1 && i < g.mx;
 
     const double dtd = *p.dt;
     const R dtdx = R(dtd / p.dx), dtdy = R(dtd / p.dy);
@@ -100,42 +128,42 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     const auto &P = p.P;
 
     R cfl = R(0);
-    R qprev[NE], qlprev[NE];
-    R a_m2[NA], a_m1[NA], a_0[NA], al_m2[NA], al_m1[NA], al_0[NA];
+    R qprev[NE];
+    R a_m2[NA], a_m1[NA], a_0[NA];
     R accOld[NE], accNew[NE], g2Xs[NE];
-    R wY[NW][NE], sY[NW], amY[NE], apY[NE], dotlY[NW];
+    R sY[NW], amY[NE], apY[NE];
     Cache cY;
-    R uAp[NE], uF[NE], uBm[NE], uBp[NE];
+    using CS = CarrySlots<RP>;
+    R *__restrict__ slot = cs + lane;            // slot k of this lane: slot[k * WAVE]
 #pragma unroll
     for (int m = 0; m < NE; ++m) {
         accOld[m] = R(0); accNew[m] = R(0); g2Xs[m] = R(0);
-        uAp[m] = R(0); uF[m] = R(0); uBm[m] = R(0); uBp[m] = R(0);
         amY[m] = R(0); apY[m] = R(0);
+        slot[(CS::UAP + m) * WAVE] = R(0); slot[(CS::UF + m) * WAVE] = R(0);
+        slot[(CS::UBM + m) * WAVE] = R(0); slot[(CS::UBP + m) * WAVE] = R(0);
     }
 #pragma unroll
     for (int mw = 0; mw < NW; ++mw) {
-        sY[mw] = R(0); dotlY[mw] = R(0);
+        sY[mw] = R(0);
+        slot[(CS::DOTL + mw) * WAVE] = R(0);
 #pragma unroll
-        for (int m = 0; m < NE; ++m) wY[mw][m] = R(0);
+        for (int m = 0; m < NE; ++m)
+            if (RP::template WNZ<1>(m, mw)) slot[(CS::WY + wnz_slot<RP, 1>(mw, m)) * WAVE] = R(0);
     }
 #pragma unroll
-    for (int a = 0; a < NA; ++a) { a_m2[a] = a_m1[a] = a_0[a] = al_m2[a] = al_m1[a] = al_0[a] = R(1); }
+    for (int a = 0; a < NA; ++a) { a_m2[a] = a_m1[a] = a_0[a] = R(1); }
 
     load_q<NE>(p.qold, g, p.bq, i, R0 - 2, qprev);
-    if (l0) load_q<NE>(p.qold, g, p.bq, i - 1, R0 - 2, qlprev);
     if constexpr (HAS_AUX) {
         load_aux<NA>(p.aux, g, p.ba, i, R0 - 2, a_m1);
-        if (l0) load_aux<NA>(p.aux, g, p.ba, i - 1, R0 - 2, al_m1);
     }
 
     for (int L = R0 - 1; L <= R1 + 1; ++L) {
         // ---- 1. load the lead row L ----------------------------------------
-        R qc[NE], qlc[NE];
+        R qc[NE];
         load_q<NE>(p.qold, g, p.bq, i, L, qc);
-        if (l0) load_q<NE>(p.qold, g, p.bq, i - 1, L, qlc);
         if constexpr (HAS_AUX) {
             load_aux<NA>(p.aux, g, p.ba, i, L, a_0);
-            if (l0) load_aux<NA>(p.aux, g, p.ba, i - 1, L, al_0);
         }
 
         // ---- 2. x-interface (i-1/2, L-1): solve, limit, transverse, x-part of cell (i, L-1)
@@ -145,16 +173,15 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             R alo[NA], alm2[NA], al0[NA];
             Cache cx;
 #pragma unroll
-            for (int m = 0; m < NE; ++m) { lo[m] = lane_get(qprev[m], up); if (l0) lo[m] = qlprev[m]; }
+            for (int m = 0; m < NE; ++m) lo[m] = lane_get(qprev[m], up);
             if constexpr (HAS_AUX) {
 #pragma unroll
                 for (int a = 0; a < NA; ++a) {
                     alo[a] = lane_get(a_m1[a], up); alm2[a] = lane_get(a_m2[a], up); al0[a] = lane_get(a_0[a], up);
-                    if (l0) { alo[a] = al_m1[a]; alm2[a] = al_m2[a]; al0[a] = al_0[a]; }
                 }
             }
             RP::template rpn<0>(P, lo, qprev, alo, a_m1, wave, s, amdq, apdq, cx);
-            if (row >= -1 && row <= g.my && i >= 0 && i <= g.mx) {
+            if (lane >= 1 && row >= -1 && row <= g.my && i >= 0 && i <= g.mx) {
 #pragma unroll
                 for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdx * s[mw]); cfl = rmax2<R>(cfl, -dtdx * s[mw]); }
             }
@@ -230,6 +257,12 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 #pragma unroll
             for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdy * sN[mw]); cfl = rmax2<R>(cfl, -dtdy * sN[mw]); }
         }
+        R wY[NW][NE];
+#pragma unroll
+        for (int mw = 0; mw < NW; ++mw)
+#pragma unroll
+            for (int m = 0; m < NE; ++m)
+                wY[mw][m] = RP::template WNZ<1>(m, mw) ? slot[(CS::WY + wnz_slot<RP, 1>(mw, m)) * WAVE] : R(0);
 #pragma unroll
         for (int mw = 0; mw < NW; ++mw) {
             R d = R(0);
@@ -254,7 +287,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     for (int m = 0; m < NE; ++m)
                         if (RP::template WNZ<1>(m, mw)) wnorm2 = wnorm2 + wY[mw][m] * wY[mw][m];
                     if (wnorm2 != R(0)) {
-                        R phi = philim<R>(wnorm2, sY[mw] > R(0) ? dotlY[mw] : dotN[mw], mth);
+                        R phi = philim<R>(wnorm2, sY[mw] > R(0) ? slot[(CS::DOTL + mw) * WAVE] : dotN[mw], mth);
 #pragma unroll
                         for (int m = 0; m < NE; ++m)
                             if (RP::template WNZ<1>(m, mw)) wY[mw][m] = phi * wY[mw][m];
@@ -287,7 +320,6 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     for (int a = 0; a < NA; ++a) {
                         am2l[a] = lane_get(a_m2[a], up); am2r[a] = lane_get(a_m2[a], dn);
                         am1l[a] = lane_get(a_m1[a], up); am1r[a] = lane_get(a_m1[a], dn);
-                        if (l0) { am2l[a] = al_m2[a]; am1l[a] = al_m1[a]; }
                     }
                 }
                 RP::template rpt<1>(P, 1, tA, am2l, a_m2, am2r, bmA, bpA, cY);
@@ -298,38 +330,43 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                 const long long off = (long long)(row + g.ng) * g.ldq + (i + g.ng);
 #pragma unroll
                 for (int m = 0; m < NE; ++m) {
-                    R qadd = (R(0) - dtdy * uAp[m]) - dtdy * amY[m];
-                    R g1 = (R(0) - hy * bmA[m]) - hy * uBm[m];
-                    R g2 = (R(0) - hy * bpA[m]) - hy * uBp[m];
+                    const R uAp = slot[(CS::UAP + m) * WAVE], uF = slot[(CS::UF + m) * WAVE];
+                    const R uBm = slot[(CS::UBM + m) * WAVE], uBp = slot[(CS::UBP + m) * WAVE];
+                    R qadd = (R(0) - dtdy * uAp) - dtdy * amY[m];
+                    R g1 = (R(0) - hy * bmA[m]) - hy * uBm;
+                    R g2 = (R(0) - hy * bpA[m]) - hy * uBp;
                     R g2l = lane_get(g2, up), g1r = lane_get(g1, dn);
                     R v = accOld[m] + dtdx * g2l;
-                    v = v + ((qadd - dtdy * (fY[m] - uF[m])) - dtdx * (g2 - g1));
+                    v = v + ((qadd - dtdy * (fY[m] - uF)) - dtdx * (g2 - g1));
                     v = v - dtdx * g1r;
                     if (produce) p.qnew[m * g.plane + off] = v;
                 }
             }
 #pragma unroll
-            for (int m = 0; m < NE; ++m) { uAp[m] = apY[m]; uF[m] = fY[m]; uBm[m] = bmP[m]; uBp[m] = bpP[m]; }
+            for (int m = 0; m < NE; ++m) {
+                slot[(CS::UAP + m) * WAVE] = apY[m]; slot[(CS::UF + m) * WAVE] = fY[m];
+                slot[(CS::UBM + m) * WAVE] = bmP[m]; slot[(CS::UBP + m) * WAVE] = bpP[m];
+            }
         }
 
         // ---- 5. rotate ---------------------------------------------------------
 #pragma unroll
         for (int mw = 0; mw < NW; ++mw) {
-            sY[mw] = sN[mw]; dotlY[mw] = dotN[mw];
+            sY[mw] = sN[mw]; slot[(CS::DOTL + mw) * WAVE] = dotN[mw];
 #pragma unroll
-            for (int m = 0; m < NE; ++m) wY[mw][m] = wN[mw][m];
+            for (int m = 0; m < NE; ++m)
+                if (RP::template WNZ<1>(m, mw)) slot[(CS::WY + wnz_slot<RP, 1>(mw, m)) * WAVE] = wN[mw][m];
         }
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             amY[m] = amN[m]; apY[m] = apN[m]; accOld[m] = accNew[m];
-            qprev[m] = qc[m]; qlprev[m] = qlc[m];
+            qprev[m] = qc[m];
         }
         cY = cN;
         if constexpr (HAS_AUX) {
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 a_m2[a] = a_m1[a]; a_m1[a] = a_0[a];
-                al_m2[a] = al_m1[a]; al_m1[a] = al_0[a];
             }
         }
     }
@@ -345,7 +382,8 @@ __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParam
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
     if (sx >= p.nsx || sy >= p.nsy) return;
-    unsplit_band<RP, R>(p, sx, sy);
+    __shared__ R carry[4 * CarrySlots<RP>::COUNT * WAVE];
+    unsplit_band<RP, R>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
 }
 
 // The same kernel compiled for two wavefronts per SIMD: the register budget
@@ -358,7 +396,8 @@ void classic_unsplit_kernel_w2(const ClassicParams<RP, R> p) {
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
     if (sx >= p.nsx || sy >= p.nsy) return;
-    unsplit_band<RP, R>(p, sx, sy);
+    __shared__ R carry[4 * CarrySlots<RP>::COUNT * WAVE];
+    unsplit_band<RP, R>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
 }
 
 // ---------------------------------------------------------------------------
@@ -661,6 +700,7 @@ struct BatchStepParams {
 template <class RP, typename R>
 __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<RP, R> b) {
     const int lane = threadIdx.x & (WAVE - 1);
+    __shared__ R carry[4 * CarrySlots<RP>::COUNT * WAVE];
     for (;;) {
         // one ticket per wavefront: every lane takes part in the atomic (lane 0
         // adds 1, the others 0) so the loop head holds no lane-dependent branch;
@@ -684,7 +724,7 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<R
 #pragma unroll
         for (int q = 0; q < 8; ++q) p.mthlim[q] = b.mthlim[q];
         p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0;
-        unsplit_band<RP, R>(p, local / I.nsy, local % I.nsy);
+        unsplit_band<RP, R>(p, local / I.nsy, local % I.nsy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
     }
 }
 
@@ -774,7 +814,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy;
         p.order = meth->order; p.trans = meth->transverse;
         for (int k = 0; k < 8; ++k) p.mthlim[k] = meth->mthlim[k];
-        p.nsx = (grid->mx + XS_USEFUL - 1) / XS_USEFUL;
+        p.nsx = (grid->mx + XU_USEFUL - 1) / XU_USEFUL;
         p.H = band_height(grid->my, p.nsx);
         int w2 = 0;
         if (ctx->force_band > 0) {
@@ -1061,7 +1101,7 @@ int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, in
         if (rp->kind == CLAW_RP_VC_ACOUSTICS && I.aux_offset < 0) return CLAW_EINVAL;
         BatchInstDev d;
         d.mx = I.mx; d.my = I.my;
-        d.nsx = (I.mx + XS_USEFUL - 1) / XS_USEFUL;
+        d.nsx = (I.mx + XU_USEFUL - 1) / XU_USEFUL;
         d.H = std::min(I.my, 32);
         d.nsy = (I.my + d.H - 1) / d.H;
         d.pad_ = 0;
