@@ -126,7 +126,9 @@ template <typename R> struct VCAcoustics {
 template <typename R> struct Euler5 {
     static constexpr int NE = 5, NW = 5, NA = 0;
     struct Params { R gamma, gamma1; int efix; };
-    struct Cache { R u, v, enth, a, u2v2, g1a2, euv; };
+    // Roe averages kept for the transverse solves of the same interface; the
+    // other quantities rpt needs are recomputed with rpn's expressions
+    struct Cache { R u, v, enth, a; };
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
     template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
@@ -154,7 +156,7 @@ template <typename R> struct Euler5 {
         R a = rsqrt_exact<R>(a2);
         R g1a2 = gamma1 / a2;
         R euv = enth - u2v2;
-        C.u = u; C.v = v; C.enth = enth; C.a = a; C.u2v2 = u2v2; C.g1a2 = g1a2; C.euv = euv;
+        C.u = u; C.v = v; C.enth = enth; C.a = a;
 
         R d0 = hi[0] - lo[0];
         R d1 = hi[mu] - lo[mu];
@@ -246,18 +248,22 @@ template <typename R> struct Euler5 {
         }
     }
     template <int DIR>
-    __device__ static __forceinline__ void rpt(const Params &, int, const R (&asdq)[NE], const R *, const R *,
+    __device__ static __forceinline__ void rpt(const Params &P, int, const R (&asdq)[NE], const R *, const R *,
                                                const R *, R (&bm)[NE], R (&bp)[NE], const Cache &C) {
         constexpr int mu = MU<DIR>, mv = MV<DIR>;
         const R u = C.u, v = C.v, a = C.a, enth = C.enth;
-        R a3 = C.g1a2 * (C.euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
+        // the same expressions as in rpn (rpn2_euler_5wave.f:100-104), so the values are identical
+        const R u2v2 = u * u + v * v;
+        const R g1a2 = P.gamma1 / (P.gamma1 * (enth - R(0.5) * u2v2));
+        const R euv = enth - u2v2;
+        R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
         R a2 = asdq[mu] - u * asdq[0];
         R a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
         R a1 = asdq[0] - a3 - a4;
         R wb[4][NE], sb[4];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); wb[0][3] = a1 * (enth - v * a);
         wb[0][4] = R(0); sb[0] = v - a;
-        wb[1][0] = a3; wb[1][mu] = a3 * u + a2; wb[1][mv] = a3 * v; wb[1][3] = a3 * R(0.5) * C.u2v2 + a2 * u;
+        wb[1][0] = a3; wb[1][mu] = a3 * u + a2; wb[1][mv] = a3 * v; wb[1][3] = a3 * R(0.5) * u2v2 + a2 * u;
         wb[1][4] = R(0); sb[1] = v;
         wb[2][0] = a4; wb[2][mu] = a4 * u; wb[2][mv] = a4 * (v + a); wb[2][3] = a4 * (enth + v * a);
         wb[2][4] = R(0); sb[2] = v + a;
