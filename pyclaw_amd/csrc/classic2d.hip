@@ -274,6 +274,13 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 
         // ---- 4. finish y-interface (i, L-3/2) and cell row L-2 ---------------
         if (L >= R0 + 1) {
+            R sYv[NW];
+            if constexpr (RP::SPEEDS_FROM_CACHE) {
+                RP::speeds(cY, sYv);
+            } else {
+#pragma unroll
+                for (int mw = 0; mw < NW; ++mw) sYv[mw] = sY[mw];
+            }
             R fY[NE], cqY[NE];
 #pragma unroll
             for (int m = 0; m < NE; ++m) { fY[m] = R(0); cqY[m] = R(0); }
@@ -287,7 +294,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     for (int m = 0; m < NE; ++m)
                         if (RP::template WNZ<1>(m, mw)) wnorm2 = wnorm2 + wY[mw][m] * wY[mw][m];
                     if (wnorm2 != R(0)) {
-                        R phi = philim<R>(wnorm2, sY[mw] > R(0) ? slot[(CS::DOTL + mw) * WAVE] : dotN[mw], mth);
+                        R phi = philim<R>(wnorm2, sYv[mw] > R(0) ? slot[(CS::DOTL + mw) * WAVE] : dotN[mw], mth);
 #pragma unroll
                         for (int m = 0; m < NE; ++m)
                             if (RP::template WNZ<1>(m, mw)) wY[mw][m] = phi * wY[mw][m];
@@ -298,7 +305,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     R c = R(0);
 #pragma unroll
                     for (int mw = 0; mw < NW; ++mw)
-                        if (RP::template WNZ<1>(m, mw)) c = c + rabs(sY[mw]) * (R(1) - rabs(sY[mw]) * avy) * wY[mw][m];
+                        if (RP::template WNZ<1>(m, mw)) c = c + rabs(sYv[mw]) * (R(1) - rabs(sYv[mw]) * avy) * wY[mw][m];
                     cqY[m] = c;
                     fY[m] = R(0) + R(0.5) * c;
                 }
@@ -352,7 +359,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         // ---- 5. rotate ---------------------------------------------------------
 #pragma unroll
         for (int mw = 0; mw < NW; ++mw) {
-            sY[mw] = sN[mw]; slot[(CS::DOTL + mw) * WAVE] = dotN[mw];
+            if constexpr (!RP::SPEEDS_FROM_CACHE) sY[mw] = sN[mw];
+            slot[(CS::DOTL + mw) * WAVE] = dotN[mw];
 #pragma unroll
             for (int m = 0; m < NE; ++m)
                 if (RP::template WNZ<1>(m, mw)) slot[(CS::WY + wnz_slot<RP, 1>(mw, m)) * WAVE] = wN[mw][m];

@@ -42,6 +42,8 @@ template <typename R> struct Acoustics {
     static constexpr int NE = 3, NW = 2, NA = 0;
     struct Params { R rho, bulk, cc, zz; };
     struct Cache {};
+    static constexpr bool SPEEDS_FROM_CACHE = false;
+    __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
     template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
@@ -80,6 +82,8 @@ template <typename R> struct VCAcoustics {
     static constexpr int NE = 3, NW = 2, NA = 2;
     struct Params { R unused; };
     struct Cache {};
+    static constexpr bool SPEEDS_FROM_CACHE = false;
+    __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
     template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
@@ -129,6 +133,11 @@ template <typename R> struct Euler5 {
     // Roe averages kept for the transverse solves of the same interface; the
     // other quantities rpt needs are recomputed with rpn's expressions
     struct Cache { R u, v, enth, a; };
+    // the wave speeds of an interface follow from its cache (rpn's expressions)
+    static constexpr bool SPEEDS_FROM_CACHE = true;
+    __device__ static __forceinline__ void speeds(const Cache &C, R (&s)[NW]) {
+        s[0] = C.u - C.a; s[1] = C.u; s[2] = C.u; s[3] = C.u + C.a; s[4] = C.u;
+    }
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
     template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
@@ -289,6 +298,8 @@ template <typename R> struct ShallowRoe {
     static constexpr int NE = 3, NW = 3, NA = 0;
     struct Params { R grav; };
     struct Cache { R u, v, a; };
+    static constexpr bool SPEEDS_FROM_CACHE = false;
+    __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
     template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
