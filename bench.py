@@ -27,6 +27,11 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 80          # read q (5 x f64) + write q (5 x f64)
+# HBM bytes per launch of the Euler 4096x1024 FP64 kernel measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at the given launch shape, each scaled
+# by the same counter's reading for a 512 MiB device copy (profiles/
+# classic_unsplit_euler_4096x1024.md); reported only when the run uses that shape.
+MEASURED_TRAFFIC = {('f64', 4096, 1024, 24, 2): 399816236}
 
 
 def parse():
@@ -39,6 +44,8 @@ def parse():
     ap.add_argument('--precision', default='f64', choices=['f64', 'f32'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-autotune', action='store_true', help="keep the default row-band height")
+    ap.add_argument('--launch', default=None,
+                    help="ROWS,WAVES: set the Euler kernel's launch shape instead of autotuning (profiling runs)")
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
                     choices=['euler', 'batched', 'sharpclaw', 'acoustics', 'adjoint', 'copy'],
@@ -125,7 +132,11 @@ def main():
         _lib.check(rc, 'claw2d_classic_step_ex')
 
     band = None
-    if not args.no_autotune:
+    if args.launch:
+        rows_, waves_ = (int(v) for v in args.launch.split(','))
+        _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, prec, rows_, waves_), 'claw2d_classic_set_launch')
+        band = {"rows": rows_, "waves_per_simd": waves_}
+    elif not args.no_autotune:
         # pick the kernel's row-band height for this grid (answers do not depend on it)
         hb = ctypes.c_int(0)
         _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, prec, qa.data_ptr(), qb.data_ptr(), None,
@@ -198,7 +209,9 @@ def main():
                                "transverse correction waves, fixed dt at CFL 0.8" % (mx, my),
                    "grid": [mx, my], "global_batch": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (MEASURED_TRAFFIC.get((args.precision, mx, my, band["rows"], band["waves_per_simd"]))
+                                 if band else None),
                      "bytes_per_cell_update": bytes_per_cell,
                      "kernel": "classic_unsplit_kernel<Euler5<%s>>" % ('double' if args.precision == 'f64' else 'float')},
         "cpu_baseline": cpu,
@@ -304,7 +317,11 @@ def bench_sharpclaw(args):
 
     ctx = _lib.Context(0)
     band = None
-    if not args.no_autotune:
+    if args.launch:
+        rows_, waves_ = (int(v) for v in args.launch.split(','))
+        _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, prec, rows_, waves_), 'claw2d_classic_set_launch')
+        band = {"rows": rows_, "waves_per_simd": waves_}
+    elif not args.no_autotune:
         hb = ctypes.c_int(0)
         _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, _lib.F64, q.data_ptr(), qn.data_ptr(), None,
                                                dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
