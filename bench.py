@@ -456,6 +456,21 @@ def bench_adjoint(args):
     t_bwd = time.perf_counter() - t0
     J = float(P.objective(traces))
     finite = bool(torch.isfinite(lam).all()) and bool(torch.isfinite(gaux).all())
+    # every state kept in HBM (every=1: 2000 x 6.4 MB): no recomputation in the reverse sweep
+    del cps
+    P.every = 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    traces1, cps1, _ = P.forward(q0p, aux)
+    torch.cuda.synchronize()
+    t_fwd_all = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lam1, gaux1 = P.backward(traces1, cps1, aux)
+    torch.cuda.synchronize()
+    t_bwd_all = time.perf_counter() - t0
+    same_all = bool(torch.equal(lam1, lam)) and bool(torch.equal(gaux1, gaux))
+    del cps1
+    P.every = 45
     # per-kernel averages on the launch stream: forward step vs adjoint step
     st = torch.cuda.current_stream()
     lam2 = torch.zeros_like(lam)
@@ -489,7 +504,10 @@ def bench_adjoint(args):
            "extra": {"forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_cp, "reverse_s": t_bwd,
                      "reverse_over_forward": t_bwd / t_fwd, "forward_step_us": us_fwd, "adjoint_step_us": us_adj,
                      "adjoint_over_forward_step": us_adj / us_fwd, "J": J, "finite": finite,
-                     "checkpoints": len(cps)}}
+                     "checkpoints": (N + 44) // 45,
+                     "all_states_in_hbm": {"forward_s": t_fwd_all, "reverse_s": t_bwd_all,
+                                           "gradient_over_forward": (t_fwd_all + t_bwd_all) / t_fwd,
+                                           "bitwise_equal_to_checkpointed": same_all}}}
     print(json.dumps(out), flush=True)
 
 

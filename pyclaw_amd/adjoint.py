@@ -15,7 +15,8 @@ as a checkpoint and the pressure at the receiver cells after each step
 segment's states are recomputed from its checkpoint and the adjoint steps
 (claw2d_vc_adjoint_step, hand-written HIP kernels) run backwards through it;
 each receiver residual enters lam at its cell (claw2d_misfit_source).
-Memory: ceil(N/every) checkpoints plus one segment of `every` states.
+Memory: ceil(N/every) checkpoints plus one segment of `every` states; with
+every=1 all N+1 states stay in HBM and nothing is recomputed.
 
 VCAcousticsMisfit.apply(q0, K, rho) is the torch.autograd.Function form.
 """
@@ -134,17 +135,30 @@ class MisfitProblem(object):
         return band.value
 
     def forward(self, q0p, aux, keep_checkpoints=True):
-        """N steps from the padded state q0p; returns (traces (N, nrec), checkpoints, final state)."""
-        N = self.nsteps
+        """N steps from the padded state q0p; returns (traces (N, nrec),
+        checkpoints, final state).  The checkpoints are one preallocated tensor
+        (ceil(N/every), ...) holding states 0, every, 2 every, ...; with
+        every == 1 it holds all N+1 states and the steps write into it directly."""
+        N, E = self.nsteps, self.every
         if not self._tuned:
             self.tune(q0p, aux)
         traces = torch.empty((N, self.nrec), dtype=torch.float64, device=self.dev)
-        cps = []
-        a, b = q0p.clone(), torch.empty_like(q0p)
         self._cfl.zero_()
+        shape = tuple(q0p.shape)
+        if keep_checkpoints and E == 1:
+            states = torch.empty((N + 1,) + shape, dtype=torch.float64, device=self.dev)
+            states[0].copy_(q0p)
+            for n in range(N):
+                self._step(states[n], states[n + 1], aux)
+                self._record(states[n + 1], traces[n])
+            return traces, states, states[N]
+        cps = None
+        if keep_checkpoints:
+            cps = torch.empty(((N + E - 1) // E,) + shape, dtype=torch.float64, device=self.dev)
+        a, b = q0p.clone(), torch.empty_like(q0p)
         for n in range(N):
-            if keep_checkpoints and n % self.every == 0:
-                cps.append(a.clone())
+            if cps is not None and n % E == 0:
+                cps[n // E].copy_(a)
             self._step(a, b, aux)
             self._record(b, traces[n])
             a, b = b, a
@@ -159,8 +173,16 @@ class MisfitProblem(object):
         lam = torch.zeros((3, self.rows, self.ldq), dtype=torch.float64, device=self.dev)
         lam2 = torch.zeros_like(lam)
         gaux = torch.zeros((2, self.rows, self.ldq), dtype=torch.float64, device=self.dev)
-        seg = torch.empty((E,) + tuple(lam.shape), dtype=torch.float64, device=self.dev)
         self._source(lam, traces[N - 1], obs[N - 1])
+        if E == 1:
+            # every state is stored: no recomputation
+            for n in range(N - 1, -1, -1):
+                self._adjoint_step(cps[n], aux, lam, lam2, gaux)
+                lam, lam2 = lam2, lam
+                if n >= 1:
+                    self._source(lam, traces[n - 1], obs[n - 1])
+            return lam, gaux
+        seg = torch.empty((E,) + tuple(lam.shape), dtype=torch.float64, device=self.dev)
         nseg = (N + E - 1) // E
         for k in range(nseg - 1, -1, -1):
             s0, s1 = k * E, min((k + 1) * E, N)

@@ -199,3 +199,26 @@ def test_adjoint_step_is_finite_and_scale_invariant_at_subnormal_wave_norms(scal
     if scale >= 1e-155:
         assert np.abs(g2 - g1).max() <= 1e-10 * np.abs(g1).max()
         assert np.abs(a2 / scale - a1).max() <= 1e-10 * np.abs(a1).max()
+
+
+@pytest.mark.parametrize("every", [1, 4, 17, 40])
+def test_checkpoint_interval_does_not_change_the_gradient(every):
+    """Every checkpoint interval (1: all states kept, no recomputation; larger:
+    segment recomputation, including a last partial segment and one segment
+    longer than the run) gives bit-identical gradients."""
+    from pyclaw_amd.adjoint import MisfitProblem
+    from pyclaw_amd.problems import adjoint_medium
+    mx, my, N = 30, 22, 17
+    rng = np.random.default_rng(3)
+    q0, K, rho, K_true = adjoint_medium(rng, mx, my, width=0.15)
+    dx, dy = 1.0 / mx, 1.0 / my
+    dt = 0.8 * min(dx, dy) / np.sqrt(K / rho).max()
+    rec = [(i, my - 4) for i in range(1, mx - 1, 4)]
+    ref = MisfitProblem(mx, my, dx, dy, dt, N, rec, limiters=4, every=3)
+    obs, _, _ = ref.forward(ref.padded(q0), ref.aux_from(K_true, rho), keep_checkpoints=False)
+    ref.p_obs = obs
+    J0, g0, k0, r0 = ref.gradient(q0, K, rho)
+    P = MisfitProblem(mx, my, dx, dy, dt, N, rec, p_obs=obs, limiters=4, every=every)
+    J1, g1, k1, r1 = P.gradient(q0, K, rho)
+    assert float(J0) == float(J1)
+    assert torch.equal(g0, g1) and torch.equal(k0, k1) and torch.equal(r0, r1)
