@@ -332,7 +332,7 @@ def bench_sharpclaw(args):
         _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, _lib.F64, q.data_ptr(), qn.data_ptr(), None,
                                                dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
                    'claw2d_sharpclaw_autotune')
-        band = hb.value
+        band = {"rows": abs(hb.value), "waves_per_simd": 3 if hb.value < 0 else "default"}
 
     def step(k):
         rc = L.claw2d_sharpclaw_ssp104_step(ctx.handle, g, r, bc, _lib.F64, pair[k % 2].data_ptr(),

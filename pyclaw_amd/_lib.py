@@ -98,6 +98,8 @@ def _optional_symbols():
         "claw2d_sharpclaw_autotune": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann),
                                       ctypes.POINTER(ClawBC), ctypes.c_int, vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(ctypes.c_int)],
+        "claw2d_sharpclaw_set_launch": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int],
         "claw2d_classic_set_launch": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod),
                                       ctypes.POINTER(ClawRiemann), ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "claw2d_vc_adjoint_workspace_bytes": [ctypes.POINTER(ClawGrid)],

@@ -109,7 +109,7 @@ __device__ __forceinline__ int xcd_block_s() {
 }
 
 template <class RP, typename R>
-__global__ __launch_bounds__(256) void sharp_stage_kernel(const SharpParams<RP, R> p) {
+__device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
     constexpr int NA = HAS_AUX ? RP::NA : 1;
@@ -257,6 +257,20 @@ __global__ __launch_bounds__(256) void sharp_stage_kernel(const SharpParams<RP, 
     if (lane == 0) atomic_max_nonneg(p.cfl, c);
 }
 
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void sharp_stage_kernel(const SharpParams<RP, R> p) {
+    sharp_stage_body<RP, R>(p);
+}
+
+// The same kernel compiled for three wavefronts per SIMD (the FP64
+// shallow-water and acoustics instantiations fit without spilling); the
+// SharpClaw autotuner times both builds.
+template <class RP, typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void sharp_stage_kernel_w3(const SharpParams<RP, R> p) {
+    sharp_stage_body<RP, R>(p);
+}
+
 __global__ void zero_slot_kernel(double *p) { *p = 0.0; }
 
 template <class RP, typename R>
@@ -288,12 +302,14 @@ static TuneKey sharp_key(const ClawGrid *grid, const ClawRiemann *rp, int precis
     return TuneKey{rp->kind, SHARP_KEY, 0, precision, grid->mx, grid->my};
 }
 
-// band height for this call: the tuner's candidate, the tuned value, or the default
-static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int strips) {
-    if (ctx && ctx->force_band > 0) return std::min(ctx->force_band, grid->my);
+// band height and build (1: default, 3: three waves per SIMD) for this call:
+// the tuner's candidate, the tuned value, or the default
+static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int strips, int &w) {
+    w = 1;
+    if (ctx && ctx->force_band > 0) { w = ctx->force_w; return std::min(ctx->force_band, grid->my); }
     if (ctx) {
         auto it = ctx->band.find(sharp_key(grid, rp, precision));
-        if (it != ctx->band.end()) return std::min(it->second.first, grid->my);
+        if (it != ctx->band.end()) { w = it->second.second; return std::min(it->second.first, grid->my); }
     }
     return sharp_band_height(grid->my, strips);
 }
@@ -311,12 +327,16 @@ static int launch_sharp(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, c
     p.P = make_sharp_params<RP, R>(rp);
     p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy; p.mode = mode;
     p.nsx = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
-    p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_F64, p.nsx);
+    int w = 1;
+    p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_F64, p.nsx, w);
     p.nsy = (grid->my + p.H - 1) / p.H;
     p.wpb = wpb;
     p.nsyb = (p.nsy + wpb - 1) / wpb;
     if (zero_cfl) hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, st, cfl);
-    hipLaunchKernelGGL((sharp_stage_kernel<RP, R>), dim3(p.nsx * p.nsyb), dim3(WAVE * wpb), 0, st, p);
+    if (w == 3)
+        hipLaunchKernelGGL((sharp_stage_kernel_w3<RP, R>), dim3(p.nsx * p.nsyb), dim3(WAVE * wpb), 0, st, p);
+    else
+        hipLaunchKernelGGL((sharp_stage_kernel<RP, R>), dim3(p.nsx * p.nsyb), dim3(WAVE * wpb), 0, st, p);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 
@@ -410,6 +430,13 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
     return CLAW_OK;
 }
 
+int claw2d_sharpclaw_set_launch(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int band,
+                                int waves_per_simd) {
+    if (!ctx || !grid || !rp || band < 1 || (waves_per_simd != 1 && waves_per_simd != 3)) return CLAW_EINVAL;
+    ((Ctx *)ctx)->band[sharp_key(grid, rp, precision)] = std::make_pair(band, waves_per_simd);
+    return CLAW_OK;
+}
+
 int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
                               int precision, const void *q, void *dq, const void *aux, const double *dt_dev,
                               double *cfl_dev, void *stream, int *band_out) {
@@ -425,11 +452,13 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
         return CLAW_EDEVICE;
     }
     const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
-    int best = 0;
+    int best = 0, best_w = 1;
     float best_ms = 0.f;
-    for (int h : cand) {
-        if (h > grid->my && h != cand[0]) break;
+    for (int k = 0; k < 18; ++k) {
+        const int h = cand[k % 9], w = k < 9 ? 1 : 3;
+        if (h > grid->my && h != cand[0]) continue;
         c->force_band = h;
+        c->force_w = w;
         rc = sharp_dispatch(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, false, st);
         if (rc) break;
         const int reps = 5;
@@ -441,14 +470,15 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
         if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (best == 0 || ms < best_ms) { best = h; best_ms = ms; }
+        if (best == 0 || ms < best_ms) { best = h; best_w = w; best_ms = ms; }
     }
     c->force_band = 0;
+    c->force_w = 0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    c->band[sharp_key(grid, rp, precision)] = std::make_pair(best, 1);
-    if (band_out) *band_out = best;
+    c->band[sharp_key(grid, rp, precision)] = std::make_pair(best, best_w);
+    if (band_out) *band_out = best_w == 3 ? -best : best;
     return CLAW_OK;
 }
 
