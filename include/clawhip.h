@@ -227,6 +227,42 @@ int claw2d_step_update(int n, double cfl_max, double cfl_desired, double dt_max,
                        double *cfl, double *t, double *cflmax, int32_t *parity,
                        int32_t *naccepted, int32_t *nrejected, void *stream);
 
+/* ---- Discrete adjoint of the classic step (variable-coefficient acoustics) ---
+   No counterpart in the reference, which has no adjoint.  The forward map is
+   claw2d_classic_step with CLAW_RP_VC_ACOUSTICS (aux = (rho, c), c > 0 in
+   every cell), unsplit, order 1 or 2 with any limiter, transverse 0/1/2, at a
+   fixed dt, with boundary codes extrap, periodic or wall for q and aux
+   (stored ghost cells are not differentiated: CLAW_EINVAL). */
+
+/* Bytes of device workspace claw2d_vc_adjoint_step needs for this grid. */
+long long claw2d_vc_adjoint_workspace_bytes(const ClawGrid *grid);
+
+/* lam_out = (d q_{n+1} / d q_n)^T lam_in, and gaux += (d q_{n+1} / d aux)^T lam_in,
+   with q = q_n (the state the forward step started from).  lam_in is read at
+   interior cells only; lam_out is written at interior cells; gaux is
+   accumulated at interior cells (component 0: rho, 1: c).  lam_in != lam_out.
+   Deterministic (fixed summation order, no atomics). */
+int claw2d_vc_adjoint_step(const ClawGrid *grid, const ClawMethod *method, const ClawBC *bc,
+                           const double *q, const double *aux, const double *lam_in, double *lam_out,
+                           double *gaux, double dt, void *workspace, long long workspace_bytes,
+                           void *stream);
+
+/* The same step followed by the receiver-misfit source of the previous step:
+   lam_out[offsets[r]] += traces[r] - observed[r] for r < nrec (component 0
+   offsets into the padded layout; receivers must be distinct interior cells). */
+int claw2d_vc_adjoint_step_misfit(const ClawGrid *grid, const ClawMethod *method, const ClawBC *bc,
+                                  const double *q, const double *aux, const double *lam_in,
+                                  double *lam_out, double *gaux, double dt, void *workspace,
+                                  long long workspace_bytes, const long long *offsets, int nrec,
+                                  const double *traces, const double *observed, void *stream);
+
+/* out[k] = a[offsets[k]] (receiver traces). */
+int claw2d_gather(const double *a, const long long *offsets, int n, double *out, void *stream);
+
+/* a[offsets[k]] += traces[k] - observed[k] (distinct offsets). */
+int claw2d_misfit_source(double *a, const long long *offsets, int n, const double *traces,
+                         const double *observed, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

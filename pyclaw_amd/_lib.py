@@ -105,6 +105,9 @@ def _optional_symbols():
         "claw2d_vc_adjoint_workspace_bytes": [ctypes.POINTER(ClawGrid)],
         "claw2d_vc_adjoint_step": [ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod), ctypes.POINTER(ClawBC),
                                    vp, vp, vp, vp, vp, ctypes.c_double, vp, ctypes.c_longlong, vp],
+        "claw2d_vc_adjoint_step_misfit": [ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod),
+                                          ctypes.POINTER(ClawBC), vp, vp, vp, vp, vp, ctypes.c_double, vp,
+                                          ctypes.c_longlong, vp, ctypes.c_int, vp, vp, vp],
         "claw2d_gather": [vp, vp, ctypes.c_int, vp, vp],
         "claw2d_misfit_source": [vp, vp, ctypes.c_int, vp, vp, vp],
     }

@@ -69,6 +69,8 @@ class MisfitProblem(object):
         for (i, j) in rec:
             if not (0 <= i < self.mx and 0 <= j < self.my):
                 raise ValueError("receiver (%d, %d) outside the grid" % (i, j))
+        if len(set(rec)) != len(rec):
+            raise ValueError("receivers must be distinct cells")
         self.nrec = len(rec)
         self.offsets = torch.tensor([(j + ng) * self.ldq + (i + ng) for (i, j) in rec], dtype=torch.int64,
                                     device=self.dev)
@@ -101,10 +103,12 @@ class MisfitProblem(object):
     def _stream(self):
         return torch.cuda.current_stream(self.dev).cuda_stream
 
-    def _step(self, qold, qnew, aux):
+    def _step(self, qold, qnew, aux, stream=None, cfl=None):
         rc = self.L.claw2d_classic_step_ex(self._ctx.handle, self.grid, self.method, self.rp, self.bc, _lib.F64,
                                            qold.data_ptr(), qnew.data_ptr(), aux.data_ptr(), self._dt.data_ptr(),
-                                           self._cfl.data_ptr(), _lib.STEP_ACCUMULATE_CFL, self._stream())
+                                           (self._cfl if cfl is None else cfl).data_ptr(),
+                                           _lib.STEP_ACCUMULATE_CFL,
+                                           self._stream() if stream is None else stream.cuda_stream)
         _lib.check(rc, "claw2d_classic_step_ex")
 
     def _record(self, q, out):
@@ -116,11 +120,25 @@ class MisfitProblem(object):
                                                traces_n.data_ptr(), obs_n.data_ptr(), self._stream()),
                    "claw2d_misfit_source")
 
+    def _adjoint_step_misfit(self, q, aux, lam_in, lam_out, gaux, traces_n, obs_n):
+        """Adjoint step, then the residual of the previous step's traces enters lam_out."""
+        rc = self.L.claw2d_vc_adjoint_step_misfit(self.grid, self.method, self.bc, q.data_ptr(), aux.data_ptr(),
+                                                  lam_in.data_ptr(), lam_out.data_ptr(), gaux.data_ptr(), self.dt,
+                                                  self._ws.data_ptr(), self._ws.numel() * 8,
+                                                  self.offsets.data_ptr(), self.nrec, traces_n.data_ptr(),
+                                                  obs_n.data_ptr(), self._stream())
+        _lib.check(rc, "claw2d_vc_adjoint_step_misfit")
+
     def _adjoint_step(self, q, aux, lam_in, lam_out, gaux):
         rc = self.L.claw2d_vc_adjoint_step(self.grid, self.method, self.bc, q.data_ptr(), aux.data_ptr(),
                                            lam_in.data_ptr(), lam_out.data_ptr(), gaux.data_ptr(), self.dt,
                                            self._ws.data_ptr(), self._ws.numel() * 8, self._stream())
         _lib.check(rc, "claw2d_vc_adjoint_step")
+
+    def _side_stream(self):
+        if getattr(self, "_side", None) is None:
+            self._side = torch.cuda.Stream(self.dev)     # default priority
+        return self._side
 
     # ---- passes ------------------------------------------------------------
     def tune(self, q0p, aux):
@@ -177,23 +195,50 @@ class MisfitProblem(object):
         if E == 1:
             # every state is stored: no recomputation
             for n in range(N - 1, -1, -1):
-                self._adjoint_step(cps[n], aux, lam, lam2, gaux)
-                lam, lam2 = lam2, lam
                 if n >= 1:
-                    self._source(lam, traces[n - 1], obs[n - 1])
+                    self._adjoint_step_misfit(cps[n], aux, lam, lam2, gaux, traces[n - 1], obs[n - 1])
+                else:
+                    self._adjoint_step(cps[n], aux, lam, lam2, gaux)
+                lam, lam2 = lam2, lam
             return lam, gaux
-        seg = torch.empty((E,) + tuple(lam.shape), dtype=torch.float64, device=self.dev)
+        # Segments are recomputed on a second stream into the other of two
+        # buffers while the adjoint steps of the previous segment run: the
+        # recomputation of segment k-1 does not depend on segment k's adjoint.
+        main = torch.cuda.current_stream(self.dev)
+        side = self._side_stream()
+        seg = [torch.empty((E,) + tuple(lam.shape), dtype=torch.float64, device=self.dev) for _ in range(2)]
+        ready = [torch.cuda.Event() for _ in range(2)]
+        free = [torch.cuda.Event() for _ in range(2)]
+        cfl_side = torch.zeros(1, dtype=torch.float64, device=self.dev)
         nseg = (N + E - 1) // E
-        for k in range(nseg - 1, -1, -1):
+
+        def recompute(k, b):
             s0, s1 = k * E, min((k + 1) * E, N)
-            seg[0].copy_(cps[k])
-            for n in range(s0, s1 - 1):
-                self._step(seg[n - s0], seg[n - s0 + 1], aux)
+            with torch.cuda.stream(side):
+                side.wait_event(free[b])
+                seg[b][0].copy_(cps[k])
+                for n in range(s0, s1 - 1):
+                    self._step(seg[b][n - s0], seg[b][n - s0 + 1], aux, stream=side, cfl=cfl_side)
+                ready[b].record(side)
+
+        side.wait_stream(main)           # checkpoints and aux are complete
+        for b in range(2):
+            free[b].record(main)
+        recompute(nseg - 1, (nseg - 1) % 2)
+        for k in range(nseg - 1, -1, -1):
+            b = k % 2
+            if k >= 1:
+                recompute(k - 1, (k - 1) % 2)
+            main.wait_event(ready[b])
+            s0, s1 = k * E, min((k + 1) * E, N)
             for n in range(s1 - 1, s0 - 1, -1):
-                self._adjoint_step(seg[n - s0], aux, lam, lam2, gaux)
-                lam, lam2 = lam2, lam
                 if n >= 1:
-                    self._source(lam, traces[n - 1], obs[n - 1])
+                    self._adjoint_step_misfit(seg[b][n - s0], aux, lam, lam2, gaux, traces[n - 1], obs[n - 1])
+                else:
+                    self._adjoint_step(seg[b][n - s0], aux, lam, lam2, gaux)
+                lam, lam2 = lam2, lam
+            free[b].record(main)
+        main.wait_stream(side)
         return lam, gaux
 
     def objective(self, traces, p_obs=None):

@@ -47,6 +47,9 @@ namespace adj {
 constexpr int ANG = 2;   // ghost width of the classic step
 
 struct Params {
+    const long long *rec_off = nullptr;   // receivers (distinct interior cells, component-0 offsets)
+    const double *rec_tr = nullptr, *rec_obs = nullptr;   // their traces and observed values at this step
+    int nrec = 0;
     const double *q;      // q_n, padded planar layout
     const double *aux;    // (rho, c), padded planar layout
     const double *lam;    // lam_{n+1}, interior cells read only
@@ -668,12 +671,28 @@ __global__ __launch_bounds__(256) void adjoint_tile_kernel(const Params P) {
     if (i < P.g.mx + ANG && j < P.g.my + ANG) position_write(A, i, j);
 }
 
+// Ring cells fold their ghosts; then the misfit's residuals (trace minus
+// observed) enter lam at the receiver cells: a ring cell adds its own (after
+// its fold), every other receiver has a thread of its own.  Receivers are
+// distinct cells, so no cell is written by two threads.
 __global__ __launch_bounds__(256) void adjoint_fold_kernel(const Params P) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ring_count(P)) return;
-    int i, j;
-    ring_cell(P, k, i, j);
-    fold_one(P, i, j);
+    const long long nring = ring_count(P);
+    if (k < nring) {
+        int i, j;
+        ring_cell(P, k, i, j);
+        fold_one(P, i, j);
+        const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
+        for (int r = 0; r < P.nrec; ++r)
+            if (P.rec_off[r] == off) P.lam_out[off] += P.rec_tr[r] - P.rec_obs[r];
+        return;
+    }
+    const long long r = k - nring;
+    if (r >= P.nrec) return;
+    const long long off = P.rec_off[r];
+    const int j = (int)(off / P.g.ldq) - P.g.ng, i = (int)(off % P.g.ldq) - P.g.ng;
+    if (i < ANG || j < ANG || i >= P.g.mx - ANG || j >= P.g.my - ANG) return;   // a ring cell: done above
+    P.lam_out[off] += P.rec_tr[r] - P.rec_obs[r];
 }
 
 }  // namespace adj
@@ -693,17 +712,14 @@ __global__ void misfit_source_kernel(double *a, const long long *off, int n, con
 
 using namespace clawhip;
 
-extern "C" {
+extern "C" long long claw2d_vc_adjoint_workspace_bytes(const ClawGrid *grid);
 
-long long claw2d_vc_adjoint_workspace_bytes(const ClawGrid *grid) {
-    if (!grid) return -1;
-    return 5LL * grid->plane * (long long)sizeof(double);
-}
-
-int claw2d_vc_adjoint_step(const ClawGrid *grid, const ClawMethod *meth, const ClawBC *bc, const double *q,
-                           const double *aux, const double *lam_in, double *lam_out, double *gaux, double dt,
-                           void *workspace, long long workspace_bytes, void *stream) {
+static int vc_adjoint_launch(const ClawGrid *grid, const ClawMethod *meth, const ClawBC *bc, const double *q,
+                             const double *aux, const double *lam_in, double *lam_out, double *gaux, double dt,
+                             void *workspace, long long workspace_bytes, const long long *rec_off, int nrec,
+                             const double *rec_tr, const double *rec_obs, void *stream) {
     if (!grid || !meth || !bc || !q || !aux || !lam_in || !lam_out || !gaux || !workspace) return CLAW_EINVAL;
+    if (nrec < 0 || (nrec > 0 && (!rec_off || !rec_tr || !rec_obs))) return CLAW_EINVAL;
     if (lam_in == lam_out) return CLAW_EINVAL;
     if (grid->mx < 2 || grid->my < 2 || grid->ng < adj::ANG) return CLAW_EINVAL;
     if (grid->ldq < grid->mx + 2 * grid->ng || grid->plane < grid->ldq * (grid->my + 2 * grid->ng)) return CLAW_EINVAL;
@@ -725,12 +741,36 @@ int claw2d_vc_adjoint_step(const ClawGrid *grid, const ClawMethod *meth, const C
     P.dtdx = dt / grid->dx; P.dtdy = dt / grid->dy;
     P.order = meth->order; P.trans = meth->transverse;
     P.mth[0] = meth->mthlim[0]; P.mth[1] = meth->mthlim[1];
+    P.rec_off = rec_off; P.rec_tr = rec_tr; P.rec_obs = rec_obs; P.nrec = nrec;
     hipStream_t st = (hipStream_t)stream;
     const dim3 tiles((grid->mx + 2 * adj::ANG + adj::TW - 1) / adj::TW, (grid->my + 2 * adj::ANG + adj::TH - 1) / adj::TH);
     hipLaunchKernelGGL(adj::adjoint_tile_kernel, tiles, dim3(256), 0, st, P);
-    const long long nring = adj::ring_count(P);
-    hipLaunchKernelGGL(adj::adjoint_fold_kernel, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, st, P);
+    const long long nfold = adj::ring_count(P) + nrec;
+    hipLaunchKernelGGL(adj::adjoint_fold_kernel, dim3((unsigned)((nfold + 255) / 256)), dim3(256), 0, st, P);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+extern "C" {
+
+long long claw2d_vc_adjoint_workspace_bytes(const ClawGrid *grid) {
+    if (!grid) return -1;
+    return 5LL * grid->plane * (long long)sizeof(double);
+}
+
+
+int claw2d_vc_adjoint_step(const ClawGrid *grid, const ClawMethod *meth, const ClawBC *bc, const double *q,
+                           const double *aux, const double *lam_in, double *lam_out, double *gaux, double dt,
+                           void *workspace, long long workspace_bytes, void *stream) {
+    return vc_adjoint_launch(grid, meth, bc, q, aux, lam_in, lam_out, gaux, dt, workspace, workspace_bytes, nullptr, 0,
+                             nullptr, nullptr, stream);
+}
+
+int claw2d_vc_adjoint_step_misfit(const ClawGrid *grid, const ClawMethod *meth, const ClawBC *bc, const double *q,
+                                  const double *aux, const double *lam_in, double *lam_out, double *gaux, double dt,
+                                  void *workspace, long long workspace_bytes, const long long *offsets, int nrec,
+                                  const double *traces, const double *observed, void *stream) {
+    return vc_adjoint_launch(grid, meth, bc, q, aux, lam_in, lam_out, gaux, dt, workspace, workspace_bytes, offsets,
+                             nrec, traces, observed, stream);
 }
 
 int claw2d_gather(const double *a, const long long *offsets, int n, double *out, void *stream) {
