@@ -81,8 +81,8 @@ def cpu_baseline(q0, dx, dy, dt, mx, my, nsteps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the grid would shard)", file=sys.stderr)
+        print("bench.py measures one GPU: multi-GPU runs (domain decomposition with halo "
+              "exchange, or sharded batches) are not part of this implementation", file=sys.stderr)
         sys.exit(2)
     if args.workload != 'euler':
         return OTHER[args.workload](args)
