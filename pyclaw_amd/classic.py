@@ -7,6 +7,7 @@ transverse_waves (no_trans / trans_inc / trans_cor), step(), setup(),
 teardown().  step_hyperbolic calls the fused HIP kernel through
 claw2d_classic_step instead of the f2py step2/step2ds.
 """
+import ctypes
 import warnings
 
 import torch
@@ -154,6 +155,16 @@ class ClawSolver2D(ClawSolver):
         self._dt_dev.fill_(float(self.dt))
         aux = state._aux.buf.data_ptr() if state._aux is not None else None
         stream = torch.cuda.current_stream().cuda_stream
+        key = (grid.mx, grid.my, prec, self._method_struct.order, self._method_struct.transverse,
+               self._method_struct.split, self._rp_struct.kind)
+        if self.autotune and key not in self._tuned:
+            band = ctypes.c_int(0)
+            rc = _lib.lib().claw2d_classic_autotune(self._ctx.handle, grid, self._method_struct, self._rp_struct,
+                                                    bc, prec, state._q.buf.data_ptr(), self._qalt.data_ptr(), aux,
+                                                    self._dt_dev.data_ptr(), self._cfl_dev.data_ptr(), stream,
+                                                    ctypes.byref(band))
+            _lib.check(rc, "claw2d_classic_autotune")
+            self._tuned.add(key)
         rc = _lib.lib().claw2d_classic_step(self._ctx.handle, grid, self._method_struct, self._rp_struct, bc,
                                             prec, state._q.buf.data_ptr(), self._qalt.data_ptr(), aux,
                                             self._dt_dev.data_ptr(), self._cfl_dev.data_ptr(), stream)

@@ -14,6 +14,8 @@ then applied on the host: the first stage above cfl_max aborts the step
 (CFLError, sharpclaw/solver.py:276-277) and q is left unchanged, otherwise the
 last stage's number is the step's (cfl.update_global_max per dq call).
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -152,11 +154,32 @@ class SharpClawSolver(Solver):
         state.sync_to_device()
         has_custom = any(b == 0 for b in self.bc_lower + self.bc_upper) and not self.custom_bc_static
         simple = (self.dq_src is None and not self.call_before_step_each_stage and not has_custom)
+        if self.autotune:
+            self._tune(state)
         if simple:
             ok = self._step_fused(state)
         else:
             ok = self._step_staged(state)
         return ok
+
+    def _tune(self, state):
+        """Time the stage kernel's launch shapes once per grid shape
+        (claw2d_sharpclaw_autotune); the choice never changes results."""
+        grid = self._grid_struct(state)
+        prec = _lib.F32 if state.dtype == torch.float32 else _lib.F64
+        key = (grid.mx, grid.my, prec, self._rp_struct.kind)
+        if key in self._tuned:
+            return
+        bc = self._kernel_bcs(state)
+        self._dt_dev.fill_(float(self.dt))
+        aux = state._aux.buf.data_ptr() if state._aux is not None else None
+        band = ctypes.c_int(0)
+        rc = _lib.lib().claw2d_sharpclaw_autotune(self._ctx.handle, grid, self._rp_struct, bc, prec,
+                                                  state._q.buf.data_ptr(), self._qalt.data_ptr(), aux,
+                                                  self._dt_dev.data_ptr(), self._cfl_stages.data_ptr(),
+                                                  torch.cuda.current_stream().cuda_stream, ctypes.byref(band))
+        _lib.check(rc, "claw2d_sharpclaw_autotune")
+        self._tuned.add(key)
 
     def _launch_stage(self, state, mode, qin, q0, out, cfl_ptr, bc):
         grid = self._grid_struct(state)

@@ -124,6 +124,10 @@ class Solver(object):
         self._cfl_dev = None
         self._static_filled = set()
         self._backup_buf = None
+        # When True, the first step on each grid shape times the kernel's launch
+        # shapes and keeps the fastest for the solver's context; results do not change.
+        self.autotune = False
+        self._tuned = set()
         if riemann_solver is not None:
             self.rp = riemann_solver
             self.num_eqn = riemann_solver.num_eqn
@@ -171,6 +175,7 @@ class Solver(object):
         state.sync_to_device()
         if self._ctx is None:
             self._ctx = _lib.Context(dev.index)
+            self._tuned = set()
         buf = state._q.buf
         if self._qalt is None or self._qalt.shape != buf.shape or self._qalt.dtype != buf.dtype \
                 or self._qalt.device != buf.device:

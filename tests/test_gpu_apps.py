@@ -20,7 +20,7 @@ RP = {'acoustics': 'rp2_acoustics', 'vc_acoustics': 'rp2_vc_acoustics', 'euler':
       'shallow': 'rp2_shallow_roe_with_efix'}
 
 
-def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None):
+def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False):
     import pyclaw_amd as pc
     x = pc.Dimension('x', opts['lower'][0], opts['upper'][0], opts['mx'])
     y = pc.Dimension('y', opts['lower'][1], opts['upper'][1], opts['my'])
@@ -41,6 +41,7 @@ def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None):
     else:
         solver = pc.SharpClawSolver2D(rp)
     solver.limiters = opts['limiters']
+    solver.autotune = autotune
     solver.cfl_max = opts['cfl_max']
     solver.cfl_desired = opts['cfl_desired']
     solver.dt_initial = opts['dt_initial']
@@ -117,4 +118,15 @@ def test_variable_acoustics_sharpclaw():
     got, status = amd_run(q0, aux, opts)
     assert np.abs(GOLD['acoustics_vc_pressure_classic'] - got[20][0]).max() < 1e-1
     ref, _ = oracle_run(q0, aux, opts)
+    _compare_frames(got, ref)
+
+
+@pytest.mark.parametrize("kind", ['classic', 'sharpclaw'])
+def test_autotune_keeps_frames(kind):
+    """solver.autotune = True times the kernel's launch shapes on the first
+    step; every frame must match the untuned run bit for bit."""
+    print("app: acoustics_2d_variable %s autotune" % kind, flush=True)
+    q0, aux, opts = apps.acoustics_variable(kind)
+    ref, _ = amd_run(q0, aux, opts)
+    got, _ = amd_run(q0, aux, opts, autotune=True)
     _compare_frames(got, ref)
