@@ -228,7 +228,7 @@ def main():
 
 def bench_batched(args):
     """512 independent layered vc-acoustics grids (sizes from {64,...,512}^2),
-    200 steps each with its own CFL-0.9 dt: one hipGraph-captured sequence of
+    --steps steps each (300 by default) with its own CFL-0.9 dt: one hipGraph-captured sequence of
     persistent work-queue launches vs the same grids stepped one at a time
     through the plain single-grid entry point."""
     import torch
