@@ -540,7 +540,7 @@ __host__ __device__ __forceinline__ void position_raw(const PT &P, int i, int j,
 // Writes the raw contributions at position (i, j) of the padded region
 // [-ANG, mx+ANG) x [-ANG, my+ANG): interior cells get lam_n = lam_{n+1} + raw
 // and their aux gradient; ghost positions go to the frame workspace, folded
-// back into their source cells by fold_one.
+// back into their source cells by the fold launch (fold_field).
 template <class PT>
 __host__ __device__ __forceinline__ void position_write(const PT &A, int i, int j) {
     double gq[3], grho, gc;
@@ -584,34 +584,41 @@ __host__ __device__ __forceinline__ void ring_cell(const Params &P, long long k,
     }
 }
 
-// adds the frame contributions of every ghost position copied from cell (i, j)
-__host__ __device__ __forceinline__ void fold_one(const Params &P, int i, int j) {
+// Sum, in a fixed order, of field m's frame contributions over the ghost
+// positions that the boundary map copies from ring cell (i, j): the candidates
+// are (gi, gj) with gi in {i, -2, -1, mx, mx+1} and gj in {j, -2, -1, my, my+1};
+// the map is separable (x index from the x codes, y index from the y codes;
+// bc_source) and a wall negates the normal momentum it reflects.  Every
+// candidate is a ghost position inside the frame, so all 24 loads are issued
+// unconditionally and the match selects them: no load waits on another.
+__host__ __device__ __forceinline__ double fold_field(const Params &P, int i, int j, int m) {
     const int mx = P.g.mx, my = P.g.my;
-    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
-    double gq[3] = {0.0, 0.0, 0.0}, grho = 0.0, gc = 0.0;
+    const BCDev &bc = m < 3 ? P.bq : P.ba;
+    const double *src = P.frame + (long long)m * P.g.plane;
     const int gis[5] = {i, -2, -1, mx, mx + 1};
     const int gjs[5] = {j, -2, -1, my, my + 1};
+    bool mxa[5], mya[5], fxa[5], fya[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+        bool f = false;
+        const int cx = a <= 2 ? bc.lo[0] : bc.hi[0], cy = a <= 2 ? bc.lo[1] : bc.hi[1];
+        mxa[a] = a == 0 || bc_map1(gis[a], mx, cx, f) == i;
+        fxa[a] = a != 0 && m == 1 && cx == CLAW_BC_WALL;
+        mya[a] = a == 0 || bc_map1(gjs[a], my, cy, f) == j;
+        fya[a] = a != 0 && m == 2 && cy == CLAW_BC_WALL;
+    }
+    double acc = 0.0;
+#pragma unroll
     for (int b = 0; b < 5; ++b) {
+#pragma unroll
         for (int a = 0; a < 5; ++a) {
             if (a == 0 && b == 0) continue;
-            const int gi = gis[a], gj = gjs[b];
-            const long long goff = (long long)(gj + P.g.ng) * P.g.ldq + (gi + P.g.ng);
-            const SrcCell sq = bc_source(gi, gj, P.g, P.bq);
-            if (sq.off == off) {
-                gq[0] += P.frame[goff];
-                gq[1] += sq.fx ? -P.frame[P.g.plane + goff] : P.frame[P.g.plane + goff];
-                gq[2] += sq.fy ? -P.frame[2 * P.g.plane + goff] : P.frame[2 * P.g.plane + goff];
-            }
-            const SrcCell sa = bc_source(gi, gj, P.g, P.ba);
-            if (sa.off == off) {
-                grho += P.frame[3 * P.g.plane + goff];
-                gc += P.frame[4 * P.g.plane + goff];
-            }
+            const double v = src[(long long)(gjs[b] + P.g.ng) * P.g.ldq + (gis[a] + P.g.ng)];
+            const bool neg = fxa[a] != fya[b];
+            acc += (mxa[a] && mya[b]) ? (neg ? -v : v) : 0.0;
         }
     }
-    for (int m = 0; m < 3; ++m) P.lam_out[m * P.g.plane + off] += gq[m];
-    P.gaux[off] += grho;
-    P.gaux[P.g.plane + off] += gc;
+    return acc;
 }
 
 // Wave strengths and 1/D of every face inside the staged box (faces need both
@@ -671,28 +678,51 @@ __global__ __launch_bounds__(256) void adjoint_tile_kernel(const Params P) {
     if (i < P.g.mx + ANG && j < P.g.my + ANG) position_write(A, i, j);
 }
 
-// Ring cells fold their ghosts; then the misfit's residuals (trace minus
-// observed) enter lam at the receiver cells: a ring cell adds its own (after
-// its fold), every other receiver has a thread of its own.  Receivers are
-// distinct cells, so no cell is written by two threads.
-__global__ __launch_bounds__(256) void adjoint_fold_kernel(const Params P) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Thread t < 5 * nring folds field m = t / nring of ring cell t % nring (lam
+// for m < 3, the aux gradient for m = 3, 4); the misfit's residuals (trace
+// minus observed) then enter lam at the receiver cells: a ring cell's pressure
+// thread adds its own after its fold, every other receiver has a thread of its
+// own (t >= 5 * nring).  Receivers are distinct cells, so no element is written
+// by two threads.  The ring threads find their receiver with a wave-wide scan
+// of the receiver offsets (64 per load, broadcast lane by lane).
+__global__ __launch_bounds__(128) void adjoint_fold_kernel(const Params P) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long nring = ring_count(P);
-    if (k < nring) {
-        int i, j;
-        ring_cell(P, k, i, j);
-        fold_one(P, i, j);
-        const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
-        for (int r = 0; r < P.nrec; ++r)
-            if (P.rec_off[r] == off) P.lam_out[off] += P.rec_tr[r] - P.rec_obs[r];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const long long wave_t0 = t - lane;
+    const bool fold = t < 5 * nring;
+    const int m = fold ? (int)(t / nring) : 0;
+    int i = 0, j = 0;
+    if (fold) ring_cell(P, t - m * nring, i, j);
+    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
+    int hit = -1;
+    if (P.nrec > 0 && wave_t0 < nring) {   // wave-uniform: some lane folds a pressure
+        for (int r0 = 0; r0 < P.nrec; r0 += WAVE) {
+            const int r = r0 + lane;
+            const long long ro = r < P.nrec ? P.rec_off[r] : -1;
+            const int lo = (int)(unsigned)(ro & 0xffffffffLL), hi = (int)(ro >> 32);
+            const int cnt = P.nrec - r0 < WAVE ? P.nrec - r0 : WAVE;
+            for (int s = 0; s < cnt; ++s) {
+                const long long o = ((long long)__builtin_amdgcn_readlane(hi, s) << 32) |
+                                    (long long)(unsigned)__builtin_amdgcn_readlane(lo, s);
+                if (o == off) hit = r0 + s;
+            }
+        }
+    }
+    if (fold) {
+        const double acc = fold_field(P, i, j, m);
+        double *dst = m < 3 ? P.lam_out + (long long)m * P.g.plane + off : P.gaux + (long long)(m - 3) * P.g.plane + off;
+        double v = *dst + acc;
+        if (m == 0 && hit >= 0) v += P.rec_tr[hit] - P.rec_obs[hit];
+        *dst = v;
         return;
     }
-    const long long r = k - nring;
+    const long long r = t - 5 * nring;
     if (r >= P.nrec) return;
-    const long long off = P.rec_off[r];
-    const int j = (int)(off / P.g.ldq) - P.g.ng, i = (int)(off % P.g.ldq) - P.g.ng;
-    if (i < ANG || j < ANG || i >= P.g.mx - ANG || j >= P.g.my - ANG) return;   // a ring cell: done above
-    P.lam_out[off] += P.rec_tr[r] - P.rec_obs[r];
+    const long long ro = P.rec_off[r];
+    const int rj = (int)(ro / P.g.ldq) - P.g.ng, ri = (int)(ro % P.g.ldq) - P.g.ng;
+    if (ri < ANG || rj < ANG || ri >= P.g.mx - ANG || rj >= P.g.my - ANG) return;   // a ring cell: done above
+    P.lam_out[ro] += P.rec_tr[r] - P.rec_obs[r];
 }
 
 }  // namespace adj
@@ -745,8 +775,8 @@ static int vc_adjoint_launch(const ClawGrid *grid, const ClawMethod *meth, const
     hipStream_t st = (hipStream_t)stream;
     const dim3 tiles((grid->mx + 2 * adj::ANG + adj::TW - 1) / adj::TW, (grid->my + 2 * adj::ANG + adj::TH - 1) / adj::TH);
     hipLaunchKernelGGL(adj::adjoint_tile_kernel, tiles, dim3(256), 0, st, P);
-    const long long nfold = adj::ring_count(P) + nrec;
-    hipLaunchKernelGGL(adj::adjoint_fold_kernel, dim3((unsigned)((nfold + 255) / 256)), dim3(256), 0, st, P);
+    const long long nfold = 5 * adj::ring_count(P) + nrec;
+    hipLaunchKernelGGL(adj::adjoint_fold_kernel, dim3((unsigned)((nfold + 127) / 128)), dim3(128), 0, st, P);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 
