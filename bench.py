@@ -9,6 +9,11 @@ timed region; dt is fixed at the value giving Courant number 0.8 on the
 initial data (the kernel still computes and max-combines the Courant number
 of every step; it is reported).
 
+The headline runs the FP64 kernels built with fused multiply-add (precision
+CLAW_F64_FMA); the same line reports the reference-arithmetic build's speed
+and the per-field difference between the two after one step and after the
+whole run.  --precision f64 times the bit-exact build instead.
+
 Prints one JSON line (see the keys below).  --gpus N>1 is not supported: this
 build targets one GPU.
 """
@@ -27,11 +32,11 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 80          # read q (5 x f64) + write q (5 x f64)
-# HBM bytes per launch of the Euler 4096x1024 FP64 kernel measured with
+# HBM bytes per launch of the Euler 4096x1024 FP64 kernels measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at the given launch shape, each scaled
 # by the same counter's reading for a 512 MiB device copy (profiles/
 # classic_unsplit_euler_4096x1024.md); reported only when the run uses that shape.
-MEASURED_TRAFFIC = {('f64', 4096, 1024, 24, 2): 399816236}
+MEASURED_TRAFFIC = {('f64', 4096, 1024, 24, 2): 399816236, ('f64fma', 4096, 1024, 24, 2): 396847663}
 
 
 def parse():
@@ -41,12 +46,14 @@ def parse():
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--mx', type=int, default=4096)
     ap.add_argument('--my', type=int, default=1024)
-    ap.add_argument('--precision', default='f64', choices=['f64', 'f32'])
+    ap.add_argument('--precision', default='f64fma', choices=['f64', 'f64fma', 'f32'],
+                    help="f64fma (default): FP64 with fused multiply-add, within rounding of the reference "
+                         "arithmetic per step; f64: the reference arithmetic bit for bit; f32")
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-autotune', action='store_true', help="keep the default row-band height")
     ap.add_argument('--launch', default=None,
                     help="ROWS,WAVES: set the Euler kernel's launch shape instead of autotuning; the "
-                         "4096x1024 FP64 line defaults to 24,2 (the tuner's pick on that grid, as profiled)")
+                         "4096x1024 FP64 lines default to 24,2 (the tuner's pick on that grid, as profiled)")
     ap.add_argument('--autotune', action='store_true', help="autotune the launch shape even where a default is pinned")
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
@@ -95,7 +102,7 @@ def main():
     smax = euler_max_speed(q0)
     dt = 0.8 * min(dx, dy) / smax
     dev = torch.device('cuda', 0)
-    dtype = torch.float64 if args.precision == 'f64' else torch.float32
+    dtype = torch.float32 if args.precision == 'f32' else torch.float64
     ng = 3
     nxp, nyp = mx + 2 * ng, my + 2 * ng
     buf = torch.zeros((5, nyp, nxp), dtype=torch.float64)
@@ -122,7 +129,7 @@ def main():
     L = _lib.lib()
     dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
     cfl = torch.zeros((), dtype=torch.float64, device=dev)
-    prec = _lib.F64 if args.precision == 'f64' else _lib.F32
+    prec = {'f64': _lib.F64, 'f64fma': _lib.F64_FMA, 'f32': _lib.F32}[args.precision]
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     bufs = [qa, qb]
@@ -134,11 +141,13 @@ def main():
         _lib.check(rc, 'claw2d_classic_step_ex')
 
     band = None
-    if args.launch is None and not args.autotune and (args.precision, mx, my) == ('f64', 4096, 1024):
+    if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
         args.launch = '24,2'
     if args.launch:
         rows_, waves_ = (int(v) for v in args.launch.split(','))
-        _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, prec, rows_, waves_), 'claw2d_classic_set_launch')
+        for pr_ in {prec, _lib.F64}:
+            _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, pr_, rows_, waves_),
+                       'claw2d_classic_set_launch')
         band = {"rows": rows_, "waves_per_simd": waves_}
     elif not args.no_autotune:
         # pick the kernel's row-band height for this grid (answers do not depend on it)
@@ -165,7 +174,7 @@ def main():
     ms_per_step = gpu_ms / args.steps
     cells = mx * my
     value = cells * args.steps / (gpu_ms * 1e-3) / 1e9
-    bytes_per_cell = BYTES_PER_CELL if args.precision == 'f64' else BYTES_PER_CELL // 2
+    bytes_per_cell = BYTES_PER_CELL // 2 if args.precision == 'f32' else BYTES_PER_CELL
     achieved = bytes_per_cell * cells / (ms_per_step * 1e-3) / 1e9
     cfl_max = float(cfl.item())
     qfin = bufs[(args.warmup + args.steps) % 2]
@@ -187,6 +196,46 @@ def main():
                  max(float(r64[4].abs().max()), 1e-300)]
         f32_err = [float(((r32[c] - r64[c]).abs().max() / scale[c]).item()) for c in range(5)]
 
+    fma_err = None
+    if args.precision == 'f64fma':
+        # one step from the final state through both FP64 builds (untimed), and
+        # the whole run repeated with the reference-arithmetic build
+        fin = qfin.clone()
+        te, tf = torch.zeros_like(fin), torch.zeros_like(fin)
+        for out_, pr_ in ((te, _lib.F64), (tf, _lib.F64_FMA)):
+            out_.copy_(fin)
+            _lib.check(L.claw2d_classic_step_ex(ctx.handle, grid, m, r, bc, pr_, fin.data_ptr(), out_.data_ptr(),
+                                                None, dtd.data_ptr(), cfl.data_ptr(), _lib.STEP_ACCUMULATE_CFL, sh),
+                       'comparison step')
+        # the same run through the reference-arithmetic build, its timed steps
+        # bracketed like the headline's
+        ref = [buf.to(dev, torch.float64), None]
+        ref[1] = ref[0].clone()
+        f0 = torch.cuda.Event(enable_timing=True)
+        f1 = torch.cuda.Event(enable_timing=True)
+        for k in range(args.warmup + args.steps):
+            if k == args.warmup:
+                torch.cuda.synchronize()
+                f0.record(stream)
+            _lib.check(L.claw2d_classic_step_ex(ctx.handle, grid, m, r, bc, _lib.F64, ref[k % 2].data_ptr(),
+                                                ref[(k + 1) % 2].data_ptr(), None, dtd.data_ptr(), cfl.data_ptr(),
+                                                _lib.STEP_ACCUMULATE_CFL, sh), 'f64 reference step')
+        f1.record(stream)
+        torch.cuda.synchronize()
+        ref_ms = f0.elapsed_time(f1) / args.steps
+        rfin = ref[(args.warmup + args.steps) % 2]
+        sl = (slice(None), slice(ng, ng + my), slice(ng, ng + mx))
+
+        def per_field(a, b):
+            # per field, relative to the field's scale (both momenta share the momentum scale)
+            a, b = a[sl], b[sl]
+            scale = [b[0].abs().max(), b[1:3].abs().max(), b[1:3].abs().max(), b[3].abs().max(), b[4].abs().max()]
+            return [float(((a[c] - b[c]).abs().max() / scale[c]).item()) for c in range(5)]
+        fma_err = {"reference_arithmetic_kernel_gcell_s": cells / (ref_ms * 1e-3) / 1e9,
+                   "reference_arithmetic_kernel_ms_per_step": ref_ms,
+                   "one_step_rel_linf_per_field": per_field(tf, te),
+                   "after_%d_steps_rel_linf_per_field" % (args.warmup + args.steps): per_field(qfin, rfin)}
+
     cpu = None
     if not args.no_cpu_baseline:
         v, el = cpu_baseline(q0, dx, dy, dt, mx, my, args.cpu_sample_steps)
@@ -207,21 +256,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": 'f32' if args.precision == 'f32' else 'f64',
         "data": "synthetic (shock-bubble initial data, generated in-process)",
         "config": {"workload": "classic 2D Euler shock-bubble %dx%d, unsplit, Roe+efix, MC on 5 waves, "
-                               "transverse correction waves, fixed dt at CFL 0.8" % (mx, my),
+                               "transverse correction waves, fixed dt at CFL 0.8%s" % (
+                                   mx, my, ", FP64 with fused multiply-add" if args.precision == 'f64fma' else ""),
                    "grid": [mx, my], "global_batch": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (MEASURED_TRAFFIC.get((args.precision, mx, my, band["rows"], band["waves_per_simd"]))
                                  if band else None),
                      "bytes_per_cell_update": bytes_per_cell,
-                     "kernel": "classic_unsplit_kernel<Euler5<%s>>" % ('double' if args.precision == 'f64' else 'float')},
+                     "kernel": "%sclassic_unsplit_kernel%s<Euler5<%s>>" % (
+                         'fused::' if args.precision == 'f64fma' else '',
+                         '_w2' if band and band["waves_per_simd"] == 2 else '',
+                         'float' if args.precision == 'f32' else 'double')},
         "cpu_baseline": cpu,
         "extra": {"cfl_max_over_timed_steps": cfl_max, "finite": finite, "dt": dt, "wall_s": wall,
                   "hbm_pct": 100.0 * achieved / HBM_PEAK_GBS, "launch_shape": band,
-                  "f32_rel_linf_vs_f64_per_field": f32_err},
+                  "f32_rel_linf_vs_f64_per_field": f32_err, "fma_vs_reference_arithmetic": fma_err},
     }
     print(json.dumps(out), flush=True)
 
@@ -321,11 +374,13 @@ def bench_sharpclaw(args):
 
     ctx = _lib.Context(0)
     band = None
-    if args.launch is None and not args.autotune and (args.precision, mx, my) == ('f64', 4096, 1024):
+    if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
         args.launch = '24,2'
     if args.launch:
         rows_, waves_ = (int(v) for v in args.launch.split(','))
-        _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, prec, rows_, waves_), 'claw2d_classic_set_launch')
+        for pr_ in {prec, _lib.F64}:
+            _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, pr_, rows_, waves_),
+                       'claw2d_classic_set_launch')
         band = {"rows": rows_, "waves_per_simd": waves_}
     elif not args.no_autotune:
         hb = ctypes.c_int(0)

@@ -47,7 +47,12 @@ enum {
    they are in the array (the caller filled them, e.g. for user/custom BCs). */
 enum { CLAW_BC_STORED = 0, CLAW_BC_EXTRAP = 1, CLAW_BC_PERIODIC = 2, CLAW_BC_WALL = 3 };
 
-enum { CLAW_F64 = 0, CLAW_F32 = 1 };
+/* Arithmetic of a step.  CLAW_F64: double precision with every multiply and
+   add rounded separately, as the reference's CPU build (bit-identical
+   results).  CLAW_F32: single precision.  CLAW_F64_FMA: double precision with
+   multiply-adds fused (one rounding); the classic step only; results agree
+   with CLAW_F64 to rounding (tested at 1e-13 relative per step). */
+enum { CLAW_F64 = 0, CLAW_F32 = 1, CLAW_F64_FMA = 2 };
 
 typedef struct {
     int32_t mx, my;      /* interior cells                                     */
@@ -89,7 +94,8 @@ int claw_ctx_destroy(void *ctx);
    (src/pyclaw/classic/solver.py:498-547, step2.f90, step2ds.f90).
    dt_dev: device double, the step size.  cfl_dev: device double, receives the
    Courant number of the step (max over the same interfaces as the reference);
-   it is overwritten.  precision: CLAW_F64 (reference arithmetic) or CLAW_F32.
+   it is overwritten.  precision: CLAW_F64 (reference arithmetic), CLAW_F64_FMA
+   or CLAW_F32.
    aux may be NULL when the solver uses no aux fields. */
 int claw2d_classic_step(void *ctx, const ClawGrid *grid, const ClawMethod *method,
                         const ClawRiemann *rp, const ClawBC *bc, int precision,

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libclawhip.so")
 CLAW_OK, CLAW_EINVAL, CLAW_EDEVICE, CLAW_ENOMEM, CLAW_ETOOLARGE = 0, 1, 2, 3, 4
 RP_ACOUSTICS, RP_VC_ACOUSTICS, RP_EULER, RP_SHALLOW = 0, 1, 2, 3
 BC_STORED, BC_EXTRAP, BC_PERIODIC, BC_WALL = 0, 1, 2, 3
-F64, F32 = 0, 1
+F64, F32, F64_FMA = 0, 1, 2
 STEP_ACCUMULATE_CFL = 1
 SM_DQ, SM_EULER, SM_SSP104_A, SM_SSP104_MID, SM_SSP104_END, SM_SSP33_2, SM_SSP33_3 = range(7)
 

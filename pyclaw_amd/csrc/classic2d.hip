@@ -39,7 +39,19 @@
 #include "ctx.hpp"
 #include "riemann.hpp"
 
+// The kernels and their launcher are compiled twice: here without
+// floating-point contraction (every multiply and add rounded on its own, as in
+// the reference's CPU build), in namespace clawhip::exact, and from
+// classic2d_fma.hip with contraction on, in clawhip::fused, for the CLAW_F64_FMA
+// precision.  The batched runs and the C entry points are compiled here only.
+#ifndef CLAW_CLASSIC_VARIANT
+#define CLAW_CLASSIC_VARIANT exact
+#define CLAW_CLASSIC_MAIN 1
+#define CLAW_CLASSIC_F64_KEY CLAW_F64
+#endif
+
 namespace clawhip {
+namespace CLAW_CLASSIC_VARIANT {
 
 constexpr int XS_USEFUL = WAVE - 3;   // produced columns per wavefront, split x pass
 constexpr int XU_USEFUL = WAVE - 4;   // produced columns per wavefront, unsplit step
@@ -671,6 +683,7 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
 
 __global__ void zero_scalar_kernel(double *p) { *p = 0.0; }
 
+#ifdef CLAW_CLASSIC_MAIN
 // ---------------------------------------------------------------------------
 // Batched runs: many independent grids of different sizes advanced together.
 // One launch per step: a persistent grid of wavefronts pulls (instance, strip,
@@ -766,6 +779,7 @@ __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
     if (k == 0) *counter = 0;
     if (k < n) cfl[k] = 0.0;
 }
+#endif  // CLAW_CLASSIC_MAIN
 
 
 // ---------------------------------------------------------------------------
@@ -829,7 +843,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
             p.H = std::min(ctx->force_band, grid->my);
             w2 = ctx->force_w == 2;
         } else {
-            auto it = ctx->band.find(tune_key(grid, meth, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_F64));
+            auto it = ctx->band.find(tune_key(grid, meth, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_CLASSIC_F64_KEY));
             if (it != ctx->band.end()) { p.H = std::min(it->second.first, grid->my); w2 = it->second.second == 2; }
         }
         p.nsy = (grid->my + p.H - 1) / p.H;
@@ -879,6 +893,19 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
     return check(hipGetLastError());
 }
 
+#ifndef CLAW_CLASSIC_MAIN
+// FP64 step of this build (arguments validated by claw2d_classic_step_ex).
+int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
+                     const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
+                     hipStream_t st) {
+    switch (rp->kind) {
+    case CLAW_RP_ACOUSTICS: return launch_classic<Acoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    case CLAW_RP_VC_ACOUSTICS: return launch_classic<VCAcoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    case CLAW_RP_EULER: return launch_classic<Euler5<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    default: return launch_classic<ShallowRoe<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    }
+}
+#else
 
 // ---- batched runs: host side -----------------------------------------------
 struct BatchPlan {
@@ -963,9 +990,21 @@ static void batch_free(BatchPlan *pl) {
     (void)hipFree(pl->d_counter); (void)hipFree(pl->d_cfl);
     delete pl;
 }
+#endif  // CLAW_CLASSIC_MAIN
+}  // namespace CLAW_CLASSIC_VARIANT
+}  // namespace clawhip
+
+#ifdef CLAW_CLASSIC_MAIN
+namespace clawhip {
+namespace fused {
+int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
+                     const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
+                     hipStream_t st);
+}  // namespace fused
 }  // namespace clawhip
 
 using namespace clawhip;
+using namespace clawhip::exact;
 
 extern "C" {
 
@@ -1023,6 +1062,8 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
     }
     if (meth->num_waves != nw_expected) return CLAW_EINVAL;
     const bool f32 = (precision == CLAW_F32);
+    if (precision == CLAW_F64_FMA)
+        return fused::classic_step_f64(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
     if (precision != CLAW_F32 && precision != CLAW_F64) return CLAW_EINVAL;
 #define DISPATCH(RPT)                                                                                      \
     return f32 ? launch_classic<RPT<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st) \
@@ -1197,3 +1238,4 @@ int claw2d_step_update(int n, double cfl_max, double cfl_desired, double dt_max,
 }
 
 }  // extern "C"
+#endif  // CLAW_CLASSIC_MAIN

@@ -73,7 +73,7 @@ def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False
     ne, nxp, nyp = qbc_ref.shape
     g = (nxp - (nxp - 4)) // 2
     mx, my = nxp - 2 * g, nyp - 2 * g
-    dtype = torch.float64 if precision == 'f64' else torch.float32
+    dtype = torch.float32 if precision == 'f32' else torch.float64
     dev = torch.device('cuda', 0)
 
     def to_dev(a):
@@ -105,7 +105,8 @@ def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False
     dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
     cfl = torch.zeros((), dtype=torch.float64, device=dev)
     ctx = _lib.Context(0)
-    rc = _lib.lib().claw2d_classic_step(ctx.handle, grid, m, r, bc, _lib.F32 if precision == 'f32' else _lib.F64,
+    prec = {'f64': _lib.F64, 'f32': _lib.F32, 'f64fma': _lib.F64_FMA}[precision]
+    rc = _lib.lib().claw2d_classic_step(ctx.handle, grid, m, r, bc, prec,
                                         qd.data_ptr(), qn.data_ptr(), ad.data_ptr() if ad is not None else None,
                                         dtd.data_ptr(), cfl.data_ptr(),
                                         torch.cuda.current_stream().cuda_stream)
