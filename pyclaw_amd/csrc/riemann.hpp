@@ -31,6 +31,18 @@ enum RiemannKind { RP_ACOUSTICS = 0, RP_VC_ACOUSTICS = 1, RP_EULER = 2, RP_SHALL
 template <typename R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
 template <typename R> __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 
+// Division policy of the Euler solver.  By default every quotient is the
+// reference's (x / y, correctly rounded, in the reference's order).  A build
+// that defines CLAW_SHARED_RECIPROCALS (the fused-multiply-add FP64 build)
+// divides once where the reference divides several values by the same
+// denominator and multiplies by that reciprocal: a few ulp from the
+// reference's quotients, about a third fewer FP64 division sequences.
+#ifdef CLAW_SHARED_RECIPROCALS
+constexpr bool SHARED_RCP = true;
+#else
+constexpr bool SHARED_RCP = false;
+#endif
+
 template <typename R> __device__ __forceinline__ R rsqrt_exact(R x);
 template <> __device__ __forceinline__ double rsqrt_exact<double>(double x) { return __builtin_sqrt(x); }
 template <> __device__ __forceinline__ float rsqrt_exact<float>(float x) { return __builtin_sqrtf(x); }
@@ -154,12 +166,26 @@ template <typename R> struct Euler5 {
         const R gamma1 = P.gamma1;
         R rhsqrtl = rsqrt_exact<R>(lo[0]);
         R rhsqrtr = rsqrt_exact<R>(hi[0]);
-        R pl = gamma1 * (lo[3] - R(0.5) * (lo[1] * lo[1] + lo[2] * lo[2]) / lo[0]);
-        R pr = gamma1 * (hi[3] - R(0.5) * (hi[1] * hi[1] + hi[2] * hi[2]) / hi[0]);
-        R rhsq2 = rhsqrtl + rhsqrtr;
-        R u = (lo[mu] / rhsqrtl + hi[mu] / rhsqrtr) / rhsq2;
-        R v = (lo[mv] / rhsqrtl + hi[mv] / rhsqrtr) / rhsq2;
-        R enth = ((lo[3] + pl) / rhsqrtl + (hi[3] + pr) / rhsqrtr) / rhsq2;
+        R pl, pr, u, v, enth;
+        R il0 = R(0), ih0 = R(0);   // 1/rho of each side (shared-reciprocal build)
+        if constexpr (SHARED_RCP) {
+            il0 = R(1) / lo[0];
+            ih0 = R(1) / hi[0];
+            const R irl = R(1) / rhsqrtl, irr = R(1) / rhsqrtr;
+            pl = gamma1 * (lo[3] - R(0.5) * (lo[1] * lo[1] + lo[2] * lo[2]) * il0);
+            pr = gamma1 * (hi[3] - R(0.5) * (hi[1] * hi[1] + hi[2] * hi[2]) * ih0);
+            const R irs = R(1) / (rhsqrtl + rhsqrtr);
+            u = (lo[mu] * irl + hi[mu] * irr) * irs;
+            v = (lo[mv] * irl + hi[mv] * irr) * irs;
+            enth = ((lo[3] + pl) * irl + (hi[3] + pr) * irr) * irs;
+        } else {
+            pl = gamma1 * (lo[3] - R(0.5) * (lo[1] * lo[1] + lo[2] * lo[2]) / lo[0]);
+            pr = gamma1 * (hi[3] - R(0.5) * (hi[1] * hi[1] + hi[2] * hi[2]) / hi[0]);
+            R rhsq2 = rhsqrtl + rhsqrtr;
+            u = (lo[mu] / rhsqrtl + hi[mu] / rhsqrtr) / rhsq2;
+            v = (lo[mv] / rhsqrtl + hi[mv] / rhsqrtr) / rhsq2;
+            enth = ((lo[3] + pl) / rhsqrtl + (hi[3] + pr) / rhsqrtr) / rhsq2;
+        }
         R u2v2 = u * u + v * v;
         R a2 = gamma1 * (enth - R(0.5) * u2v2);
         R a = rsqrt_exact<R>(a2);
@@ -203,10 +229,16 @@ template <typename R> struct Euler5 {
         // rpn2_euler_5wave.f:205-286: amdq with the transonic-rarefaction fix
         do {
             R sfract;
-            R rhoim1 = lo[0];
-            R pim1 = gamma1 * (lo[3] - R(0.5) * (lo[mu] * lo[mu] + lo[mv] * lo[mv]) / rhoim1);
-            R cim1 = rsqrt_exact<R>(gamma * pim1 / rhoim1);
-            R s0 = lo[mu] / rhoim1 - cim1;
+            R s0;
+            if constexpr (SHARED_RCP) {
+                // the fix's pim1 is the pressure pl above
+                s0 = lo[mu] * il0 - rsqrt_exact<R>(gamma * pl * il0);
+            } else {
+                R rhoim1 = lo[0];
+                R pim1 = gamma1 * (lo[3] - R(0.5) * (lo[mu] * lo[mu] + lo[mv] * lo[mv]) / rhoim1);
+                R cim1 = rsqrt_exact<R>(gamma * pim1 / rhoim1);
+                s0 = lo[mu] / rhoim1 - cim1;
+            }
             if (s0 >= R(0) && s[0] > R(0)) {
 #pragma unroll
                 for (int m = 0; m < NE; ++m) amdq[m] = R(0);
@@ -216,9 +248,16 @@ template <typename R> struct Euler5 {
             R rhou1 = lo[mu] + wave[0][mu];
             R rhov1 = lo[mv] + wave[0][mv];
             R en1 = lo[3] + wave[0][3];
-            R p1 = gamma1 * (en1 - R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
-            R c1 = rsqrt_exact<R>(gamma * p1 / rho1);
-            R s1 = rhou1 / rho1 - c1;
+            R s1;
+            if constexpr (SHARED_RCP) {
+                const R i1 = R(1) / rho1;
+                R p1 = gamma1 * (en1 - R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) * i1);
+                s1 = rhou1 * i1 - rsqrt_exact<R>(gamma * p1 * i1);
+            } else {
+                R p1 = gamma1 * (en1 - R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+                R c1 = rsqrt_exact<R>(gamma * p1 / rho1);
+                s1 = rhou1 / rho1 - c1;
+            }
             if (s0 < R(0) && s1 > R(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
             else if (s[0] < R(0)) sfract = s[0];
             else sfract = R(0);
@@ -231,17 +270,25 @@ template <typename R> struct Euler5 {
                 amdq[m] = amdq[m] + s[2] * wave[2][m];
                 amdq[m] = amdq[m] + s[4] * wave[4][m];
             }
-            R rhoi = hi[0];
-            R pi = gamma1 * (hi[3] - R(0.5) * (hi[mu] * hi[mu] + hi[mv] * hi[mv]) / rhoi);
-            R ci = rsqrt_exact<R>(gamma * pi / rhoi);
-            R s3 = hi[mu] / rhoi + ci;
+            R s3, s2;
             R rho2 = hi[0] - wave[3][0];
             R rhou2 = hi[mu] - wave[3][mu];
             R rhov2 = hi[mv] - wave[3][mv];
             R en2 = hi[3] - wave[3][3];
-            R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
-            R c2 = rsqrt_exact<R>(gamma * p2 / rho2);
-            R s2 = rhou2 / rho2 + c2;
+            if constexpr (SHARED_RCP) {
+                s3 = hi[mu] * ih0 + rsqrt_exact<R>(gamma * pr * ih0);
+                const R i2 = R(1) / rho2;
+                R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) * i2);
+                s2 = rhou2 * i2 + rsqrt_exact<R>(gamma * p2 * i2);
+            } else {
+                R rhoi = hi[0];
+                R pi = gamma1 * (hi[3] - R(0.5) * (hi[mu] * hi[mu] + hi[mv] * hi[mv]) / rhoi);
+                R ci = rsqrt_exact<R>(gamma * pi / rhoi);
+                s3 = hi[mu] / rhoi + ci;
+                R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+                R c2 = rsqrt_exact<R>(gamma * p2 / rho2);
+                s2 = rhou2 / rho2 + c2;
+            }
             if (s2 < R(0) && s3 > R(0)) sfract = s2 * (s3 - s[3]) / (s3 - s2);
             else if (s[3] < R(0)) sfract = s[3];
             else break;
@@ -267,7 +314,11 @@ template <typename R> struct Euler5 {
         const R euv = enth - u2v2;
         R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
         R a2 = asdq[mu] - u * asdq[0];
-        R a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
+        R a4;
+        if constexpr (SHARED_RCP)   // 1/(2a) is shared by the interface's two transverse solves
+            a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) * (R(1) / (R(2) * a));
+        else
+            a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
         R a1 = asdq[0] - a3 - a4;
         R wb[4][NE], sb[4];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); wb[0][3] = a1 * (enth - v * a);
