@@ -36,7 +36,7 @@ BYTES_PER_CELL = 80          # read q (5 x f64) + write q (5 x f64)
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at the given launch shape, each scaled
 # by the same counter's reading for a 512 MiB device copy (profiles/
 # classic_unsplit_euler_4096x1024.md); reported only when the run uses that shape.
-MEASURED_TRAFFIC = {('f64', 4096, 1024, 24, 2): 399816236, ('f64fma', 4096, 1024, 24, 2): 396847663}
+MEASURED_TRAFFIC = {('f64', 4096, 1024, 24, 2, 4): 399816236, ('f64fma', 4096, 1024, 36, 2, 1): 377795580}
 
 
 def parse():
@@ -52,8 +52,9 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-autotune', action='store_true', help="keep the default row-band height")
     ap.add_argument('--launch', default=None,
-                    help="ROWS,WAVES: set the Euler kernel's launch shape instead of autotuning; the "
-                         "4096x1024 FP64 lines default to 24,2 (the tuner's pick on that grid, as profiled)")
+                    help="ROWS,WAVES[,WPB]: set the Euler kernel's launch shape (row-band height, waves per "
+                         "SIMD build, waves per block) instead of autotuning; the "
+                         "4096x1024 FP64 lines default to 36,2,1 (as profiled)")
     ap.add_argument('--autotune', action='store_true', help="autotune the launch shape even where a default is pinned")
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
@@ -142,20 +143,26 @@ def main():
 
     band = None
     if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
-        args.launch = '24,2'
+        # one 36-row band per wavefront slot (69 strips x 29 bands on 2048 slots), one-wavefront
+        # blocks: the fastest shape in repeated 300-step runs (the tuner's short bursts from the
+        # initial data rank 24 rows marginally ahead)
+        args.launch = '36,2,1'
     if args.launch:
-        rows_, waves_ = (int(v) for v in args.launch.split(','))
+        shape = [int(v) for v in args.launch.split(',')]
+        rows_, waves_, wpb_ = shape[0], shape[1], (shape[2] if len(shape) > 2 else 4)
         for pr_ in {prec, _lib.F64}:
-            _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, pr_, rows_, waves_),
-                       'claw2d_classic_set_launch')
-        band = {"rows": rows_, "waves_per_simd": waves_}
+            _lib.check(L.claw2d_classic_set_launch_ex(ctx.handle, grid, m, r, pr_, rows_, waves_, wpb_),
+                       'claw2d_classic_set_launch_ex')
     elif not args.no_autotune:
-        # pick the kernel's row-band height for this grid (answers do not depend on it)
+        # pick the kernel's launch shape for this grid (answers do not depend on it)
         hb = ctypes.c_int(0)
         _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, prec, qa.data_ptr(), qb.data_ptr(), None,
                                              dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
                    'claw2d_classic_autotune')
-        band = {"rows": abs(hb.value), "waves_per_simd": 2 if hb.value < 0 else 1}
+    sb, sw, sp = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _lib.check(L.claw2d_classic_get_launch(ctx.handle, grid, m, r, prec, ctypes.byref(sb), ctypes.byref(sw),
+                                           ctypes.byref(sp)), 'claw2d_classic_get_launch')
+    band = {"rows": sb.value, "waves_per_simd": sw.value, "waves_per_block": sp.value}
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
@@ -264,12 +271,12 @@ def main():
                    "grid": [mx, my], "global_batch": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (MEASURED_TRAFFIC.get((args.precision, mx, my, band["rows"], band["waves_per_simd"]))
-                                 if band else None),
+                     "traffic": MEASURED_TRAFFIC.get((args.precision, mx, my, band["rows"], band["waves_per_simd"],
+                                                      band["waves_per_block"])),
                      "bytes_per_cell_update": bytes_per_cell,
                      "kernel": "%sclassic_unsplit_kernel%s<Euler5<%s>>" % (
                          'fused::' if args.precision == 'f64fma' else '',
-                         '_w2' if band and band["waves_per_simd"] == 2 else '',
+                         '_w2' if band["waves_per_simd"] == 2 else '',
                          'float' if args.precision == 'f32' else 'double')},
         "cpu_baseline": cpu,
         "extra": {"cfl_max_over_timed_steps": cfl_max, "finite": finite, "dt": dt, "wall_s": wall,
@@ -375,7 +382,10 @@ def bench_sharpclaw(args):
     ctx = _lib.Context(0)
     band = None
     if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
-        args.launch = '24,2'
+        # one 36-row band per wavefront slot (69 strips x 29 bands on 2048 slots), one-wavefront
+        # blocks: the fastest shape in repeated 300-step runs (the tuner's short bursts from the
+        # initial data rank 24 rows marginally ahead)
+        args.launch = '36,2,1'
     if args.launch:
         rows_, waves_ = (int(v) for v in args.launch.split(','))
         for pr_ in {prec, _lib.F64}:

@@ -114,8 +114,10 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
 
 /* Autotuner of the unsplit step's launch shape: times the fused kernel for a
    set of row-band heights (the rows one wavefront marches through; more
-   bands mean more wavefronts in flight, fewer mean fewer halo rows), each
-   with the kernel compiled for one and for two wavefronts per SIMD, and
+   bands mean more wavefronts in flight, fewer mean fewer halo rows; the
+   candidates include the height that makes one band per wavefront slot of
+   the device), each with the kernel compiled for one and for two
+   wavefronts per SIMD and with blocks of one or four wavefronts, and
    keeps the fastest for this context and problem shape (Riemann solver,
    order, transverse mode, precision, mx, my); later classic steps of that
    shape use it.  Results do not depend on the launch shape (every cell is
@@ -136,6 +138,18 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
    that tune once and reuse the shape.  No counterpart in the reference. */
 int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod *method,
                               const ClawRiemann *rp, int precision, int band, int waves_per_simd);
+
+/* Same, also setting the wavefronts per block (1, 2 or 4; the shape set by
+   claw2d_classic_set_launch uses 4).  No counterpart in the reference. */
+int claw2d_classic_set_launch_ex(void *ctx, const ClawGrid *grid, const ClawMethod *method,
+                                 const ClawRiemann *rp, int precision, int band, int waves_per_simd,
+                                 int waves_per_block);
+
+/* The launch shape classic steps of this problem shape use on this context
+   (tuned, set, or the default).  No counterpart in the reference. */
+int claw2d_classic_get_launch(void *ctx, const ClawGrid *grid, const ClawMethod *method,
+                              const ClawRiemann *rp, int precision, int *band, int *waves_per_simd,
+                              int *waves_per_block);
 
 /* Number of kernel launches one classic step issues (for timing). */
 int claw2d_classic_launches(const ClawMethod *method);

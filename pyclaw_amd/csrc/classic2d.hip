@@ -396,14 +396,23 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     atomic_max_nonneg_all_lanes(p.cfl, wave_max((double)cfl));
 }
 
+// Blocks of p.wpb wavefronts (1, 2 or 4), each with its own carry slots in
+// dynamic LDS (p.wpb x CarrySlots::COUNT x WAVE values).
+extern __shared__ __attribute__((aligned(16))) unsigned char classic_carry_lds[];
+
 template <class RP, typename R>
-__global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
+__device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
     const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
     if (sx >= p.nsx || sy >= p.nsy) return;
-    __shared__ R carry[4 * CarrySlots<RP>::COUNT * WAVE];
+    R *carry = reinterpret_cast<R *>(classic_carry_lds);
     unsplit_band<RP, R>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+}
+
+template <class RP, typename R>
+__global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
+    unsplit_block<RP, R>(p);
 }
 
 // The same kernel compiled for two wavefronts per SIMD: the register budget
@@ -412,12 +421,7 @@ __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParam
 template <class RP, typename R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void classic_unsplit_kernel_w2(const ClassicParams<RP, R> p) {
-    const int vb = xcd_block();
-    const int sx = vb / p.nsyb;
-    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-    if (sx >= p.nsx || sy >= p.nsy) return;
-    __shared__ R carry[4 * CarrySlots<RP>::COUNT * WAVE];
-    unsplit_band<RP, R>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+    unsplit_block<RP, R>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -785,6 +789,18 @@ __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Band height that gives about one band per wavefront slot of the device
+// (CUs x 4 SIMDs x waves per SIMD): a single round of wavefronts, none left
+// over for a second one.
+static int balanced_band(int device, int rows, int strips_x, int waves_per_simd) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        cus = 256;
+    const long long slots = (long long)cus * 4 * waves_per_simd;
+    const long long bands = std::max<long long>(1, slots / std::max(strips_x, 1));
+    return (int)std::max<long long>(4, (rows + bands - 1) / bands);
+}
+
 static int band_height(int rows, int strips_x) {
     // about 2048 wavefronts (two per SIMD) with bands of at least 4 rows; small
     // grids trade the 3 extra rows each band marches through for more waves
@@ -823,7 +839,7 @@ template <class RP, typename R>
 static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
                           const ClawBC *bc, const void *qold, void *qnew, const void *aux,
                           const double *dt, double *cfl, int flags, hipStream_t st) {
-    const int wpb = 4;
+    int wpb = 4;
     GridDev g{grid->mx, grid->my, grid->ng, grid->ldq, grid->plane};
     BCDev bq{{bc->lower[0], bc->lower[1]}, {bc->upper[0], bc->upper[1]}};
     BCDev ba{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
@@ -842,18 +858,23 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         if (ctx->force_band > 0) {
             p.H = std::min(ctx->force_band, grid->my);
             w2 = ctx->force_w == 2;
+            wpb = ctx->force_wpb > 0 ? ctx->force_wpb : wpb;
         } else {
-            auto it = ctx->band.find(tune_key(grid, meth, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_CLASSIC_F64_KEY));
+            const TuneKey key = tune_key(grid, meth, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_CLASSIC_F64_KEY);
+            auto it = ctx->band.find(key);
             if (it != ctx->band.end()) { p.H = std::min(it->second.first, grid->my); w2 = it->second.second == 2; }
+            auto iw = ctx->wpb.find(key);
+            if (iw != ctx->wpb.end()) wpb = iw->second;
         }
         p.nsy = (grid->my + p.H - 1) / p.H;
         p.wpb = wpb;
         p.nsyb = (p.nsy + wpb - 1) / wpb;
         int nblocks = p.nsx * p.nsyb;
+        const size_t lds = (size_t)wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
         if (w2)
-            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
+            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
         else
-            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
+            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
         return check(hipGetLastError());
     }
     // split: x pass into scratch (padded layout, ghost rows included), then y pass
@@ -1079,9 +1100,36 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
 
 int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
                               int precision, int band, int waves_per_simd) {
+    return claw2d_classic_set_launch_ex(ctx, grid, meth, rp, precision, band, waves_per_simd, 4);
+}
+
+int claw2d_classic_set_launch_ex(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                                 int precision, int band, int waves_per_simd, int waves_per_block) {
     if (!ctx || !grid || !meth || !rp) return CLAW_EINVAL;
     if (band < 1 || (waves_per_simd != 1 && waves_per_simd != 2)) return CLAW_EINVAL;
-    ((Ctx *)ctx)->band[tune_key(grid, meth, rp, precision)] = std::make_pair(band, waves_per_simd);
+    if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) return CLAW_EINVAL;
+    Ctx *c = (Ctx *)ctx;
+    const TuneKey key = tune_key(grid, meth, rp, precision);
+    c->band[key] = std::make_pair(band, waves_per_simd);
+    c->wpb[key] = waves_per_block;
+    return CLAW_OK;
+}
+
+int claw2d_classic_get_launch(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                              int precision, int *band, int *waves_per_simd, int *waves_per_block) {
+    if (!ctx || !grid || !meth || !rp || !band || !waves_per_simd || !waves_per_block) return CLAW_EINVAL;
+    Ctx *c = (Ctx *)ctx;
+    const TuneKey key = tune_key(grid, meth, rp, precision);
+    auto it = c->band.find(key);
+    auto iw = c->wpb.find(key);
+    if (it == c->band.end()) {   // the default shape
+        *band = std::min(band_height(grid->my, (grid->mx + XU_USEFUL - 1) / XU_USEFUL), grid->my);
+        *waves_per_simd = 1;
+    } else {
+        *band = std::min(it->second.first, grid->my);
+        *waves_per_simd = it->second.second;
+    }
+    *waves_per_block = iw == c->wpb.end() ? 4 : iw->second;
     return CLAW_OK;
 }
 
@@ -1100,15 +1148,22 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         if (e0) (void)hipEventDestroy(e0);
         return CLAW_EDEVICE;
     }
-    const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
-    int best = 0, best_w = 1;
+    // row-band heights x (1 or 2 wavefronts per SIMD) x (1 or 4 wavefronts per
+    // block); the list ends with the balanced height of each build
+    const int strips = (grid->mx + XU_USEFUL - 1) / XU_USEFUL;
+    const int cand[11] = {4, 6, 8, 12, 16, 24, 32, 48, 64, balanced_band(c->device, grid->my, strips, 1),
+                          balanced_band(c->device, grid->my, strips, 2)};
+    int best = 0, best_w = 1, best_wpb = 4;
     float best_ms = 0.f;
     int rc = CLAW_OK;
-    for (int k = 0; k < 18 && !rc; ++k) {
-        const int h = cand[k % 9], w = k < 9 ? 1 : 2;
+    for (int k = 0; k < 44 && !rc; ++k) {
+        const int h = cand[k % 11], w = (k / 11) % 2 == 0 ? 1 : 2, wpb = k < 22 ? 4 : 1;
         if (h > grid->my && h != cand[0]) continue;
+        if ((k % 11) == 9 && w != 1) continue;    // each balanced height with its own build
+        if ((k % 11) == 10 && w != 2) continue;
         c->force_band = h;
         c->force_w = w;
+        c->force_wpb = wpb;
         rc = claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev,
                                     CLAW_STEP_ACCUMULATE_CFL, stream);
         if (rc) break;
@@ -1121,14 +1176,17 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (best == 0 || ms < best_ms) { best = h; best_w = w; best_ms = ms; }
+        if (best == 0 || ms < best_ms) { best = h; best_w = w; best_wpb = wpb; best_ms = ms; }
     }
     c->force_band = 0;
     c->force_w = 0;
+    c->force_wpb = 0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    c->band[tune_key(grid, meth, rp, precision)] = std::make_pair(best, best_w);
+    const TuneKey key = tune_key(grid, meth, rp, precision);
+    c->band[key] = std::make_pair(best, best_w);
+    c->wpb[key] = best_wpb;
     if (band_out) *band_out = best_w == 2 ? -best : best;
     return CLAW_OK;
 }

@@ -24,7 +24,8 @@ struct Ctx {
     void *scratch = nullptr;   // split intermediate state
     size_t scratch_bytes = 0;
     std::map<TuneKey, std::pair<int, int>> band;   // tuned (band height, waves per SIMD)
-    int force_band = 0, force_w = 0;               // set while the tuner times a candidate
+    std::map<TuneKey, int> wpb;                     // tuned wavefronts per block (classic step)
+    int force_band = 0, force_w = 0, force_wpb = 0; // set while a tuner times a candidate
 };
 
 constexpr int SHARP_KEY = 100;

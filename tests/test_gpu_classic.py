@@ -132,11 +132,17 @@ def test_autotuned_band_heights_give_identical_results():
     dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
     sh = torch.cuda.current_stream().cuda_stream
     outs, cfls = [], []
-    for tuned in (False, True, (5, 1), (5, 2), (70, 2)):
+    for tuned in (False, True, (5, 1), (5, 2), (70, 2), (36, 2, 1), (7, 1, 2), (3, 2, 1)):
         ctx = _lib.Context(0)
         cfl = torch.zeros((), dtype=torch.float64, device=dev)
         qb = torch.zeros_like(qa)
-        if isinstance(tuned, tuple):
+        if isinstance(tuned, tuple) and len(tuned) == 3:
+            _lib.check(L.claw2d_classic_set_launch_ex(ctx.handle, grid, m, r, _lib.F64, *tuned), "set_launch_ex")
+            sb, sw, sp = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+            _lib.check(L.claw2d_classic_get_launch(ctx.handle, grid, m, r, _lib.F64, ctypes.byref(sb),
+                                                   ctypes.byref(sw), ctypes.byref(sp)), "get_launch")
+            assert (sb.value, sw.value, sp.value) == tuned
+        elif isinstance(tuned, tuple):
             _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, _lib.F64, tuned[0], tuned[1]), "set_launch")
         elif tuned:
             band = ctypes.c_int(0)
