@@ -46,9 +46,10 @@ def parse():
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--mx', type=int, default=4096)
     ap.add_argument('--my', type=int, default=1024)
-    ap.add_argument('--precision', default='f64fma', choices=['f64', 'f64fma', 'f32'],
-                    help="f64fma (default): FP64 with fused multiply-add, within rounding of the reference "
-                         "arithmetic per step; f64: the reference arithmetic bit for bit; f32")
+    ap.add_argument('--precision', default=None, choices=['f64', 'f64fma', 'f32'],
+                    help="f64fma: FP64 with fused multiply-add and shared reciprocals (default of the euler "
+                         "workload; within rounding of the reference arithmetic per step); f64: the reference "
+                         "arithmetic bit for bit (default of the other workloads); f32")
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-autotune', action='store_true', help="keep the default row-band height")
     ap.add_argument('--launch', default=None,
@@ -92,6 +93,8 @@ def main():
         print("bench.py measures one GPU: multi-GPU runs (domain decomposition with halo "
               "exchange, or sharded batches) are not part of this implementation", file=sys.stderr)
         sys.exit(2)
+    if args.precision is None:
+        args.precision = 'f64fma' if args.workload == 'euler' else 'f64'
     if args.workload != 'euler':
         return OTHER[args.workload](args)
     import torch
@@ -351,7 +354,11 @@ def _grid_bufs(q0, ng, dtype, dev):
 
 def bench_sharpclaw(args):
     """SharpClaw shallow water 1024x1024 periodic, WENO5 + SSP104, FP64: one
-    step = 10 fused stage launches (one C call)."""
+    step = 10 fused stage launches (one C call).  The reference-arithmetic
+    build by default (bit-exact); --precision f64fma times the fused build,
+    with the reference build's speed and the difference between the two
+    reported beside it (on this smooth data WENO5's weights amplify rounding
+    differences: one-ulp input noise moves a step by ~1e-9)."""
     import torch
     from pyclaw_amd import _lib
     from pyclaw_amd.problems import shallow_hump
@@ -380,28 +387,28 @@ def bench_sharpclaw(args):
     tot0 = q0.reshape(3, -1).sum(axis=1)
 
     ctx = _lib.Context(0)
+    if args.precision == 'f32':
+        raise SystemExit("the sharpclaw workload runs FP64 (--precision f64fma or f64)")
+    prec = _lib.F64_FMA if args.precision == 'f64fma' else _lib.F64
     band = None
-    if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
-        # one 36-row band per wavefront slot (69 strips x 29 bands on 2048 slots), one-wavefront
-        # blocks: the fastest shape in repeated 300-step runs (the tuner's short bursts from the
-        # initial data rank 24 rows marginally ahead)
-        args.launch = '36,2,1'
     if args.launch:
-        rows_, waves_ = (int(v) for v in args.launch.split(','))
+        rows_, waves_ = (int(v) for v in args.launch.split(',')[:2])
         for pr_ in {prec, _lib.F64}:
-            _lib.check(L.claw2d_classic_set_launch(ctx.handle, grid, m, r, pr_, rows_, waves_),
-                       'claw2d_classic_set_launch')
+            _lib.check(L.claw2d_sharpclaw_set_launch(ctx.handle, g, r, pr_, rows_, waves_),
+                       'claw2d_sharpclaw_set_launch')
         band = {"rows": rows_, "waves_per_simd": waves_}
     elif not args.no_autotune:
-        hb = ctypes.c_int(0)
-        _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, _lib.F64, q.data_ptr(), qn.data_ptr(), None,
-                                               dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
-                   'claw2d_sharpclaw_autotune')
-        band = {"rows": abs(hb.value), "waves_per_simd": 3 if hb.value < 0 else "default"}
+        for pr_ in ([prec, _lib.F64] if prec != _lib.F64 else [prec]):
+            hb = ctypes.c_int(0)
+            _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, pr_, q.data_ptr(), qn.data_ptr(), None,
+                                                   dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
+                       'claw2d_sharpclaw_autotune')
+            if pr_ == prec:
+                band = {"rows": abs(hb.value), "waves_per_simd": 3 if hb.value < 0 else 1}
 
-    def step(k):
-        rc = L.claw2d_sharpclaw_ssp104_step(ctx.handle, g, r, bc, _lib.F64, pair[k % 2].data_ptr(),
-                                            pair[(k + 1) % 2].data_ptr(), s1a.data_ptr(), s1b.data_ptr(),
+    def step(k, pr_=prec, bufs=pair):
+        rc = L.claw2d_sharpclaw_ssp104_step(ctx.handle, g, r, bc, pr_, bufs[k % 2].data_ptr(),
+                                            bufs[(k + 1) % 2].data_ptr(), s1a.data_ptr(), s1b.data_ptr(),
                                             s2.data_ptr(), None, dtd.data_ptr(), cfl.data_ptr(), sh)
         _lib.check(rc, 'ssp104')
     for k in range(args.warmup):
@@ -418,17 +425,50 @@ def bench_sharpclaw(args):
     tot1 = np.array([np.sum(fin[m], dtype=np.float64) for m in range(3)])
     h_rel = abs(tot1[0] - tot0[0]) / tot0[0]
     mom_rel = np.abs(tot1[1:] - tot0[1:]) / tot0[0]
+    cmp = None
+    if prec == _lib.F64_FMA:
+        # the same run through the reference-arithmetic build (timed the same
+        # way), then one step from the fused build's final state through both
+        sl = (slice(None), slice(ng, ng + my), slice(ng, ng + mx))
+
+        def per_field(a, b):
+            a, b = a[sl], b[sl]
+            scale = [b[0].abs().max(), b[1:3].abs().max(), b[1:3].abs().max()]
+            return [float(((a[c] - b[c]).abs().max() / scale[c]).item()) for c in range(3)]
+        ref = [_grid_bufs(q0, ng, torch.float64, dev), None]
+        ref[1] = torch.zeros_like(ref[0])
+        f0 = torch.cuda.Event(enable_timing=True); f1 = torch.cuda.Event(enable_timing=True)
+        for k in range(args.warmup + args.steps):
+            if k == args.warmup:
+                torch.cuda.synchronize()
+                f0.record()
+            step(k, _lib.F64, ref)
+        f1.record()
+        torch.cuda.synchronize()
+        ref_ms = f0.elapsed_time(f1) / args.steps
+        qf = pair[(args.warmup + args.steps) % 2]
+        after = per_field(qf, ref[(args.warmup + args.steps) % 2])
+        te, tf = [qf.clone(), torch.zeros_like(qf)], [qf.clone(), torch.zeros_like(qf)]
+        step(0, _lib.F64, te)
+        step(0, _lib.F64_FMA, tf)
+        cmp = {"reference_arithmetic_gcell_s": mx * my / (ref_ms * 1e-3) / 1e9,
+               "reference_arithmetic_ms_per_step": ref_ms,
+               "one_step_rel_linf_per_field": per_field(tf[1], te[1]),
+               "after_%d_steps_rel_linf_per_field" % (args.warmup + args.steps): after}
     bytes_step = 552
     achieved = bytes_step * mx * my / (ms * 1e-3) / 1e9
     out = {"metric": "Gcell-updates/s", "value": mx * my / (ms * 1e-3) / 1e9, "unit": "Gcell-updates/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded Gaussian hump)",
            "config": {"workload": "SharpClaw shallow water 1024x1024 periodic, WENO5 comp-wise + SSP104 (10 fused "
-                                  "stage launches per step), CFL 0.5", "global_batch": 1, "parallelism": "single GPU"},
+                                  "stage launches per step), CFL 0.5%s" % (
+                                      ", FP64 with fused multiply-add" if prec == _lib.F64_FMA else ""),
+                      "global_batch": 1, "parallelism": "single GPU"},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": bytes_step},
            "extra": {"mass_rel_change": float(h_rel), "momentum_rel_change_over_mass": mom_rel.tolist(),
-                     "cfl_last_stage": float(cfl[9].item()), "band_rows": band}}
+                     "cfl_last_stage": float(cfl[9].item()), "band_rows": band,
+                     "fma_vs_reference_arithmetic": cmp}}
     print(json.dumps(out), flush=True)
 
 

@@ -50,8 +50,12 @@ enum { CLAW_BC_STORED = 0, CLAW_BC_EXTRAP = 1, CLAW_BC_PERIODIC = 2, CLAW_BC_WAL
 /* Arithmetic of a step.  CLAW_F64: double precision with every multiply and
    add rounded separately, as the reference's CPU build (bit-identical
    results).  CLAW_F32: single precision.  CLAW_F64_FMA: double precision with
-   multiply-adds fused (one rounding); the classic step only; results agree
-   with CLAW_F64 to rounding (tested at 1e-13 relative per step). */
+   multiply-adds fused (one rounding) and one division per shared
+   denominator, for the classic and SharpClaw steps (not the batched runs or
+   the adjoint).  Classic steps agree with CLAW_F64 to 1e-13 relative per
+   step (tested); SharpClaw steps agree to rounding on rough data, while on
+   smooth data WENO5's weights amplify any rounding difference (one-ulp input
+   noise moves the reference arithmetic itself by ~1e-9 per step). */
 enum { CLAW_F64 = 0, CLAW_F32 = 1, CLAW_F64_FMA = 2 };
 
 typedef struct {

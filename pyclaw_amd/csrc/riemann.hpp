@@ -366,16 +366,30 @@ template <typename R> struct ShallowRoe {
         R hbar = (hl + hr) * R(0.5);
         R hsqrtl = rsqrt_exact<R>(hl), hsqrtr = rsqrt_exact<R>(hr);
         R hsq2 = hsqrtl + hsqrtr;
-        R u = (lo[mu] / hsqrtl + hi[mu] / hsqrtr) / hsq2;
-        R v = (lo[mv] / hsqrtl + hi[mv] / hsqrtr) / hsq2;
+        R u, v;
+        if constexpr (SHARED_RCP) {
+            const R il = R(1) / hsqrtl, ir = R(1) / hsqrtr, is = R(1) / hsq2;
+            u = (lo[mu] * il + hi[mu] * ir) * is;
+            v = (lo[mv] * il + hi[mv] * ir) * is;
+        } else {
+            u = (lo[mu] / hsqrtl + hi[mu] / hsqrtr) / hsq2;
+            v = (lo[mv] / hsqrtl + hi[mv] / hsqrtr) / hsq2;
+        }
         R a = rsqrt_exact<R>(grav * hbar);
         C.u = u; C.v = v; C.a = a;
         R d0 = hr - hl;
         R d1 = hi[mu] - lo[mu];
         R d2 = hi[mv] - lo[mv];
-        R a1 = ((u + a) * d0 - d1) / (R(2) * a);
+        R a1, a3;
+        if constexpr (SHARED_RCP) {
+            const R i2a = R(1) / (R(2) * a);
+            a1 = ((u + a) * d0 - d1) * i2a;
+            a3 = (-(u - a) * d0 + d1) * i2a;
+        } else {
+            a1 = ((u + a) * d0 - d1) / (R(2) * a);
+            a3 = (-(u - a) * d0 + d1) / (R(2) * a);
+        }
         R a2 = -v * d0 + d2;
-        R a3 = (-(u - a) * d0 + d1) / (R(2) * a);
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
         wave[1][0] = R(0); wave[1][mu] = R(0); wave[1][mv] = a2; s[1] = u;
         wave[2][0] = a3; wave[2][mu] = a3 * (u + a); wave[2][mv] = a3 * v; s[2] = u + a;

@@ -26,7 +26,19 @@
 #include "ctx.hpp"
 #include "riemann.hpp"
 
+// The kernels and their launcher are compiled twice: here without
+// floating-point contraction (the reference's arithmetic bit for bit), in
+// namespace clawhip::exact, and from sharpclaw2d_fma.hip with contraction and
+// shared reciprocals, in clawhip::fused, for the CLAW_F64_FMA precision.  The
+// C entry points are compiled here only.
+#ifndef CLAW_SHARP_VARIANT
+#define CLAW_SHARP_VARIANT exact
+#define CLAW_SHARP_MAIN 1
+#define CLAW_SHARP_F64_KEY CLAW_F64
+#endif
+
 namespace clawhip {
+namespace CLAW_SHARP_VARIANT {
 
 constexpr int SX_USEFUL = WAVE - 6;
 constexpr int SNG = 3;   // SharpClaw ghost width ((weno_order+1)/2, sharpclaw/solver.py:159)
@@ -47,28 +59,42 @@ enum StageMode {
 template <typename R>
 __device__ __forceinline__ void weno5(R qm2, R qm1, R q0, R qp1, R qp2, R &ql, R &qr) {
     const R eps = R((double)1.0e-36f);
-    R sigma0 = R(3.3333333333333333333333333333333333) * (q0 * q0)
-        - R(10.333333333333333333333333333333333) * q0 * qp1
-        + R(3.6666666666666666666666666666666667) * q0 * qp2
-        + R(8.3333333333333333333333333333333333) * (qp1 * qp1)
-        - R(6.3333333333333333333333333333333333) * qp1 * qp2
-        + R(1.3333333333333333333333333333333333) * (qp2 * qp2);
-    R sigma1 = R(4.3333333333333333333333333333333333) * (q0 * q0)
-        - R(4.3333333333333333333333333333333333) * q0 * qp1
-        - R(4.3333333333333333333333333333333333) * q0 * qm1
-        + R(1.3333333333333333333333333333333333) * (qp1 * qp1)
-        + R(1.6666666666666666666666666666666667) * qp1 * qm1
-        + R(1.3333333333333333333333333333333333) * (qm1 * qm1);
-    R sigma2 = R(3.3333333333333333333333333333333333) * (q0 * q0)
-        - R(10.333333333333333333333333333333333) * q0 * qm1
-        + R(3.6666666666666666666666666666666667) * q0 * qm2
-        + R(8.3333333333333333333333333333333333) * (qm1 * qm1)
-        - R(6.3333333333333333333333333333333333) * qm1 * qm2
-        + R(1.3333333333333333333333333333333333) * (qm2 * qm2);
+    // The smoothness indicators are expanded quadratic forms whose O(q^2)
+    // terms cancel to O(dq^2) on smooth data; they are never contracted, so
+    // every build computes them with the reference's roundings.
+    R sigma0, sigma1, sigma2;
+    {
+#pragma clang fp contract(off)
+        sigma0 = R(3.3333333333333333333333333333333333) * (q0 * q0)
+            - R(10.333333333333333333333333333333333) * q0 * qp1
+            + R(3.6666666666666666666666666666666667) * q0 * qp2
+            + R(8.3333333333333333333333333333333333) * (qp1 * qp1)
+            - R(6.3333333333333333333333333333333333) * qp1 * qp2
+            + R(1.3333333333333333333333333333333333) * (qp2 * qp2);
+        sigma1 = R(4.3333333333333333333333333333333333) * (q0 * q0)
+            - R(4.3333333333333333333333333333333333) * q0 * qp1
+            - R(4.3333333333333333333333333333333333) * q0 * qm1
+            + R(1.3333333333333333333333333333333333) * (qp1 * qp1)
+            + R(1.6666666666666666666666666666666667) * qp1 * qm1
+            + R(1.3333333333333333333333333333333333) * (qm1 * qm1);
+        sigma2 = R(3.3333333333333333333333333333333333) * (q0 * q0)
+            - R(10.333333333333333333333333333333333) * q0 * qm1
+            + R(3.6666666666666666666666666666666667) * q0 * qm2
+            + R(8.3333333333333333333333333333333333) * (qm1 * qm1)
+            - R(6.3333333333333333333333333333333333) * qm1 * qm2
+            + R(1.3333333333333333333333333333333333) * (qm2 * qm2);
+    }
     R t0 = eps + sigma0, t1 = eps + sigma1, t2 = eps + sigma2;
     R t00 = t0 * t0, t11 = t1 * t1, t22 = t2 * t2;
-    R omega0 = R(0.1) / t00, omega1 = R(0.6) / t11, omega2 = R(0.3) / t22;
-    R omega3 = R(0.3) / t00, omega4 = R(0.6) / t11, omega5 = R(0.1) / t22;
+    R omega0, omega1, omega2, omega3, omega4, omega5;
+    if constexpr (SHARED_RCP) {   // one division per smoothness indicator
+        const R i0 = R(1) / t00, i1 = R(1) / t11, i2 = R(1) / t22;
+        omega0 = R(0.1) * i0; omega1 = R(0.6) * i1; omega2 = R(0.3) * i2;
+        omega3 = R(0.3) * i0; omega4 = R(0.6) * i1; omega5 = R(0.1) * i2;
+    } else {
+        omega0 = R(0.1) / t00; omega1 = R(0.6) / t11; omega2 = R(0.3) / t22;
+        omega3 = R(0.3) / t00; omega4 = R(0.6) / t11; omega5 = R(0.1) / t22;
+    }
     R fr0 = R(1.8333333333333333333333333333333333) * q0 - R(1.1666666666666666666666666666666667) * qp1
         + R(0.33333333333333333333333333333333333) * qp2;
     R fr1 = R(0.83333333333333333333333333333333333) * q0 - R(0.16666666666666666666666666666666667) * qp1
@@ -328,7 +354,7 @@ static int launch_sharp(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, c
     p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy; p.mode = mode;
     p.nsx = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
     int w = 1;
-    p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_F64, p.nsx, w);
+    p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_SHARP_F64_KEY, p.nsx, w);
     p.nsy = (grid->my + p.H - 1) / p.H;
     p.wpb = wpb;
     p.nsyb = (p.nsy + wpb - 1) / wpb;
@@ -349,7 +375,7 @@ static int sharp_validate(const ClawGrid *grid, const ClawRiemann *rp, const Cla
         if (bc->lower[s] < 0 || bc->lower[s] > 3 || bc->upper[s] < 0 || bc->upper[s] > 3) return CLAW_EINVAL;
         if ((bc->lower[s] == CLAW_BC_PERIODIC) != (bc->upper[s] == CLAW_BC_PERIODIC)) return CLAW_EINVAL;
     }
-    if (precision != CLAW_F64 && precision != CLAW_F32) return CLAW_EINVAL;
+    if (precision != CLAW_F64 && precision != CLAW_F32 && precision != CLAW_F64_FMA) return CLAW_EINVAL;
     if (rp->kind == CLAW_RP_VC_ACOUSTICS && !aux) return CLAW_EINVAL;
     if (rp->kind < 0 || rp->kind > 3) return CLAW_EINVAL;
     return CLAW_OK;
@@ -371,9 +397,37 @@ static int sharp_dispatch(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp,
 #undef SDISPATCH
 }
 
+#ifndef CLAW_SHARP_MAIN
+// FP64 launches of this build (arguments validated by the C entry points).
+int sharp_dispatch_f64(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int mode,
+                       const void *qin, const void *q0, void *s2, void *out, const void *aux, const double *dt,
+                       double *cfl, bool zero_cfl, hipStream_t st) {
+    return sharp_dispatch(ctx, grid, rp, bc, CLAW_F64, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st);
+}
+#endif
+}  // namespace CLAW_SHARP_VARIANT
+}  // namespace clawhip
+
+#ifdef CLAW_SHARP_MAIN
+namespace clawhip {
+namespace fused {
+int sharp_dispatch_f64(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int mode,
+                       const void *qin, const void *q0, void *s2, void *out, const void *aux, const double *dt,
+                       double *cfl, bool zero_cfl, hipStream_t st);
+}  // namespace fused
 }  // namespace clawhip
 
 using namespace clawhip;
+using namespace clawhip::exact;
+
+// the build for the precision: CLAW_F64_FMA runs the fused kernels
+static int sharp_route(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int precision,
+                       int mode, const void *qin, const void *q0, void *s2, void *out, const void *aux,
+                       const double *dt, double *cfl, bool zero_cfl, hipStream_t st) {
+    if (precision == CLAW_F64_FMA)
+        return fused::sharp_dispatch_f64(ctx, grid, rp, bc, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st);
+    return sharp_dispatch(ctx, grid, rp, bc, precision, mode, qin, q0, s2, out, aux, dt, cfl, zero_cfl, st);
+}
 
 extern "C" {
 
@@ -383,7 +437,7 @@ int claw2d_sharpclaw_dq(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, 
     int rc = sharp_validate(grid, rp, bc, precision, aux);
     if (rc) return rc;
     if (!q || !dq || q == dq || !dt_dev || !cfl_dev) return CLAW_EINVAL;
-    return sharp_dispatch((Ctx *)ctx, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, true,
+    return sharp_route((Ctx *)ctx, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, true,
                           (hipStream_t)stream);
 }
 
@@ -397,7 +451,7 @@ int claw2d_sharpclaw_stage(void *ctx, const ClawGrid *grid, const ClawRiemann *r
     if ((mode == SM_SSP104_MID || mode == SM_SSP33_2 || mode == SM_SSP33_3) && !q0) return CLAW_EINVAL;
     if ((mode == SM_SSP104_MID || mode == SM_SSP104_END) && !s2) return CLAW_EINVAL;
     if (mode == SM_SSP104_MID && (s2 == qin || s2 == out)) return CLAW_EINVAL;
-    return sharp_dispatch((Ctx *)ctx, grid, rp, bc, precision, mode, qin, q0, s2, out, aux, dt_dev, cfl_dev, true,
+    return sharp_route((Ctx *)ctx, grid, rp, bc, precision, mode, qin, q0, s2, out, aux, dt_dev, cfl_dev, true,
                           (hipStream_t)stream);
 }
 
@@ -417,7 +471,7 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
     int k = 0, cur = 0;
 #define STAGE(MODE, IN, OUT)                                                                                   \
     do {                                                                                                       \
-        rc = sharp_dispatch((Ctx *)ctx, grid, rp, bc, precision, MODE, IN, q, s2, OUT, aux, dt_dev, cfl10_dev + k, true, st); \
+        rc = sharp_route((Ctx *)ctx, grid, rp, bc, precision, MODE, IN, q, s2, OUT, aux, dt_dev, cfl10_dev + k, true, st); \
         if (rc) return rc;                                                                                     \
         ++k;                                                                                                   \
     } while (0)
@@ -459,12 +513,12 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
         if (h > grid->my && h != cand[0]) continue;
         c->force_band = h;
         c->force_w = w;
-        rc = sharp_dispatch(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, false, st);
+        rc = sharp_route(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, false, st);
         if (rc) break;
         const int reps = 5;
         (void)hipEventRecord(e0, st);
         for (int r = 0; r < reps && !rc; ++r)
-            rc = sharp_dispatch(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev,
+            rc = sharp_route(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev,
                                 false, st);
         (void)hipEventRecord(e1, st);
         if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
@@ -483,4 +537,4 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
 }
 
 }  // extern "C"
-
+#endif  // CLAW_SHARP_MAIN

@@ -64,6 +64,10 @@ class SharpClawSolver(Solver):
         self.cfl_max = 2.5
         self.dq_src = None
         self.call_before_step_each_stage = False
+        # FP64 states only: use the kernels built with fused multiply-add and
+        # shared reciprocals (CLAW_F64_FMA); each stage agrees with the
+        # reference arithmetic to rounding (not bit for bit).
+        self.fma = False
         self._mthlim = self.limiters
         self._method = None
         self._rk_stages = None
@@ -162,11 +166,16 @@ class SharpClawSolver(Solver):
             ok = self._step_staged(state)
         return ok
 
+    def _precision(self, state):
+        if state.dtype == torch.float32:
+            return _lib.F32
+        return _lib.F64_FMA if self.fma else _lib.F64
+
     def _tune(self, state):
         """Time the stage kernel's launch shapes once per grid shape
         (claw2d_sharpclaw_autotune); the choice never changes results."""
         grid = self._grid_struct(state)
-        prec = _lib.F32 if state.dtype == torch.float32 else _lib.F64
+        prec = self._precision(state)
         key = (grid.mx, grid.my, prec, self._rp_struct.kind)
         if key in self._tuned:
             return
@@ -183,7 +192,7 @@ class SharpClawSolver(Solver):
 
     def _launch_stage(self, state, mode, qin, q0, out, cfl_ptr, bc):
         grid = self._grid_struct(state)
-        prec = _lib.F32 if state.dtype == torch.float32 else _lib.F64
+        prec = self._precision(state)
         aux = state._aux.buf.data_ptr() if state._aux is not None else None
         rc = _lib.lib().claw2d_sharpclaw_stage(self._ctx.handle, grid, self._rp_struct, bc, prec, mode,
                                                qin.data_ptr(), q0.data_ptr(), self._regs['s2'].data_ptr(),
@@ -207,7 +216,7 @@ class SharpClawSolver(Solver):
         sched = _SCHEDULES[self.time_integrator]
         if self.time_integrator == 'SSP104':
             grid = self._grid_struct(state)
-            prec = _lib.F32 if state.dtype == torch.float32 else _lib.F64
+            prec = self._precision(state)
             aux = state._aux.buf.data_ptr() if state._aux is not None else None
             rc = _lib.lib().claw2d_sharpclaw_ssp104_step(
                 self._ctx.handle, grid, self._rp_struct, bc, prec, q.data_ptr(), qnew.data_ptr(),

@@ -169,7 +169,7 @@ def from_device_interior(t, mx, my):
 
 
 def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), abc_upper=(1, 1),
-                 dx=0.1, dy=0.07):
+                 dx=0.1, dy=0.07, precision='f64'):
     """claw2d_sharpclaw_dq on reference-layout padded arrays with 3 ghost layers."""
     import torch
     from pyclaw_amd import _lib
@@ -181,7 +181,8 @@ def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), a
     dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
     cfl = torch.zeros((), dtype=torch.float64, device='cuda')
     rc = _lib.lib().claw2d_sharpclaw_dq(None, _dev_grid(mx, my, dx, dy), _rp_struct(case),
-                                        _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper), _lib.F64,
+                                        _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper),
+                                        _lib.F64_FMA if precision == 'f64fma' else _lib.F64,
                                         q.data_ptr(), dq.data_ptr(), aux.data_ptr() if aux is not None else None,
                                         dtd.data_ptr(), cfl.data_ptr(), torch.cuda.current_stream().cuda_stream)
     _lib.check(rc, 'claw2d_sharpclaw_dq')
@@ -190,7 +191,7 @@ def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), a
 
 
 def gpu_sharp_ssp104(q0, aux, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), abc_upper=(1, 1),
-                     dx=0.1, dy=0.07):
+                     dx=0.1, dy=0.07, precision='f64'):
     """One fused SSP104 step; q0/aux interior arrays (BCs synthesised in the kernel)."""
     import torch
     from pyclaw_amd import _lib
@@ -207,7 +208,8 @@ def gpu_sharp_ssp104(q0, aux, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), ab
     dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
     cfl = torch.zeros(10, dtype=torch.float64, device='cuda')
     rc = _lib.lib().claw2d_sharpclaw_ssp104_step(None, _dev_grid(mx, my, dx, dy), _rp_struct(case),
-                                                 _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper), _lib.F64,
+                                                 _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper),
+                                                 _lib.F64_FMA if precision == 'f64fma' else _lib.F64,
                                                  q.data_ptr(), qn.data_ptr(), s1a.data_ptr(), s1b.data_ptr(),
                                                  s2.data_ptr(), auxd.data_ptr() if auxd is not None else None,
                                                  dtd.data_ptr(), cfl.data_ptr(),
