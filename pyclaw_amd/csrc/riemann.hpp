@@ -43,6 +43,18 @@ constexpr bool SHARED_RCP = true;
 constexpr bool SHARED_RCP = false;
 #endif
 
+// 1/x in the shared-reciprocal build: the hardware reciprocal refined by two
+// Newton steps (within about one ulp of 1/x for normal x; used for
+// positive Roe-solver quantities only).  Other builds never call it.
+template <typename R> __device__ __forceinline__ R rcp(R x) { return R(1) / x; }
+template <> __device__ __forceinline__ double rcp<double>(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 template <typename R> __device__ __forceinline__ R rsqrt_exact(R x);
 template <> __device__ __forceinline__ double rsqrt_exact<double>(double x) { return __builtin_sqrt(x); }
 template <> __device__ __forceinline__ float rsqrt_exact<float>(float x) { return __builtin_sqrtf(x); }
@@ -169,12 +181,12 @@ template <typename R> struct Euler5 {
         R pl, pr, u, v, enth;
         R il0 = R(0), ih0 = R(0);   // 1/rho of each side (shared-reciprocal build)
         if constexpr (SHARED_RCP) {
-            il0 = R(1) / lo[0];
-            ih0 = R(1) / hi[0];
-            const R irl = R(1) / rhsqrtl, irr = R(1) / rhsqrtr;
+            il0 = rcp<R>(lo[0]);
+            ih0 = rcp<R>(hi[0]);
+            const R irl = rcp<R>(rhsqrtl), irr = rcp<R>(rhsqrtr);
             pl = gamma1 * (lo[3] - R(0.5) * (lo[1] * lo[1] + lo[2] * lo[2]) * il0);
             pr = gamma1 * (hi[3] - R(0.5) * (hi[1] * hi[1] + hi[2] * hi[2]) * ih0);
-            const R irs = R(1) / (rhsqrtl + rhsqrtr);
+            const R irs = rcp<R>(rhsqrtl + rhsqrtr);
             u = (lo[mu] * irl + hi[mu] * irr) * irs;
             v = (lo[mv] * irl + hi[mv] * irr) * irs;
             enth = ((lo[3] + pl) * irl + (hi[3] + pr) * irr) * irs;
@@ -189,7 +201,7 @@ template <typename R> struct Euler5 {
         R u2v2 = u * u + v * v;
         R a2 = gamma1 * (enth - R(0.5) * u2v2);
         R a = rsqrt_exact<R>(a2);
-        R g1a2 = gamma1 / a2;
+        R g1a2 = SHARED_RCP ? gamma1 * rcp<R>(a2) : gamma1 / a2;
         R euv = enth - u2v2;
         C.u = u; C.v = v; C.enth = enth; C.a = a;
 
@@ -199,7 +211,7 @@ template <typename R> struct Euler5 {
         R d3 = hi[3] - lo[3];
         R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
         R a2w = d2 - v * d0;
-        R a4 = (d1 + (a - u) * d0 - a * a3) / (R(2) * a);
+        R a4 = SHARED_RCP ? (d1 + (a - u) * d0 - a * a3) * rcp<R>(R(2) * a) : (d1 + (a - u) * d0 - a * a3) / (R(2) * a);
         R a1 = d0 - a3 - a4;
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v;
         wave[0][3] = a1 * (enth - u * a); wave[0][4] = R(0); s[0] = u - a;
@@ -250,7 +262,7 @@ template <typename R> struct Euler5 {
             R en1 = lo[3] + wave[0][3];
             R s1;
             if constexpr (SHARED_RCP) {
-                const R i1 = R(1) / rho1;
+                const R i1 = rcp<R>(rho1);
                 R p1 = gamma1 * (en1 - R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) * i1);
                 s1 = rhou1 * i1 - rsqrt_exact<R>(gamma * p1 * i1);
             } else {
@@ -277,7 +289,7 @@ template <typename R> struct Euler5 {
             R en2 = hi[3] - wave[3][3];
             if constexpr (SHARED_RCP) {
                 s3 = hi[mu] * ih0 + rsqrt_exact<R>(gamma * pr * ih0);
-                const R i2 = R(1) / rho2;
+                const R i2 = rcp<R>(rho2);
                 R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) * i2);
                 s2 = rhou2 * i2 + rsqrt_exact<R>(gamma * p2 * i2);
             } else {
@@ -310,13 +322,14 @@ template <typename R> struct Euler5 {
         const R u = C.u, v = C.v, a = C.a, enth = C.enth;
         // the same expressions as in rpn (rpn2_euler_5wave.f:100-104), so the values are identical
         const R u2v2 = u * u + v * v;
-        const R g1a2 = P.gamma1 / (P.gamma1 * (enth - R(0.5) * u2v2));
+        const R g1a2 = SHARED_RCP ? P.gamma1 * rcp<R>(P.gamma1 * (enth - R(0.5) * u2v2))
+                                  : P.gamma1 / (P.gamma1 * (enth - R(0.5) * u2v2));
         const R euv = enth - u2v2;
         R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
         R a2 = asdq[mu] - u * asdq[0];
         R a4;
         if constexpr (SHARED_RCP)   // 1/(2a) is shared by the interface's two transverse solves
-            a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) * (R(1) / (R(2) * a));
+            a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) * rcp<R>(R(2) * a);
         else
             a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
         R a1 = asdq[0] - a3 - a4;
@@ -368,7 +381,7 @@ template <typename R> struct ShallowRoe {
         R hsq2 = hsqrtl + hsqrtr;
         R u, v;
         if constexpr (SHARED_RCP) {
-            const R il = R(1) / hsqrtl, ir = R(1) / hsqrtr, is = R(1) / hsq2;
+            const R il = rcp<R>(hsqrtl), ir = rcp<R>(hsqrtr), is = rcp<R>(hsq2);
             u = (lo[mu] * il + hi[mu] * ir) * is;
             v = (lo[mv] * il + hi[mv] * ir) * is;
         } else {
@@ -382,7 +395,7 @@ template <typename R> struct ShallowRoe {
         R d2 = hi[mv] - lo[mv];
         R a1, a3;
         if constexpr (SHARED_RCP) {
-            const R i2a = R(1) / (R(2) * a);
+            const R i2a = rcp<R>(R(2) * a);
             a1 = ((u + a) * d0 - d1) * i2a;
             a3 = (-(u - a) * d0 + d1) * i2a;
         } else {
