@@ -310,6 +310,9 @@ def bench_batched(args):
     rp = pc.riemann.rp2_vc_acoustics
 
     b = BatchedClawSolver2D(rp, {}, inst)
+    shape = None
+    if not args.no_autotune:
+        shape = b.autotune()               # tile rows x wavefronts per block (results do not depend on it)
     b.run(nsteps, use_graph=True)          # graph capture + first replay (warm-up)
     torch.cuda.synchronize()
     for k in range(b.n):                   # back to the initial data, then the timed replay
@@ -340,7 +343,8 @@ def bench_batched(args):
            "extra": {"batched_s": t_batch, "one_by_one_s": t_loop, "cells_per_step": cells,
                      "batched_gcell_updates_per_s": cells * nsteps / t_batch / 1e9,
                      "bitwise_equal_first8": same, "all_accepted": bool((acc == nsteps).all()),
-                     "work_items_per_step": b3.work_items}}
+                     "work_items_per_step": b3.work_items,
+                     "tile_shape": None if shape is None else {"rows": shape[0], "waves_per_block": shape[1]}}}
     print(json.dumps(out), flush=True)
 
 

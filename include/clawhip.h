@@ -227,12 +227,28 @@ typedef struct {
 } ClawBatchInstance;
 
 /* Builds the work list (instances sorted largest first, one item per
-   60-column strip x 32-row band) and device descriptors. */
+   60-column strip x 32-row band; blocks of four wavefronts) and device
+   descriptors. */
 int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, int ng,
                              const ClawMethod *method, const ClawRiemann *rp, const ClawBC *bc,
                              int precision, void **plan);
 int claw2d_batch_plan_destroy(void *plan);
 int claw2d_batch_work_items(void *plan);
+
+/* Tile shape of the plan's work items: rows per band (>= 1) and wavefronts
+   per block of the persistent launch (1, 2 or 4).  Rebuilds the work list
+   (waits for the device first) and drops a captured step graph.  Results
+   do not depend on the shape.  No counterpart in the reference. */
+int claw2d_batch_set_shape(void *plan, int rows, int waves_per_block);
+
+/* Autotuner of the batched step: times one step for band heights
+   {8,16,32,64,128} x {1,2,4} wavefronts per block and keeps the fastest.
+   The timed steps write the instances' other buffers (parity is not
+   flipped), so the state, dt, t and the counters are unchanged; blocks
+   until done.  rows_out / waves_per_block_out (optional) receive the
+   choice.  No counterpart in the reference. */
+int claw2d_batch_autotune(void *plan, void *q_pool, const void *aux_pool, const double *dt,
+                          const int *parity, void *stream, int *rows_out, int *waves_per_block_out);
 
 /* nsteps unsplit steps of every instance.  Per-instance device arrays (length
    n): dt (in/out), t (in/out), cflmax (in/out), parity (0: state in buffer 0,

@@ -35,6 +35,9 @@ def _bind():
                                      vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.claw2d_step_update.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      vp, vp, vp, vp, vp, vp, vp, vp]
+    L.claw2d_batch_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.claw2d_batch_autotune.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]
     return L
 
 
@@ -127,6 +130,22 @@ class BatchedClawSolver2D(object):
             a = self.auxpool[aoff:aoff + na * plane].view(na, my + 2 * ng, ldq)
             at = torch.from_numpy(np.ascontiguousarray(np.asarray(aux, dtype=np.float64).transpose(0, 2, 1)))
             a[:, ng:ng + my, ng:ng + mx] = at.to(self.dev, self.auxpool.dtype)
+
+    def set_shape(self, rows, waves_per_block):
+        """Tile shape of the persistent launch: rows per band, wavefronts per block."""
+        _lib.check(self.L.claw2d_batch_set_shape(self._plan, int(rows), int(waves_per_block)),
+                   "claw2d_batch_set_shape")
+
+    def autotune(self, stream=None):
+        """Time the tile shapes on the current state (left unchanged) and keep
+        the fastest; returns (rows, waves_per_block)."""
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rows, wpb = ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(self.L.claw2d_batch_autotune(self._plan, self.qpool.data_ptr(),
+                                                self.auxpool.data_ptr() if self.auxpool is not None else None,
+                                                self.dt.data_ptr(), self.parity.data_ptr(), st.cuda_stream,
+                                                ctypes.byref(rows), ctypes.byref(wpb)), "claw2d_batch_autotune")
+        return rows.value, wpb.value
 
     def run(self, nsteps, use_graph=True, stream=None):
         """Advance every instance nsteps steps (device-side dt and accept/reject)."""

@@ -725,7 +725,8 @@ struct BatchStepParams {
 template <class RP, typename R>
 __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<RP, R> b) {
     const int lane = threadIdx.x & (WAVE - 1);
-    __shared__ R carry[4 * CarrySlots<RP>::COUNT * WAVE];
+    // blockDim.x / WAVE wavefronts, each with its carry slots in dynamic LDS
+    R *carry = reinterpret_cast<R *>(classic_carry_lds);
     for (;;) {
         // one ticket per wavefront: every lane takes part in the atomic (lane 0
         // adds 1, the others 0) so the loop head holds no lane-dependent branch;
@@ -932,6 +933,9 @@ int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const
 struct BatchPlan {
     Ctx *ctx = nullptr;
     int n = 0, ng = 0, nitems = 0, precision = 0, blocks = 0;
+    int H = 32, wpb = 4;                 // tile rows (band height) and wavefronts per block
+    int items_cap = 0;                   // allocated length of the work-item arrays
+    std::vector<BatchInstDev> hinst;     // instance descriptors (host copy)
     ClawMethod meth;
     ClawRiemann rp;
     ClawBC bc;
@@ -959,18 +963,20 @@ static int batch_step_launch(BatchPlan *pl, void *qpool, const void *auxpool, co
     b.P = make_params<RP, R>(&pl->rp);
     b.order = pl->meth.order; b.trans = pl->meth.transverse; b.ng = pl->ng;
     for (int k = 0; k < 8; ++k) b.mthlim[k] = pl->meth.mthlim[k];
+    const size_t lds = (size_t)pl->wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
     if (pl->blocks == 0) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<RP, R>, 256, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<RP, R>, WAVE * pl->wpb, lds) !=
+            hipSuccess)
             per_cu = 1;
         int dev = 0;
         (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
         pl->blocks = std::max(1, per_cu) * std::max(1, cus);
         // no more waves than work items
-        pl->blocks = std::min(pl->blocks, (pl->nitems + 3) / 4);
+        pl->blocks = std::min(pl->blocks, (pl->nitems + pl->wpb - 1) / pl->wpb);
     }
-    hipLaunchKernelGGL((batch_step_kernel<RP, R>), dim3(pl->blocks), dim3(256), 0, st, b);
+    hipLaunchKernelGGL((batch_step_kernel<RP, R>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 
@@ -1001,6 +1007,46 @@ static int batch_sequence(BatchPlan *pl, void *qpool, const void *auxpool, int n
                            pl->d_cfl, t, cflmax, parity, nacc, nrej, pl->d_counter);
     }
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+// Work items of the plan's instances for band height H (largest instances
+// first, longest-processing-time order); invalidates a captured step graph.
+static int batch_build_items(BatchPlan *pl, int H) {
+    // no step of this plan may still be reading the arrays being replaced
+    if (hipDeviceSynchronize() != hipSuccess) return CLAW_EDEVICE;
+    std::vector<std::pair<long long, int>> order;
+    for (int k = 0; k < pl->n; ++k) {
+        BatchInstDev &d = pl->hinst[k];
+        d.H = std::min(d.my, H);
+        d.nsy = (d.my + d.H - 1) / d.H;
+        order.push_back({-(long long)d.mx * d.my, k});
+    }
+    std::stable_sort(order.begin(), order.end());
+    std::vector<int> item_inst, item_local;
+    for (auto &o : order) {
+        const BatchInstDev &d = pl->hinst[o.second];
+        for (int l = 0; l < d.nsx * d.nsy; ++l) { item_inst.push_back(o.second); item_local.push_back(l); }
+    }
+    const int nitems = (int)item_inst.size();
+    if (nitems > pl->items_cap) {
+        (void)hipFree(pl->d_item_inst); (void)hipFree(pl->d_item_local);
+        pl->d_item_inst = pl->d_item_local = nullptr;
+        pl->items_cap = 0;
+        if (hipMalloc(&pl->d_item_inst, sizeof(int) * nitems) != hipSuccess ||
+            hipMalloc(&pl->d_item_local, sizeof(int) * nitems) != hipSuccess)
+            return CLAW_ENOMEM;
+        pl->items_cap = nitems;
+    }
+    if (hipMemcpy(pl->d_inst, pl->hinst.data(), sizeof(BatchInstDev) * pl->n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->d_item_inst, item_inst.data(), sizeof(int) * nitems, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->d_item_local, item_local.data(), sizeof(int) * nitems, hipMemcpyHostToDevice) != hipSuccess)
+        return CLAW_EDEVICE;
+    pl->nitems = nitems;
+    pl->H = H;
+    pl->blocks = 0;
+    if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); pl->exec = nullptr; }
+    if (pl->graph) { (void)hipGraphDestroy(pl->graph); pl->graph = nullptr; }
+    return CLAW_OK;
 }
 
 static void batch_free(BatchPlan *pl) {
@@ -1201,7 +1247,6 @@ int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, in
     for (int s = 0; s < 2; ++s)
         if ((bc->lower[s] == CLAW_BC_PERIODIC) != (bc->upper[s] == CLAW_BC_PERIODIC)) return CLAW_EINVAL;
     std::vector<BatchInstDev> hi(n);
-    std::vector<std::pair<long long, int>> order;
     for (int k = 0; k < n; ++k) {
         const ClawBatchInstance &I = inst[k];
         if (I.mx < 2 || I.my < 2 || I.ldq < I.mx + 2 * ng || I.plane < I.ldq * (I.my + 2 * ng)) return CLAW_EINVAL;
@@ -1209,35 +1254,24 @@ int claw2d_batch_plan_create(void *ctx, const ClawBatchInstance *inst, int n, in
         BatchInstDev d;
         d.mx = I.mx; d.my = I.my;
         d.nsx = (I.mx + XU_USEFUL - 1) / XU_USEFUL;
-        d.H = std::min(I.my, 32);
-        d.nsy = (I.my + d.H - 1) / d.H;
+        d.H = 0; d.nsy = 0;
         d.pad_ = 0;
         d.q0 = I.q_offset[0]; d.q1 = I.q_offset[1]; d.aux = I.aux_offset < 0 ? 0 : I.aux_offset;
         d.ldq = I.ldq; d.plane = I.plane; d.dx = I.dx; d.dy = I.dy;
         hi[k] = d;
-        order.push_back({-(long long)I.mx * I.my, k});
-    }
-    // work items of the largest grids first (longest-processing-time order)
-    std::stable_sort(order.begin(), order.end());
-    std::vector<int> item_inst, item_local;
-    for (auto &o : order) {
-        const BatchInstDev &d = hi[o.second];
-        for (int l = 0; l < d.nsx * d.nsy; ++l) { item_inst.push_back(o.second); item_local.push_back(l); }
     }
     BatchPlan *pl = new (std::nothrow) BatchPlan();
     if (!pl) return CLAW_ENOMEM;
-    pl->ctx = (Ctx *)ctx; pl->n = n; pl->ng = ng; pl->nitems = (int)item_inst.size(); pl->precision = precision;
+    pl->ctx = (Ctx *)ctx; pl->n = n; pl->ng = ng; pl->precision = precision;
     pl->meth = *meth; pl->rp = *rp; pl->bc = *bc;
+    pl->hinst = hi;
     bool ok = hipMalloc(&pl->d_inst, sizeof(BatchInstDev) * n) == hipSuccess &&
-              hipMalloc(&pl->d_item_inst, sizeof(int) * pl->nitems) == hipSuccess &&
-              hipMalloc(&pl->d_item_local, sizeof(int) * pl->nitems) == hipSuccess &&
               hipMalloc(&pl->d_counter, sizeof(int)) == hipSuccess &&
               hipMalloc(&pl->d_cfl, sizeof(double) * n) == hipSuccess;
     if (!ok) { batch_free(pl); return CLAW_ENOMEM; }
-    ok = hipMemcpy(pl->d_inst, hi.data(), sizeof(BatchInstDev) * n, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(pl->d_item_inst, item_inst.data(), sizeof(int) * pl->nitems, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(pl->d_item_local, item_local.data(), sizeof(int) * pl->nitems, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemset(pl->d_counter, 0, sizeof(int)) == hipSuccess &&
+    int rc = batch_build_items(pl, 32);
+    if (rc) { batch_free(pl); return rc; }
+    ok = hipMemset(pl->d_counter, 0, sizeof(int)) == hipSuccess &&
          hipMemset(pl->d_cfl, 0, sizeof(double) * n) == hipSuccess &&
          hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking) == hipSuccess;
     if (!ok) { batch_free(pl); return CLAW_EDEVICE; }
@@ -1252,6 +1286,60 @@ int claw2d_batch_plan_destroy(void *plan) {
 }
 
 int claw2d_batch_work_items(void *plan) { return plan ? ((BatchPlan *)plan)->nitems : -1; }
+
+int claw2d_batch_set_shape(void *plan, int rows, int waves_per_block) {
+    BatchPlan *pl = (BatchPlan *)plan;
+    if (!pl || rows < 1 || (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)) return CLAW_EINVAL;
+    pl->wpb = waves_per_block;
+    return batch_build_items(pl, rows);
+}
+
+int claw2d_batch_autotune(void *plan, void *q_pool, const void *aux_pool, const double *dt, const int *parity,
+                          void *stream, int *rows_out, int *waves_per_block_out) {
+    BatchPlan *pl = (BatchPlan *)plan;
+    if (!pl || !q_pool || !dt || !parity) return CLAW_EINVAL;
+    if (pl->rp.kind == CLAW_RP_VC_ACOUSTICS && !aux_pool) return CLAW_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return CLAW_EDEVICE;
+    }
+    const int tb = 256, nb = (pl->n + tb - 1) / tb;
+    const int rows[5] = {8, 16, 32, 64, 128}, wpbs[3] = {1, 2, 4};
+    int best_h = pl->H, best_w = pl->wpb, rc = CLAW_OK;
+    float best_ms = 0.f;
+    bool have = false;
+    // each timed launch is a real step into the buffers the state is not in
+    // (parity is not flipped), after re-arming the queue and the Courant slots
+    for (int a = 0; a < 5 && !rc; ++a)
+        for (int b = 0; b < 3 && !rc; ++b) {
+            pl->wpb = wpbs[b];
+            rc = batch_build_items(pl, rows[a]);
+            if (rc) break;
+            float ms = 0.f;
+            for (int r = 0; r < 4 && !rc; ++r) {
+                if (r == 1) (void)hipEventRecord(e0, st);
+                hipLaunchKernelGGL(batch_arm_kernel, dim3(nb), dim3(tb), 0, st, pl->n, pl->d_cfl, pl->d_counter);
+                rc = batch_step_dispatch(pl, q_pool, aux_pool, dt, parity, st);
+            }
+            (void)hipEventRecord(e1, st);
+            if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (!have || ms < best_ms) { best_ms = ms; best_h = rows[a]; best_w = wpbs[b]; have = true; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    pl->wpb = best_w;
+    rc = batch_build_items(pl, best_h);
+    if (rc) return rc;
+    hipLaunchKernelGGL(batch_arm_kernel, dim3(nb), dim3(tb), 0, st, pl->n, pl->d_cfl, pl->d_counter);
+    if (hipStreamSynchronize(st) != hipSuccess) return CLAW_EDEVICE;
+    if (rows_out) *rows_out = best_h;
+    if (waves_per_block_out) *waves_per_block_out = best_w;
+    return CLAW_OK;
+}
 
 int claw2d_batch_steps(void *plan, void *q_pool, const void *aux_pool, int nsteps, double cfl_max,
                        double cfl_desired, double dt_max, double *dt, double *t, double *cflmax, int *parity,

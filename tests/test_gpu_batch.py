@@ -72,3 +72,29 @@ def test_batch_reject_then_accept_matches_reference_rule():
         assert float(b.t[k]) == t and float(b.dt[k]) == dt
         assert np.array_equal(b.q(k), q)
     assert int(b.nrejected[1]) == 1
+
+
+@pytest.mark.parametrize("shape", ["tuned", (8, 1), (128, 2), (5, 4)])
+def test_batch_shapes_equal_one_by_one(shape):
+    """The tile shape (rows per band, wavefronts per block), set or picked by
+    claw2d_batch_autotune, changes only scheduling: the batch still equals the
+    one-by-one runs bit for bit, and tuning leaves the state untouched."""
+    import pyclaw_amd as pc
+    from pyclaw_amd.batch import BatchedClawSolver2D, run_one_by_one
+    print("batch shape", shape, flush=True)
+    inst = _instances(9, 3)
+    b = BatchedClawSolver2D(pc.riemann.rp2_vc_acoustics, {}, inst)
+    b.run(3, use_graph=True)
+    q3 = [b.q(k) for k in range(len(inst))]
+    if shape == "tuned":
+        rows, wpb = b.autotune()
+        assert rows in (8, 16, 32, 64, 128) and wpb in (1, 2, 4)
+        for k in range(len(inst)):
+            assert np.array_equal(b.q(k), q3[k]), k
+    else:
+        b.set_shape(*shape)
+    b.run(9, use_graph=True)
+    ref = run_one_by_one(pc.riemann.rp2_vc_acoustics, {}, inst, 12)
+    assert (b.naccepted.cpu().numpy() == 12).all()
+    for k in range(len(inst)):
+        assert np.array_equal(b.q(k), ref[k]), k
