@@ -395,20 +395,28 @@ def bench_sharpclaw(args):
         raise SystemExit("the sharpclaw workload runs FP64 (--precision f64fma or f64)")
     prec = _lib.F64_FMA if args.precision == 'f64fma' else _lib.F64
     band = None
+    if args.launch is None and not args.autotune:
+        # 7-row bands (18 strips x 147 bands, one round on 3072 slots), three waves per SIMD: the
+        # fastest shape in 200-step runs (the tuner ranks shapes on the initial data, where the
+        # per-step cost differs)
+        args.launch = '7,3,4'
     if args.launch:
-        rows_, waves_ = (int(v) for v in args.launch.split(',')[:2])
+        shape = [int(v) for v in args.launch.split(',')]
         for pr_ in {prec, _lib.F64}:
-            _lib.check(L.claw2d_sharpclaw_set_launch(ctx.handle, g, r, pr_, rows_, waves_),
-                       'claw2d_sharpclaw_set_launch')
-        band = {"rows": rows_, "waves_per_simd": waves_}
+            _lib.check(L.claw2d_sharpclaw_set_launch_ex(ctx.handle, g, r, pr_, shape[0], shape[1],
+                                                        shape[2] if len(shape) > 2 else 4),
+                       'claw2d_sharpclaw_set_launch_ex')
     elif not args.no_autotune:
         for pr_ in ([prec, _lib.F64] if prec != _lib.F64 else [prec]):
             hb = ctypes.c_int(0)
             _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, pr_, q.data_ptr(), qn.data_ptr(), None,
                                                    dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),
                        'claw2d_sharpclaw_autotune')
-            if pr_ == prec:
-                band = {"rows": abs(hb.value), "waves_per_simd": 3 if hb.value < 0 else 1}
+
+    sb, sw, sp = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _lib.check(L.claw2d_sharpclaw_get_launch(ctx.handle, g, r, prec, ctypes.byref(sb), ctypes.byref(sw),
+                                             ctypes.byref(sp)), 'claw2d_sharpclaw_get_launch')
+    band = {"rows": sb.value, "waves_per_simd": sw.value, "waves_per_block": sp.value}
 
     def step(k, pr_=prec, bufs=pair):
         rc = L.claw2d_sharpclaw_ssp104_step(ctx.handle, g, r, bc, pr_, bufs[k % 2].data_ptr(),

@@ -212,6 +212,16 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
 int claw2d_sharpclaw_set_launch(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision,
                                 int band, int waves_per_simd);
 
+/* Same, also setting the wavefronts per block (1, 2 or 4; the shape set by
+   claw2d_sharpclaw_set_launch uses 4).  No counterpart in the reference. */
+int claw2d_sharpclaw_set_launch_ex(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision,
+                                   int band, int waves_per_simd, int waves_per_block);
+
+/* The launch shape SharpClaw stages of this problem shape use on this
+   context (tuned, set, or the default).  No counterpart in the reference. */
+int claw2d_sharpclaw_get_launch(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision,
+                                int *band, int *waves_per_simd, int *waves_per_block);
+
 /* ---- Batched runs of many independent grids ---------------------------------
    No counterpart in the reference (which runs one grid per process); this is
    the batching of the build's plan: instances of different sizes advance

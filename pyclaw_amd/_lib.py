@@ -100,6 +100,11 @@ def _optional_symbols():
                                       ctypes.POINTER(ctypes.c_int)],
         "claw2d_sharpclaw_set_launch": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int],
+        "claw2d_sharpclaw_set_launch_ex": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "claw2d_sharpclaw_get_launch": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)],
         "claw2d_classic_set_launch_ex": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod),
                                          ctypes.POINTER(ClawRiemann), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int],

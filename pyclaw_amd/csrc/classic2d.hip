@@ -790,18 +790,6 @@ __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-// Band height that gives about one band per wavefront slot of the device
-// (CUs x 4 SIMDs x waves per SIMD): a single round of wavefronts, none left
-// over for a second one.
-static int balanced_band(int device, int rows, int strips_x, int waves_per_simd) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
-        cus = 256;
-    const long long slots = (long long)cus * 4 * waves_per_simd;
-    const long long bands = std::max<long long>(1, slots / std::max(strips_x, 1));
-    return (int)std::max<long long>(4, (rows + bands - 1) / bands);
-}
-
 static int band_height(int rows, int strips_x) {
     // about 2048 wavefronts (two per SIMD) with bands of at least 4 rows; small
     // grids trade the 3 extra rows each band marches through for more waves

@@ -298,6 +298,10 @@ void sharp_stage_kernel_w3(const SharpParams<RP, R> p) {
 }
 
 __global__ void zero_slot_kernel(double *p) { *p = 0.0; }
+__global__ void zero_slots_kernel(double *p, int n) {
+    const int k = threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}
 
 template <class RP, typename R>
 static typename RP::Params make_sharp_params(const ClawRiemann *rp);
@@ -328,13 +332,22 @@ static TuneKey sharp_key(const ClawGrid *grid, const ClawRiemann *rp, int precis
     return TuneKey{rp->kind, SHARP_KEY, 0, precision, grid->mx, grid->my};
 }
 
-// band height and build (1: default, 3: three waves per SIMD) for this call:
-// the tuner's candidate, the tuned value, or the default
-static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int strips, int &w) {
+// band height, build (1: default, 3: three waves per SIMD) and wavefronts per
+// block for this call: the tuner's candidate, the tuned value, or the default
+static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int strips, int &w,
+                      int &wpb) {
     w = 1;
-    if (ctx && ctx->force_band > 0) { w = ctx->force_w; return std::min(ctx->force_band, grid->my); }
+    wpb = 4;
+    if (ctx && ctx->force_band > 0) {
+        w = ctx->force_w;
+        if (ctx->force_wpb > 0) wpb = ctx->force_wpb;
+        return std::min(ctx->force_band, grid->my);
+    }
     if (ctx) {
-        auto it = ctx->band.find(sharp_key(grid, rp, precision));
+        const TuneKey key = sharp_key(grid, rp, precision);
+        auto iw = ctx->wpb.find(key);
+        if (iw != ctx->wpb.end()) wpb = iw->second;
+        auto it = ctx->band.find(key);
         if (it != ctx->band.end()) { w = it->second.second; return std::min(it->second.first, grid->my); }
     }
     return sharp_band_height(grid->my, strips);
@@ -344,7 +357,7 @@ template <class RP, typename R>
 static int launch_sharp(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc, int mode,
                         const void *qin, const void *q0, void *s2, void *out, const void *aux,
                         const double *dt, double *cfl, bool zero_cfl, hipStream_t st) {
-    const int wpb = 4;
+    int wpb = 4;
     SharpParams<RP, R> p;
     p.qin = (const R *)qin; p.q0 = (const R *)q0; p.s2 = (R *)s2; p.out = (R *)out; p.aux = (const R *)aux;
     p.g = GridDev{grid->mx, grid->my, grid->ng, grid->ldq, grid->plane};
@@ -354,7 +367,7 @@ static int launch_sharp(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, c
     p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy; p.mode = mode;
     p.nsx = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
     int w = 1;
-    p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_SHARP_F64_KEY, p.nsx, w);
+    p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_SHARP_F64_KEY, p.nsx, w, wpb);
     p.nsy = (grid->my + p.H - 1) / p.H;
     p.wpb = wpb;
     p.nsyb = (p.nsy + wpb - 1) / wpb;
@@ -466,12 +479,14 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
         for (int b = a + 1; b < 5; ++b)
             if (bufs[a] == bufs[b]) return CLAW_EINVAL;
     hipStream_t st = (hipStream_t)stream;
+    // the ten stages' Courant slots are cleared by one launch
+    hipLaunchKernelGGL(zero_slots_kernel, dim3(1), dim3(WAVE), 0, st, cfl10_dev, 10);
     // stage k reads `in` and writes `out`; s1 ping-pongs between s1a and s1b
     void *s1[2] = {s1a, s1b};
     int k = 0, cur = 0;
 #define STAGE(MODE, IN, OUT)                                                                                   \
     do {                                                                                                       \
-        rc = sharp_route((Ctx *)ctx, grid, rp, bc, precision, MODE, IN, q, s2, OUT, aux, dt_dev, cfl10_dev + k, true, st); \
+        rc = sharp_route((Ctx *)ctx, grid, rp, bc, precision, MODE, IN, q, s2, OUT, aux, dt_dev, cfl10_dev + k, false, st); \
         if (rc) return rc;                                                                                     \
         ++k;                                                                                                   \
     } while (0)
@@ -486,8 +501,25 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
 
 int claw2d_sharpclaw_set_launch(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int band,
                                 int waves_per_simd) {
+    return claw2d_sharpclaw_set_launch_ex(ctx, grid, rp, precision, band, waves_per_simd, 4);
+}
+
+int claw2d_sharpclaw_set_launch_ex(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int band,
+                                   int waves_per_simd, int waves_per_block) {
     if (!ctx || !grid || !rp || band < 1 || (waves_per_simd != 1 && waves_per_simd != 3)) return CLAW_EINVAL;
-    ((Ctx *)ctx)->band[sharp_key(grid, rp, precision)] = std::make_pair(band, waves_per_simd);
+    if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) return CLAW_EINVAL;
+    Ctx *c = (Ctx *)ctx;
+    const TuneKey key = sharp_key(grid, rp, precision);
+    c->band[key] = std::make_pair(band, waves_per_simd);
+    c->wpb[key] = waves_per_block;
+    return CLAW_OK;
+}
+
+int claw2d_sharpclaw_get_launch(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int *band,
+                                int *waves_per_simd, int *waves_per_block) {
+    if (!ctx || !grid || !rp || !band || !waves_per_simd || !waves_per_block) return CLAW_EINVAL;
+    *band = sharp_band((Ctx *)ctx, grid, rp, precision, (grid->mx + SX_USEFUL - 1) / SX_USEFUL, *waves_per_simd,
+                       *waves_per_block);
     return CLAW_OK;
 }
 
@@ -505,14 +537,21 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
         if (e0) (void)hipEventDestroy(e0);
         return CLAW_EDEVICE;
     }
-    const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
-    int best = 0, best_w = 1;
+    // row-band heights (ending with the balanced height of each build) x
+    // (1 or 3 wavefronts per SIMD) x (1 or 4 wavefronts per block)
+    const int strips = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
+    const int cand[11] = {4, 6, 8, 12, 16, 24, 32, 48, 64, balanced_band(c->device, grid->my, strips, 1),
+                          balanced_band(c->device, grid->my, strips, 3)};
+    int best = 0, best_w = 1, best_wpb = 4;
     float best_ms = 0.f;
-    for (int k = 0; k < 18; ++k) {
-        const int h = cand[k % 9], w = k < 9 ? 1 : 3;
+    for (int k = 0; k < 44; ++k) {
+        const int h = cand[k % 11], w = (k / 11) % 2 == 0 ? 1 : 3, wpb = k < 22 ? 4 : 1;
         if (h > grid->my && h != cand[0]) continue;
+        if ((k % 11) == 9 && w != 1) continue;
+        if ((k % 11) == 10 && w != 3) continue;
         c->force_band = h;
         c->force_w = w;
+        c->force_wpb = wpb;
         rc = sharp_route(c, grid, rp, bc, precision, SM_DQ, q, nullptr, nullptr, dq, aux, dt_dev, cfl_dev, false, st);
         if (rc) break;
         const int reps = 5;
@@ -524,14 +563,16 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
         if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (best == 0 || ms < best_ms) { best = h; best_w = w; best_ms = ms; }
+        if (best == 0 || ms < best_ms) { best = h; best_w = w; best_wpb = wpb; best_ms = ms; }
     }
     c->force_band = 0;
     c->force_w = 0;
+    c->force_wpb = 0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     c->band[sharp_key(grid, rp, precision)] = std::make_pair(best, best_w);
+    c->wpb[sharp_key(grid, rp, precision)] = best_wpb;
     if (band_out) *band_out = best_w == 3 ? -best : best;
     return CLAW_OK;
 }

@@ -149,7 +149,7 @@ def test_autotuned_band_heights_give_identical_results():
             _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, _lib.F64, qa.data_ptr(), qb.data_ptr(),
                                                  None, dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(band)),
                        "autotune")
-            assert abs(band.value) in (4, 6, 8, 12, 16, 24, 32, 48, 64)
+            assert 1 <= abs(band.value) <= my
             qb.zero_()
         _lib.check(L.claw2d_classic_step(ctx.handle, grid, m, r, bc, _lib.F64, qa.data_ptr(), qb.data_ptr(), None,
                                          dtd.data_ptr(), cfl.data_ptr(), sh), "step")
