@@ -20,7 +20,7 @@ RP = {'acoustics': 'rp2_acoustics', 'vc_acoustics': 'rp2_vc_acoustics', 'euler':
       'shallow': 'rp2_shallow_roe_with_efix'}
 
 
-def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False):
+def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False, fma=False):
     import pyclaw_amd as pc
     x = pc.Dimension('x', opts['lower'][0], opts['upper'][0], opts['mx'])
     y = pc.Dimension('y', opts['lower'][1], opts['upper'][1], opts['my'])
@@ -42,6 +42,7 @@ def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False)
         solver = pc.SharpClawSolver2D(rp)
     solver.limiters = opts['limiters']
     solver.autotune = autotune
+    solver.fma = fma
     solver.cfl_max = opts['cfl_max']
     solver.cfl_desired = opts['cfl_desired']
     solver.dt_initial = opts['dt_initial']
@@ -130,3 +131,20 @@ def test_autotune_keeps_frames(kind):
     ref, _ = amd_run(q0, aux, opts)
     got, _ = amd_run(q0, aux, opts, autotune=True)
     _compare_frames(got, ref)
+
+
+@pytest.mark.parametrize("kind,tol", [('classic', 1e-11), ('sharpclaw', 1e-6)])
+def test_fma_app_frames_close(kind, tol):
+    """solver.fma = True runs the fused-multiply-add kernels through the drop-in
+    API: every frame stays close to the bit-exact run (classic: rounding level;
+    SharpClaw: WENO5 amplifies rounding differences on this smooth pulse) and
+    differs from it somewhere (the fused build really ran)."""
+    print("app: acoustics_2d_variable %s fma" % kind, flush=True)
+    q0, aux, opts = apps.acoustics_variable(kind)
+    ref, _ = amd_run(q0, aux, opts)
+    got, _ = amd_run(q0, aux, opts, fma=True)
+    assert len(got) == len(ref)
+    worst = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(got, ref))
+    print("fma app worst frame rel Linf", worst, flush=True)
+    assert worst <= tol
+    assert any(not np.array_equal(a, b) for a, b in zip(got, ref))
