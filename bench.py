@@ -36,7 +36,7 @@ BYTES_PER_CELL = 80          # read q (5 x f64) + write q (5 x f64)
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at the given launch shape, each scaled
 # by the same counter's reading for a 512 MiB device copy (profiles/
 # classic_unsplit_euler_4096x1024.md); reported only when the run uses that shape.
-MEASURED_TRAFFIC = {('f64', 4096, 1024, 24, 2, 4): 399816236, ('f64fma', 4096, 1024, 36, 2, 1): 378081364}
+MEASURED_TRAFFIC = {('f64', 4096, 1024, 24, 2, 4): 399816236, ('f64fma', 4096, 1024, 18, 2, 1): 415098340}
 
 
 def parse():
@@ -55,7 +55,7 @@ def parse():
     ap.add_argument('--launch', default=None,
                     help="ROWS,WAVES[,WPB]: set the Euler kernel's launch shape (row-band height, waves per "
                          "SIMD build, waves per block) instead of autotuning; the "
-                         "4096x1024 FP64 lines default to 36,2,1 (as profiled)")
+                         "4096x1024 FP64 lines default to 18,2,1 (as profiled)")
     ap.add_argument('--autotune', action='store_true', help="autotune the launch shape even where a default is pinned")
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
@@ -146,10 +146,10 @@ def main():
 
     band = None
     if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
-        # one 36-row band per wavefront slot (69 strips x 29 bands on 2048 slots), one-wavefront
-        # blocks: the fastest shape in repeated 300-step runs (the tuner's short bursts from the
-        # initial data rank 24 rows marginally ahead)
-        args.launch = '36,2,1'
+        # 18-row bands (69 strips x 57 bands: just under two rounds of the 2048 wavefront slots),
+        # one-wavefront blocks: the fastest shape in repeated 300-step runs (the tuner ranks shapes
+        # on the initial data, where the per-step cost differs)
+        args.launch = '18,2,1'
     if args.launch:
         shape = [int(v) for v in args.launch.split(',')]
         rows_, waves_, wpb_ = shape[0], shape[1], (shape[2] if len(shape) > 2 else 4)

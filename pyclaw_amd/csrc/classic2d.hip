@@ -125,8 +125,6 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     const int i = c0 - 2 + lane;
     const int R0 = sy * p.H;
     const int R1 = min(R0 + p.H, g.my);
-    const int up = ((lane + WAVE - 1) & (WAVE - 1)) << 2;
-    const int dn = ((lane + 1) & (WAVE - 1)) << 2;
     // lanes 0, 1 and 62, 63 are halo: lane 0's own x-interface would need the
     // column left of the strip and is never used (it is not a limiter neighbour
     // of a produced cell's interface), lane 1's is
@@ -185,11 +183,11 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             R alo[NA], alm2[NA], al0[NA];
             Cache cx;
 #pragma unroll
-            for (int m = 0; m < NE; ++m) lo[m] = lane_get(qprev[m], up);
+            for (int m = 0; m < NE; ++m) lo[m] = lane_up(qprev[m]);
             if constexpr (HAS_AUX) {
 #pragma unroll
                 for (int a = 0; a < NA; ++a) {
-                    alo[a] = lane_get(a_m1[a], up); alm2[a] = lane_get(a_m2[a], up); al0[a] = lane_get(a_0[a], up);
+                    alo[a] = lane_up(a_m1[a]); alm2[a] = lane_up(a_m2[a]); al0[a] = lane_up(a_0[a]);
                 }
             }
             RP::template rpn<0>(P, lo, qprev, alo, a_m1, wave, s, amdq, apdq, cx);
@@ -205,14 +203,14 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     R wn[NE];
 #pragma unroll
                     for (int m = 0; m < NE; ++m)
-                        wn[m] = RP::template WNZ<0>(m, mw) ? lane_get(wave[mw][m], dn) : R(0);
+                        wn[m] = RP::template WNZ<0>(m, mw) ? lane_dn(wave[mw][m]) : R(0);
                     const int mth = p.mthlim[mw];
                     if (mth == 0) continue;
                     R wnorm2 = R(0), dotr = R(0);
 #pragma unroll
                     for (int m = 0; m < NE; ++m)
                         if (RP::template WNZ<0>(m, mw)) { wnorm2 = wnorm2 + wave[mw][m] * wave[mw][m]; dotr = dotr + wave[mw][m] * wn[m]; }
-                    R dotl = lane_get(dotr, up);
+                    R dotl = lane_up(dotr);
                     if (wnorm2 != R(0)) {
                         R phi = philim<R>(wnorm2, s[mw] > R(0) ? dotl : dotr, mth);
 #pragma unroll
@@ -248,8 +246,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             }
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
-                R dAm = lane_get(amdq[m], dn), dF = lane_get(fadd[m], dn);
-                R dBm = lane_get(bmA[m], dn), dBp = lane_get(bpA[m], dn);
+                R dAm = lane_dn(amdq[m]), dF = lane_dn(fadd[m]);
+                R dBm = lane_dn(bmA[m]), dBp = lane_dn(bpA[m]);
                 R qadd = (R(0) - dtdx * apdq[m]) - dtdx * dAm;
                 R g1 = (R(0) - hx * dBm) - hx * bmP[m];
                 R g2 = (R(0) - hx * dBp) - hx * bpP[m];
@@ -337,8 +335,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                 if constexpr (HAS_AUX) {
 #pragma unroll
                     for (int a = 0; a < NA; ++a) {
-                        am2l[a] = lane_get(a_m2[a], up); am2r[a] = lane_get(a_m2[a], dn);
-                        am1l[a] = lane_get(a_m1[a], up); am1r[a] = lane_get(a_m1[a], dn);
+                        am2l[a] = lane_up(a_m2[a]); am2r[a] = lane_dn(a_m2[a]);
+                        am1l[a] = lane_up(a_m1[a]); am1r[a] = lane_dn(a_m1[a]);
                     }
                 }
                 RP::template rpt<1>(P, 1, tA, am2l, a_m2, am2r, bmA, bpA, cY);
@@ -354,7 +352,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     R qadd = (R(0) - dtdy * uAp) - dtdy * amY[m];
                     R g1 = (R(0) - hy * bmA[m]) - hy * uBm;
                     R g2 = (R(0) - hy * bpA[m]) - hy * uBp;
-                    R g2l = lane_get(g2, up), g1r = lane_get(g1, dn);
+                    R g2l = lane_up(g2), g1r = lane_dn(g1);
                     R v = accOld[m] + dtdx * g2l;
                     v = v + ((qadd - dtdy * (fY[m] - uF)) - dtdx * (g2 - g1));
                     v = v - dtdx * g1r;
@@ -469,8 +467,6 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
     const int i = c0 - 1 + lane;
     const int R0 = -CNG + sy * p.H;
     const int R1 = min(R0 + p.H, g.my + CNG);
-    const int up = ((lane + WAVE - 1) & (WAVE - 1)) << 2;
-    const int dn = ((lane + 1) & (WAVE - 1)) << 2;
     const bool l0 = (lane == 0);
     const bool produce = lane >= 1 && lane <= XS_USEFUL && i < g.mx;
     const double dtd = *p.dt;
@@ -484,13 +480,13 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
         R lo0[NE];
         if (l0) load_q<NE>(p.qin, g, p.bq, i - 1, row, lo0);
 #pragma unroll
-        for (int m = 0; m < NE; ++m) { lo[m] = lane_get(hi[m], up); if (l0) lo[m] = lo0[m]; }
+        for (int m = 0; m < NE; ++m) { lo[m] = lane_up(hi[m]); if (l0) lo[m] = lo0[m]; }
         if constexpr (HAS_AUX) {
             load_aux<NA>(p.aux, g, p.ba, i, row, ahi);
             R a0l[NA];
             if (l0) load_aux<NA>(p.aux, g, p.ba, i - 1, row, a0l);
 #pragma unroll
-            for (int a = 0; a < NA; ++a) { alo[a] = lane_get(ahi[a], up); if (l0) alo[a] = a0l[a]; }
+            for (int a = 0; a < NA; ++a) { alo[a] = lane_up(ahi[a]); if (l0) alo[a] = a0l[a]; }
         }
         R wave[NW][NE], s[NW], amdq[NE], apdq[NE], fadd[NE];
         Cache cx;
@@ -507,14 +503,14 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
                 R wn[NE];
 #pragma unroll
                 for (int m = 0; m < NE; ++m)
-                    wn[m] = RP::template WNZ<0>(m, mw) ? lane_get(wave[mw][m], dn) : R(0);
+                    wn[m] = RP::template WNZ<0>(m, mw) ? lane_dn(wave[mw][m]) : R(0);
                 const int mth = p.mthlim[mw];
                 if (mth == 0) continue;
                 R wnorm2 = R(0), dotr = R(0);
 #pragma unroll
                 for (int m = 0; m < NE; ++m)
                     if (RP::template WNZ<0>(m, mw)) { wnorm2 = wnorm2 + wave[mw][m] * wave[mw][m]; dotr = dotr + wave[mw][m] * wn[m]; }
-                R dotl = lane_get(dotr, up);
+                R dotl = lane_up(dotr);
                 if (wnorm2 != R(0)) {
                     R phi = philim<R>(wnorm2, s[mw] > R(0) ? dotl : dotr, mth);
 #pragma unroll
@@ -534,7 +530,7 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
         const long long off = (long long)(row + g.ng) * g.ldq + (i + g.ng);
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
-            R dAm = lane_get(amdq[m], dn), dF = lane_get(fadd[m], dn);
+            R dAm = lane_dn(amdq[m]), dF = lane_dn(fadd[m]);
             R qadd = (R(0) - dtdx * apdq[m]) - dtdx * dAm;
             R v = (hi[m] + qadd) - dtdx * (dF - fadd[m]);
             if (produce) p.qout[m * g.plane + off] = v;
@@ -1183,18 +1179,21 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         return CLAW_EDEVICE;
     }
     // row-band heights x (1 or 2 wavefronts per SIMD) x (1 or 4 wavefronts per
-    // block); the list ends with the balanced height of each build
+    // block); the list ends with the balanced heights (one and two rounds) of
+    // each build
     const int strips = (grid->mx + XU_USEFUL - 1) / XU_USEFUL;
-    const int cand[11] = {4, 6, 8, 12, 16, 24, 32, 48, 64, balanced_band(c->device, grid->my, strips, 1),
-                          balanced_band(c->device, grid->my, strips, 2)};
+    const int cand[13] = {4, 6, 8, 12, 16, 24, 32, 48, 64, balanced_band(c->device, grid->my, strips, 1),
+                          balanced_band(c->device, grid->my, strips, 2),
+                          balanced_band(c->device, grid->my, strips, 1, 2),
+                          balanced_band(c->device, grid->my, strips, 2, 2)};
     int best = 0, best_w = 1, best_wpb = 4;
     float best_ms = 0.f;
     int rc = CLAW_OK;
-    for (int k = 0; k < 44 && !rc; ++k) {
-        const int h = cand[k % 11], w = (k / 11) % 2 == 0 ? 1 : 2, wpb = k < 22 ? 4 : 1;
+    for (int k = 0; k < 52 && !rc; ++k) {
+        const int h = cand[k % 13], w = (k / 13) % 2 == 0 ? 1 : 2, wpb = k < 26 ? 4 : 1;
         if (h > grid->my && h != cand[0]) continue;
-        if ((k % 11) == 9 && w != 1) continue;    // each balanced height with its own build
-        if ((k % 11) == 10 && w != 2) continue;
+        if (((k % 13) == 9 || (k % 13) == 11) && w != 1) continue;    // each balanced height with its own build
+        if (((k % 13) == 10 || (k % 13) == 12) && w != 2) continue;
         c->force_band = h;
         c->force_w = w;
         c->force_wpb = wpb;

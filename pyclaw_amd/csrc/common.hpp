@@ -20,6 +20,26 @@ __device__ __forceinline__ double lane_get(double v, int addr) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// ---- neighbour exchange by whole-wavefront DPP rotates (VALU, no LDS) -----
+// lane_up: lane L receives lane L-1's value (lane 0 receives lane 63's);
+// lane_dn: lane L receives lane L+1's value (lane 63 receives lane 0's).
+// The same values as lane_get with addresses ((lane -/+ 1) & 63) * 4.
+constexpr int DPP_WAVE_ROL1 = 0x134, DPP_WAVE_ROR1 = 0x13C;
+template <int CTRL> __device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = dpp_move<CTRL>((int)(b & 0xffffffffll));
+    int hi = dpp_move<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T> __device__ __forceinline__ T lane_up(T v) { return dpp_move<DPP_WAVE_ROR1>(v); }
+template <typename T> __device__ __forceinline__ T lane_dn(T v) { return dpp_move<DPP_WAVE_ROL1>(v); }
+
 // ---- grid description on the device ---------------------------------------
 struct GridDev {
     int mx, my, ng;

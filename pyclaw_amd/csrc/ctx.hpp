@@ -30,14 +30,14 @@ struct Ctx {
 
 constexpr int SHARP_KEY = 100;
 
-// Band height that gives about one band per wavefront slot of the device
-// (CUs x 4 SIMDs x waves per SIMD): a single round of wavefronts, none left
-// over for a second one.
-inline int balanced_band(int device, int rows, int strips_x, int waves_per_simd) {
+// Band height that gives about `rounds` bands per wavefront slot of the device
+// (CUs x 4 SIMDs x waves per SIMD): whole rounds of wavefronts, none left
+// over for a partial one.
+inline int balanced_band(int device, int rows, int strips_x, int waves_per_simd, int rounds = 1) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
         cus = 256;
-    const long long slots = (long long)cus * 4 * waves_per_simd;
+    const long long slots = (long long)cus * 4 * waves_per_simd * rounds;
     const long long bands = std::max<long long>(1, slots / std::max(strips_x, 1));
     return (int)std::max<long long>(4, (rows + bands - 1) / bands);
 }
