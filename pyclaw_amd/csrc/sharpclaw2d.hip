@@ -152,10 +152,6 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     const int i = c0 - 3 + lane;
     const int R0 = sy * p.H;
     const int R1 = min(R0 + p.H, g.my);
-    const int up1 = ((lane + WAVE - 1) & (WAVE - 1)) << 2;
-    const int up2 = ((lane + WAVE - 2) & (WAVE - 1)) << 2;
-    const int dn1 = ((lane + 1) & (WAVE - 1)) << 2;
-    const int dn2 = ((lane + 2) & (WAVE - 1)) << 2;
     const bool produce = lane >= 3 && lane < 3 + SX_USEFUL && i < g.mx;
 
     const double dtd = *p.dt;
@@ -219,16 +215,16 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
             R xm2[NE], xm1[NE], xp1[NE], xp2[NE], qlX[NE], qrX[NE], qrXl[NE];
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
-                xm2[m] = lane_get(win[1][m], up2); xm1[m] = lane_get(win[1][m], up1);
-                xp1[m] = lane_get(win[1][m], dn1); xp2[m] = lane_get(win[1][m], dn2);
+                xm2[m] = lane_up(lane_up(win[1][m])); xm1[m] = lane_up(win[1][m]);
+                xp1[m] = lane_dn(win[1][m]); xp2[m] = lane_dn(lane_dn(win[1][m]));
                 weno5<R>(xm2[m], xm1[m], win[1][m], xp1[m], xp2[m], qlX[m], qrX[m]);
             }
 #pragma unroll
-            for (int m = 0; m < NE; ++m) qrXl[m] = lane_get(qrX[m], up1);
+            for (int m = 0; m < NE; ++m) qrXl[m] = lane_up(qrX[m]);
             R alx[NA];
             if constexpr (HAS_AUX) {
 #pragma unroll
-                for (int a = 0; a < NA; ++a) alx[a] = lane_get(aw[1][a], up1);
+                for (int a = 0; a < NA; ++a) alx[a] = lane_up(aw[1][a]);
             }
             R amX[NE], apX[NE], am2X[NE], ap2X[NE];
             {
@@ -245,7 +241,7 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
                 const long long off = (long long)(o + g.ng) * g.ldq + (i + g.ng);
 #pragma unroll
                 for (int m = 0; m < NE; ++m) {
-                    R amXr = lane_get(amX[m], dn1);
+                    R amXr = lane_dn(amX[m]);
                     R dqx = R(0) - dtdx * (((amXr + apX[m]) + am2X[m]) + ap2X[m]);
                     R dqy = R(0) - dtdy * (((amY[m] + apY_prev[m]) + am2Y_prev[m]) + ap2Y_prev[m]);
                     R dq = (R(0) + dqx) + dqy;
