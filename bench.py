@@ -569,6 +569,7 @@ def bench_adjoint(args):
     t_fwd_cp = time.perf_counter() - t0
     t0 = time.perf_counter()
     lam, gaux = P.backward(traces, cps, aux)
+    t_issue = time.perf_counter() - t0      # host time to issue the reverse sweep's launches
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
     J = float(P.objective(traces))
@@ -589,19 +590,21 @@ def bench_adjoint(args):
     del cps1
     P.every = 45
     # per-kernel averages on the launch stream: forward step vs adjoint step
+    from pyclaw_amd.adjoint import _Launches
     st = torch.cuda.current_stream()
+    go = _Launches(P, aux, st, P._cfl)
     lam2 = torch.zeros_like(lam)
     reps = 50
     e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     a, b = q0p.clone(), torch.empty_like(q0p)
+    ab = [a.data_ptr(), b.data_ptr()]
     e[0].record(st)
     for k in range(reps):
-        P._step(a, b, aux)
-        a, b = b, a
+        go.step(ab[k % 2], ab[1 - k % 2])
     e[1].record(st)
     e[2].record(st)
     for k in range(reps):
-        P._adjoint_step(a, aux, lam, lam2, gaux)
+        go.adjoint(ab[0], lam.data_ptr(), lam2.data_ptr(), gaux.data_ptr())
     e[3].record(st)
     torch.cuda.synchronize()
     us_fwd = e[0].elapsed_time(e[1]) * 1e3 / reps
@@ -619,7 +622,8 @@ def bench_adjoint(args):
                         "unit": "GB/s", "frac": adj_bytes / (us_adj * 1e-6) / 1e9 / HBM_PEAK_GBS,
                         "traffic": None, "bytes_per_cell_update": 120, "kernel": "adjoint step (2 launches)"},
            "extra": {"forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_cp, "reverse_s": t_bwd,
-                     "reverse_over_forward": t_bwd / t_fwd, "forward_step_us": us_fwd, "adjoint_step_us": us_adj,
+                     "reverse_over_forward": t_bwd / t_fwd, "reverse_issue_s": t_issue,
+                     "forward_step_us": us_fwd, "adjoint_step_us": us_adj,
                      "adjoint_over_forward_step": us_adj / us_fwd, "J": J, "finite": finite,
                      "checkpoints": (N + 44) // 45,
                      "all_states_in_hbm": {"forward_s": t_fwd_all, "reverse_s": t_bwd_all,

@@ -15,7 +15,10 @@ as a checkpoint and the pressure at the receiver cells after each step
 segment's states are recomputed from its checkpoint and the adjoint steps
 (claw2d_vc_adjoint_step, hand-written HIP kernels) run backwards through it;
 each receiver residual enters lam at its cell (claw2d_misfit_source).
-Memory: ceil(N/every) checkpoints plus one segment of `every` states; with
+The forward pass also keeps the last segment's states, which the reverse
+sweep needs first, so it starts without waiting for a recomputation.
+Memory: ceil(N/every) checkpoints plus three segments of `every` states (the
+last segment, and two buffers that the reverse sweep recomputes into); with
 every=1 all N+1 states stay in HBM and nothing is recomputed.
 
 VCAcousticsMisfit.apply(q0, K, rho) is the torch.autograd.Function form.
@@ -26,6 +29,67 @@ import torch
 
 from . import _lib
 from .state import NG_STORE
+
+
+class Checkpoints(object):
+    """States kept by a checkpointed forward pass: states[k] = q_{k*every}, and
+    tail[n - s0] = q_n for the last segment's steps n = s0 .. N-1
+    (s0 = (ceil(N/every) - 1) * every).  Indexing reads the checkpoints."""
+
+    def __init__(self, states, tail, first):
+        self.states, self.tail, self.first = states, tail, first
+
+    def __getitem__(self, k):
+        return self.states[k]
+
+    def __len__(self):
+        return self.states.shape[0]
+
+
+class _Launches(object):
+    """The launches of one pass with every constant argument bound once: states
+    and traces are plain device addresses (base + index * stride), so a launch
+    is one ctypes call with no tensor views (a pass issues thousands)."""
+
+    def __init__(self, P, aux, stream, cfl):
+        self.L = P.L
+        self.ctx = P._ctx.handle
+        self.grid, self.meth = ctypes.byref(P.grid), ctypes.byref(P.method)
+        self.rp, self.bc = ctypes.byref(P.rp), ctypes.byref(P.bc)
+        self.auxp, self.dtp, self.cflp = aux.data_ptr(), P._dt.data_ptr(), cfl.data_ptr()
+        self.dt = P.dt
+        self.wsp, self.wsb = P._ws.data_ptr(), P._ws.numel() * 8
+        self.offp, self.nrec = P.offsets.data_ptr(), P.nrec
+        self.st = stream.cuda_stream
+
+    def step(self, q, qn):
+        rc = self.L.claw2d_classic_step_ex(self.ctx, self.grid, self.meth, self.rp, self.bc, _lib.F64, q, qn,
+                                           self.auxp, self.dtp, self.cflp, _lib.STEP_ACCUMULATE_CFL, self.st)
+        if rc:
+            _lib.check(rc, "claw2d_classic_step_ex")
+
+    def record(self, q, out):
+        rc = self.L.claw2d_gather(q, self.offp, self.nrec, out, self.st)
+        if rc:
+            _lib.check(rc, "claw2d_gather")
+
+    def source(self, lam, tr, obs):
+        rc = self.L.claw2d_misfit_source(lam, self.offp, self.nrec, tr, obs, self.st)
+        if rc:
+            _lib.check(rc, "claw2d_misfit_source")
+
+    def adjoint_misfit(self, q, lam_in, lam_out, gaux, tr, obs):
+        """Adjoint step, then the residual of the previous step's traces enters lam_out."""
+        rc = self.L.claw2d_vc_adjoint_step_misfit(self.grid, self.meth, self.bc, q, self.auxp, lam_in, lam_out, gaux,
+                                                  self.dt, self.wsp, self.wsb, self.offp, self.nrec, tr, obs, self.st)
+        if rc:
+            _lib.check(rc, "claw2d_vc_adjoint_step_misfit")
+
+    def adjoint(self, q, lam_in, lam_out, gaux):
+        rc = self.L.claw2d_vc_adjoint_step(self.grid, self.meth, self.bc, q, self.auxp, lam_in, lam_out, gaux,
+                                           self.dt, self.wsp, self.wsb, self.st)
+        if rc:
+            _lib.check(rc, "claw2d_vc_adjoint_step")
 
 
 class MisfitProblem(object):
@@ -111,24 +175,6 @@ class MisfitProblem(object):
                                            self._stream() if stream is None else stream.cuda_stream)
         _lib.check(rc, "claw2d_classic_step_ex")
 
-    def _record(self, q, out):
-        _lib.check(self.L.claw2d_gather(q.data_ptr(), self.offsets.data_ptr(), self.nrec, out.data_ptr(),
-                                        self._stream()), "claw2d_gather")
-
-    def _source(self, lam, traces_n, obs_n):
-        _lib.check(self.L.claw2d_misfit_source(lam.data_ptr(), self.offsets.data_ptr(), self.nrec,
-                                               traces_n.data_ptr(), obs_n.data_ptr(), self._stream()),
-                   "claw2d_misfit_source")
-
-    def _adjoint_step_misfit(self, q, aux, lam_in, lam_out, gaux, traces_n, obs_n):
-        """Adjoint step, then the residual of the previous step's traces enters lam_out."""
-        rc = self.L.claw2d_vc_adjoint_step_misfit(self.grid, self.method, self.bc, q.data_ptr(), aux.data_ptr(),
-                                                  lam_in.data_ptr(), lam_out.data_ptr(), gaux.data_ptr(), self.dt,
-                                                  self._ws.data_ptr(), self._ws.numel() * 8,
-                                                  self.offsets.data_ptr(), self.nrec, traces_n.data_ptr(),
-                                                  obs_n.data_ptr(), self._stream())
-        _lib.check(rc, "claw2d_vc_adjoint_step_misfit")
-
     def _adjoint_step(self, q, aux, lam_in, lam_out, gaux):
         rc = self.L.claw2d_vc_adjoint_step(self.grid, self.method, self.bc, q.data_ptr(), aux.data_ptr(),
                                            lam_in.data_ptr(), lam_out.data_ptr(), gaux.data_ptr(), self.dt,
@@ -154,33 +200,50 @@ class MisfitProblem(object):
 
     def forward(self, q0p, aux, keep_checkpoints=True):
         """N steps from the padded state q0p; returns (traces (N, nrec),
-        checkpoints, final state).  The checkpoints are one preallocated tensor
-        (ceil(N/every), ...) holding states 0, every, 2 every, ...; with
-        every == 1 it holds all N+1 states and the steps write into it directly."""
+        checkpoints, final state).  With every == 1 the checkpoints are one
+        tensor holding all N+1 states, written by the steps directly; otherwise
+        a Checkpoints object: states 0, every, 2 every, ... and the last
+        segment's states."""
         N, E = self.nsteps, self.every
         if not self._tuned:
             self.tune(q0p, aux)
         traces = torch.empty((N, self.nrec), dtype=torch.float64, device=self.dev)
         self._cfl.zero_()
         shape = tuple(q0p.shape)
+        go = _Launches(self, aux, torch.cuda.current_stream(self.dev), self._cfl)
+        sb, trp, rb = q0p.numel() * 8, traces.data_ptr(), self.nrec * 8
         if keep_checkpoints and E == 1:
             states = torch.empty((N + 1,) + shape, dtype=torch.float64, device=self.dev)
             states[0].copy_(q0p)
+            base = states.data_ptr()
             for n in range(N):
-                self._step(states[n], states[n + 1], aux)
-                self._record(states[n + 1], traces[n])
+                go.step(base + n * sb, base + (n + 1) * sb)
+                go.record(base + (n + 1) * sb, trp + n * rb)
             return traces, states, states[N]
-        cps = None
+        cps, s0 = None, N
         if keep_checkpoints:
-            cps = torch.empty(((N + E - 1) // E,) + shape, dtype=torch.float64, device=self.dev)
-        a, b = q0p.clone(), torch.empty_like(q0p)
+            nseg = (N + E - 1) // E
+            s0 = (nseg - 1) * E
+            cps = Checkpoints(torch.empty((nseg,) + shape, dtype=torch.float64, device=self.dev),
+                              torch.empty((N - s0,) + shape, dtype=torch.float64, device=self.dev), s0)
+        ab = [q0p.clone(), torch.empty_like(q0p)]   # ab[cur] holds the current state
+        abp = [t.data_ptr() for t in ab]
+        cur = 0
+        tail = cps.tail.data_ptr() if cps is not None else 0
         for n in range(N):
             if cps is not None and n % E == 0:
-                cps[n // E].copy_(a)
-            self._step(a, b, aux)
-            self._record(b, traces[n])
-            a, b = b, a
-        return traces, cps, a
+                cps.states[n // E].copy_(ab[cur])
+            if n == s0:
+                cps.tail[0].copy_(ab[cur])
+            if s0 <= n < N - 1:
+                # the last segment's steps write into the kept states directly
+                go.step(tail + (n - s0) * sb, tail + (n - s0 + 1) * sb)
+                go.record(tail + (n - s0 + 1) * sb, trp + n * rb)
+                continue
+            go.step(tail + (n - s0) * sb if n > s0 else abp[cur], abp[1 - cur])
+            go.record(abp[1 - cur], trp + n * rb)
+            cur = 1 - cur
+        return traces, cps, ab[cur]
 
     def backward(self, traces, cps, aux, p_obs=None):
         """Adjoint sweep; returns (dJ/dq0 padded, dJ/d(rho, c) padded)."""
@@ -188,28 +251,46 @@ class MisfitProblem(object):
         obs = self.p_obs if p_obs is None else torch.as_tensor(p_obs, dtype=torch.float64, device=self.dev)
         if obs is None:
             raise ValueError("observed traces are needed")
+        obs = obs.to(device=self.dev, dtype=torch.float64).contiguous()
+        traces = traces.contiguous()
+        if tuple(obs.shape) != (N, self.nrec) or tuple(traces.shape) != (N, self.nrec):
+            raise ValueError("traces and observed traces must be (nsteps, nreceivers)")
         lam = torch.zeros((3, self.rows, self.ldq), dtype=torch.float64, device=self.dev)
         lam2 = torch.zeros_like(lam)
         gaux = torch.zeros((2, self.rows, self.ldq), dtype=torch.float64, device=self.dev)
-        self._source(lam, traces[N - 1], obs[N - 1])
+        main = torch.cuda.current_stream(self.dev)
+        go = _Launches(self, aux, main, self._cfl)
+        sb, rb = lam.numel() * 8, self.nrec * 8
+        trp, obp = traces.data_ptr(), obs.data_ptr()
+        lp = [lam.data_ptr(), lam2.data_ptr()]
+        gp = gaux.data_ptr()
+        go.source(lp[0], trp + (N - 1) * rb, obp + (N - 1) * rb)
+
+        def sweep(base, s0, s1, cur):
+            # adjoint steps n = s1-1 .. s0 with q_n at base + (n - s0) * sb; lam alternates between lp[0], lp[1]
+            for n in range(s1 - 1, s0 - 1, -1):
+                q = base + (n - s0) * sb
+                if n >= 1:
+                    go.adjoint_misfit(q, lp[cur], lp[1 - cur], gp, trp + (n - 1) * rb, obp + (n - 1) * rb)
+                else:
+                    go.adjoint(q, lp[cur], lp[1 - cur], gp)
+                cur = 1 - cur
+            return cur
+
         if E == 1:
             # every state is stored: no recomputation
-            for n in range(N - 1, -1, -1):
-                if n >= 1:
-                    self._adjoint_step_misfit(cps[n], aux, lam, lam2, gaux, traces[n - 1], obs[n - 1])
-                else:
-                    self._adjoint_step(cps[n], aux, lam, lam2, gaux)
-                lam, lam2 = lam2, lam
-            return lam, gaux
+            cur = sweep(cps.data_ptr(), 0, N, 0)
+            return (lam if cur == 0 else lam2), gaux
         # Segments are recomputed on a second stream into the other of two
         # buffers while the adjoint steps of the previous segment run: the
         # recomputation of segment k-1 does not depend on segment k's adjoint.
-        main = torch.cuda.current_stream(self.dev)
         side = self._side_stream()
         seg = [torch.empty((E,) + tuple(lam.shape), dtype=torch.float64, device=self.dev) for _ in range(2)]
+        segp = [t.data_ptr() for t in seg]
         ready = [torch.cuda.Event() for _ in range(2)]
         free = [torch.cuda.Event() for _ in range(2)]
         cfl_side = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        go_side = _Launches(self, aux, side, cfl_side)
         nseg = (N + E - 1) // E
 
         def recompute(k, b):
@@ -218,28 +299,30 @@ class MisfitProblem(object):
                 side.wait_event(free[b])
                 seg[b][0].copy_(cps[k])
                 for n in range(s0, s1 - 1):
-                    self._step(seg[b][n - s0], seg[b][n - s0 + 1], aux, stream=side, cfl=cfl_side)
+                    go_side.step(segp[b] + (n - s0) * sb, segp[b] + (n - s0 + 1) * sb)
                 ready[b].record(side)
 
+        # the last segment's states come from the forward pass when it kept them
+        tail = getattr(cps, 'tail', None)
         side.wait_stream(main)           # checkpoints and aux are complete
         for b in range(2):
             free[b].record(main)
-        recompute(nseg - 1, (nseg - 1) % 2)
+        if tail is None:
+            recompute(nseg - 1, (nseg - 1) % 2)
+        cur = 0
         for k in range(nseg - 1, -1, -1):
             b = k % 2
             if k >= 1:
                 recompute(k - 1, (k - 1) % 2)
-            main.wait_event(ready[b])
-            s0, s1 = k * E, min((k + 1) * E, N)
-            for n in range(s1 - 1, s0 - 1, -1):
-                if n >= 1:
-                    self._adjoint_step_misfit(seg[b][n - s0], aux, lam, lam2, gaux, traces[n - 1], obs[n - 1])
-                else:
-                    self._adjoint_step(seg[b][n - s0], aux, lam, lam2, gaux)
-                lam, lam2 = lam2, lam
+            if k == nseg - 1 and tail is not None:
+                base = tail.data_ptr()
+            else:
+                main.wait_event(ready[b])
+                base = segp[b]
+            cur = sweep(base, k * E, min((k + 1) * E, N), cur)
             free[b].record(main)
         main.wait_stream(side)
-        return lam, gaux
+        return (lam if cur == 0 else lam2), gaux
 
     def objective(self, traces, p_obs=None):
         obs = self.p_obs if p_obs is None else torch.as_tensor(p_obs, dtype=torch.float64, device=self.dev)

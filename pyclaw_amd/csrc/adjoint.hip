@@ -29,9 +29,10 @@
 // deterministic and needs no atomics.  A block stages its tile of positions
 // and the cells within 3 of it (q with the boundary map applied, aux, lam) in
 // LDS first, so the recomputation reads LDS, not global memory.  Tiles cover
-// the ghost frame too; ghost positions write to a workspace, and a second,
-// small launch adds each ghost's contribution to the ring cell it was copied
-// from.
+// the ghost frame too, and each ghost's contribution is added to the ring cell
+// it was copied from: through LDS inside the tile when no side is periodic
+// (the tiling puts every ghost in the tile of its source cell), otherwise
+// through a workspace and a second, small launch.
 //
 // Requirements: c > 0 in every cell (so wave 1 moves left and is limited with
 // the interface to its right, wave 2 with the interface to its left); boundary
@@ -584,98 +585,275 @@ __host__ __device__ __forceinline__ void ring_cell(const Params &P, long long k,
     }
 }
 
-// Sum, in a fixed order, of field m's frame contributions over the ghost
-// positions that the boundary map copies from ring cell (i, j): the candidates
-// are (gi, gj) with gi in {i, -2, -1, mx, mx+1} and gj in {j, -2, -1, my, my+1};
-// the map is separable (x index from the x codes, y index from the y codes;
-// bc_source) and a wall negates the normal momentum it reflects.  Every
-// candidate is a ghost position inside the frame, so all 24 loads are issued
-// unconditionally and the match selects them: no load waits on another.
-__host__ __device__ __forceinline__ double fold_field(const Params &P, int i, int j, int m) {
-    const int mx = P.g.mx, my = P.g.my;
-    const BCDev &bc = m < 3 ? P.bq : P.ba;
-    const double *src = P.frame + (long long)m * P.g.plane;
-    const int gis[5] = {i, -2, -1, mx, mx + 1};
-    const int gjs[5] = {j, -2, -1, my, my + 1};
-    bool mxa[5], mya[5], fxa[5], fya[5];
+// Ghost positions that the boundary map copies from ring cell (i, j), for
+// field m: the candidates are (gi[a], gj[b]) with gi in {i, -2, -1, mx, mx+1}
+// and gj in {j, -2, -1, my, my+1}, (a, b) != (0, 0); the map is separable (x
+// index from the x codes, y index from the y codes; bc_source) and a wall
+// negates the normal momentum it reflects.  A candidate contributes when
+// hx[a] && hy[b], negated when nx[a] != ny[b].
+struct FoldMap {
+    int gi[5], gj[5];
+    bool hx[5], hy[5], nx[5], ny[5];
+    __host__ __device__ __forceinline__ FoldMap(const Params &P, int i, int j, int m) {
+        const int mx = P.g.mx, my = P.g.my;
+        const BCDev &bc = m < 3 ? P.bq : P.ba;
+        gi[0] = i; gi[1] = -2; gi[2] = -1; gi[3] = mx; gi[4] = mx + 1;
+        gj[0] = j; gj[1] = -2; gj[2] = -1; gj[3] = my; gj[4] = my + 1;
 #pragma unroll
-    for (int a = 0; a < 5; ++a) {
-        bool f = false;
-        const int cx = a <= 2 ? bc.lo[0] : bc.hi[0], cy = a <= 2 ? bc.lo[1] : bc.hi[1];
-        mxa[a] = a == 0 || bc_map1(gis[a], mx, cx, f) == i;
-        fxa[a] = a != 0 && m == 1 && cx == CLAW_BC_WALL;
-        mya[a] = a == 0 || bc_map1(gjs[a], my, cy, f) == j;
-        fya[a] = a != 0 && m == 2 && cy == CLAW_BC_WALL;
+        for (int a = 0; a < 5; ++a) {
+            bool f = false;
+            const int cx = a <= 2 ? bc.lo[0] : bc.hi[0], cy = a <= 2 ? bc.lo[1] : bc.hi[1];
+            hx[a] = a == 0 || bc_map1(gi[a], mx, cx, f) == i;
+            nx[a] = a != 0 && m == 1 && cx == CLAW_BC_WALL;
+            hy[a] = a == 0 || bc_map1(gj[a], my, cy, f) == j;
+            ny[a] = a != 0 && m == 2 && cy == CLAW_BC_WALL;
+        }
     }
+};
+
+// Sum, in a fixed order (b outer, a inner), of field m's frame contributions
+// over the ghost positions that the boundary map copies from ring cell
+// (i, j).  Every candidate is a ghost position inside the frame, so all 24
+// loads are issued unconditionally and the match selects them: no load waits
+// on another.
+__host__ __device__ __forceinline__ double fold_field(const Params &P, int i, int j, int m) {
+    const FoldMap F(P, i, j, m);
+    const double *src = P.frame + (long long)m * P.g.plane;
     double acc = 0.0;
 #pragma unroll
     for (int b = 0; b < 5; ++b) {
 #pragma unroll
         for (int a = 0; a < 5; ++a) {
             if (a == 0 && b == 0) continue;
-            const double v = src[(long long)(gjs[b] + P.g.ng) * P.g.ldq + (gis[a] + P.g.ng)];
-            const bool neg = fxa[a] != fya[b];
-            acc += (mxa[a] && mya[b]) ? (neg ? -v : v) : 0.0;
+            const double v = src[(long long)(F.gj[b] + P.g.ng) * P.g.ldq + (F.gi[a] + P.g.ng)];
+            const bool neg = F.nx[a] != F.ny[b];
+            acc += (F.hx[a] && F.hy[b]) ? (neg ? -v : v) : 0.0;
         }
     }
     return acc;
 }
 
-// Wave strengths and 1/D of every face inside the staged box (faces need both
-// cells in the box, so the box's first column has no x-face, its first row no
-// y-face), computed once per tile from the staged cells.
-__host__ __device__ __forceinline__ void tile_faces(const TileAcc &A, int t0, int nt) {
-    double *f = const_cast<double *>(A.f);
-    for (int k = t0; k < BOX; k += nt) {
-        const int bi = k % BOXW, bj = k / BOXW;
-        const int i = A.bx + bi, j = A.by + bj;
-        if (bi > 0) {
-            double a1, a2, iD;
-            strengths<0>(getq<0>(A, i - 1, j), getq<0>(A, i, j), geta<0>(A, i - 1, j), geta<0>(A, i, j), a1, a2, iD);
-            f[k] = a1; f[BOX + k] = a2; f[2 * BOX + k] = iD;
+// Tiles over the positions [-ANG, n+ANG) of one axis: width w from -ANG,
+// except that the last tile starts at n-ANG when it would otherwise start to
+// the right of it, so that it holds the high ghost positions together with
+// the ring cells they are copied from (the first tile holds the low ones).
+// Tile t owns the positions from its origin up to the next tile's origin.
+__host__ __device__ __forceinline__ int tile_origin(int t, int nt, int n, int w) {
+    const int o = w * t - ANG;
+    return (t == nt - 1 && o > n - ANG) ? n - ANG : o;
+}
+
+__host__ __device__ __forceinline__ int tile_end(int t, int nt, int n, int w) {
+    return t == nt - 1 ? n + ANG : tile_origin(t + 1, nt, n, w);
+}
+
+// number of tiles along x and y
+__host__ __device__ __forceinline__ int tiles_x(const Params &P) { return (P.g.mx + 2 * ANG + TW - 1) / TW; }
+__host__ __device__ __forceinline__ int tiles_y(const Params &P) { return (P.g.my + 2 * ANG + TH - 1) / TH; }
+
+// Whether every ghost position lies in the same tile as the ring cells it is
+// copied from: no periodic side (a periodic ghost copies from the far side)
+// and, when an axis has more than one tile, a last tile that starts at or
+// beyond ANG (the first tile then owns the low ring cells [0, ANG)).
+__host__ __device__ __forceinline__ bool fold_in_tile(const Params &P, int ntx, int nty) {
+    for (int s = 0; s < 2; ++s)
+        if (P.bq.lo[s] == CLAW_BC_PERIODIC || P.bq.hi[s] == CLAW_BC_PERIODIC || P.ba.lo[s] == CLAW_BC_PERIODIC ||
+            P.ba.hi[s] == CLAW_BC_PERIODIC)
+            return false;
+    if (ntx > 1 && tile_origin(ntx - 1, ntx, P.g.mx, TW) < ANG) return false;
+    if (nty > 1 && tile_origin(nty - 1, nty, P.g.my, TH) < ANG) return false;
+    return true;
+}
+
+// Field m of the ghost contributions folded into ring cell (i, j) from xg, the
+// tile's ghost positions (5 planes of TW x TH, origin (tx0, ty0)), in the order
+// of fold_field.
+__host__ __device__ __forceinline__ double fold_tile(const Params &P, const double *xg, int tx0, int ty0, int i, int j,
+                                                     int m) {
+    const FoldMap F(P, i, j, m);
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+            if (a == 0 && b == 0) continue;
+            if (F.hx[a] && F.hy[b]) {
+                const double v = xg[m * TW * TH + (F.gj[b] - ty0) * TW + (F.gi[a] - tx0)];
+                acc += F.nx[a] != F.ny[b] ? -v : v;
+            }
         }
-        if (bj > 0) {
-            double a1, a2, iD;
-            strengths<1>(getq<1>(A, j - 1, i), getq<1>(A, j, i), geta<1>(A, j - 1, i), geta<1>(A, j, i), a1, a2, iD);
-            f[3 * BOX + k] = a1; f[4 * BOX + k] = a2; f[5 * BOX + k] = iD;
-        }
+    }
+    return acc;
+}
+
+// Receiver r whose cell the tile owns goes to slot[(rj - ty0) * TW + (ri - tx0)]
+// (receivers are distinct cells: no slot is written twice).
+__host__ __device__ __forceinline__ void receiver_slots(const Params &P, int tx0, int ty0, int txe, int tye, int *slot,
+                                                        int t0, int nt) {
+    for (int r = t0; r < P.nrec; r += nt) {
+        const long long ro = P.rec_off[r];
+        const int rj = (int)(ro / P.g.ldq) - P.g.ng, ri = (int)(ro % P.g.ldq) - P.g.ng;
+        if (ri >= tx0 && ri < txe && rj >= ty0 && rj < tye) slot[(rj - ty0) * TW + (ri - tx0)] = r;
     }
 }
 
-// One block per TW x TH tile of the padded region: stage q (boundary map
-// applied), aux and lam for the tile plus HALO cells around it in LDS, then
-// one thread per position.
+// Interior cell (i, j) with its raw contributions: lam_n = lam_{n+1} + raw
+// (lam holds the cell's lam_{n+1}), the aux gradient (ga0, ga1: its values
+// before this step) accumulates, a ring cell then adds its ghosts'
+// contributions and a receiver cell its residual (trace minus observed) -- the
+// same operations in the same order as the position writes followed by the
+// fold launch.
+__host__ __device__ __forceinline__ void position_finish(const Params &P, const double *xg, int tx0, int ty0, int i,
+                                                         int j, const double (&lam)[3], double ga0, double ga1,
+                                                         const double (&gq)[3], double grho, double gc, int rec) {
+    const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
+    double v[5];
+    for (int m = 0; m < 3; ++m) v[m] = lam[m] + gq[m];
+    v[3] = ga0 + grho;
+    v[4] = ga1 + gc;
+    if (i < ANG || j < ANG || i >= P.g.mx - ANG || j >= P.g.my - ANG)
+        for (int m = 0; m < 5; ++m) v[m] = v[m] + fold_tile(P, xg, tx0, ty0, i, j, m);
+    if (rec >= 0) v[0] += P.rec_tr[rec] - P.rec_obs[rec];
+    for (int m = 0; m < 3; ++m) P.lam_out[m * P.g.plane + off] = v[m];
+    P.gaux[off] = v[3];
+    P.gaux[P.g.plane + off] = v[4];
+}
+
+// Wave strengths and 1/D of every face inside the staged box (faces need both
+// cells in the box, so the box's first column has no x-face, its first row no
+// y-face), computed once per tile from the staged cells.
+template <int NT>
+__host__ __device__ __forceinline__ void tile_faces(const TileAcc &A, int t0) {
+    // thread t0 of NT handles box cells t0, t0 + NT, ...; all its faces are
+    // computed before any is stored
+    constexpr int NPER = (BOX + NT - 1) / NT;
+    double *f = const_cast<double *>(A.f);
+    double v[NPER][6];
+#pragma unroll
+    for (int r = 0; r < NPER; ++r) {
+        const int k = t0 + r * NT;
+        if (k >= BOX) break;
+        const int bi = k % BOXW, bj = k / BOXW;
+        const int i = A.bx + bi, j = A.by + bj;
+        if (bi > 0)
+            strengths<0>(getq<0>(A, i - 1, j), getq<0>(A, i, j), geta<0>(A, i - 1, j), geta<0>(A, i, j), v[r][0],
+                         v[r][1], v[r][2]);
+        if (bj > 0)
+            strengths<1>(getq<1>(A, j - 1, i), getq<1>(A, j, i), geta<1>(A, j - 1, i), geta<1>(A, j, i), v[r][3],
+                         v[r][4], v[r][5]);
+    }
+#pragma unroll
+    for (int r = 0; r < NPER; ++r) {
+        const int k = t0 + r * NT;
+        if (k >= BOX) break;
+        if (k % BOXW > 0) { f[k] = v[r][0]; f[BOX + k] = v[r][1]; f[2 * BOX + k] = v[r][2]; }
+        if (k / BOXW > 0) { f[3 * BOX + k] = v[r][3]; f[4 * BOX + k] = v[r][4]; f[5 * BOX + k] = v[r][5]; }
+    }
+}
+
+// Stages q (boundary map applied), aux and lam for the box whose first cell is
+// (bx, by): thread t0 of NT handles box cells t0, t0 + NT, ...; all its loads
+// are issued before any LDS store, so their latencies overlap.
+template <int NT>
+__host__ __device__ __forceinline__ void stage_box(const Params &P, double *tile, int bx, int by, int t0) {
+    constexpr int NPER = (BOX + NT - 1) / NT;
+    double v[NPER][8];
+#pragma unroll
+    for (int r = 0; r < NPER; ++r) {
+        const int k = t0 + r * NT;
+        if (k >= BOX) break;
+        const int i = bx + k % BOXW, j = by + k / BOXW;
+        const SrcCell sq = bc_source(i, j, P.g, P.bq);
+        const double u = P.q[P.g.plane + sq.off], w = P.q[2 * P.g.plane + sq.off];
+        v[r][0] = P.q[sq.off];
+        v[r][1] = sq.fx ? -u : u;
+        v[r][2] = sq.fy ? -w : w;
+        const SrcCell sa = bc_source(i, j, P.g, P.ba);
+        v[r][3] = P.aux[sa.off];
+        v[r][4] = P.aux[P.g.plane + sa.off];
+        const bool in = i >= 0 && j >= 0 && i < P.g.mx && j < P.g.my;
+        const long long off = in ? (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng) : 0;
+        v[r][5] = in ? P.lam[off] : 0.0;
+        v[r][6] = in ? P.lam[P.g.plane + off] : 0.0;
+        v[r][7] = in ? P.lam[2 * P.g.plane + off] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < NPER; ++r) {
+        const int k = t0 + r * NT;
+        if (k >= BOX) break;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) tile[m * BOX + k] = v[r][m];
+    }
+}
+
+// One block per TW x TH tile of the padded region (tile_origin), a 1D grid
+// of tiles_x * tiles_y blocks taken in row-major tile order after the XCD
+// remap, so each XCD's L2 holds a band of tile rows and the halo rows that
+// neighbouring tiles share: stage q (boundary map applied), aux and lam for
+// the tile plus HALO cells around it in LDS, then one thread per owned
+// position.  FOLD (fold_in_tile holds): the
+// ghost positions' contributions go through LDS to the ring cells of the same
+// tile and the receivers' residuals are added here, so no fold launch follows;
+// otherwise ghost positions write the frame workspace for the fold launch.
+static_assert(5 * TW * TH <= 5 * BOX, "the ghosts' planes must not reach the staged lam planes");
+
+template <bool FOLD>
 __global__ __launch_bounds__(256) void adjoint_tile_kernel(const Params P) {
     __shared__ double tile[8 * BOX];
     __shared__ double faces[6 * BOX];
+    __shared__ int rslot[FOLD ? TW * TH : 1];
     TileAcc A;
     static_cast<Params &>(A) = P;
-    const int tx0 = blockIdx.x * TW - ANG, ty0 = blockIdx.y * TH - ANG;
+    const int ntx = tiles_x(P), nty = tiles_y(P);
+    const int vb = xcd_block(), bxi = vb % ntx, byi = vb / ntx;
+    const int tx0 = tile_origin(bxi, ntx, P.g.mx, TW), ty0 = tile_origin(byi, nty, P.g.my, TH);
+    const int txe = tile_end(bxi, ntx, P.g.mx, TW), tye = tile_end(byi, nty, P.g.my, TH);
     A.s = tile;
     A.f = faces;
     A.bx = tx0 - HALO;
     A.by = ty0 - HALO;
-    for (int k = threadIdx.x; k < BOX; k += 256) {
-        const int i = A.bx + k % BOXW, j = A.by + k / BOXW;
-        const SrcCell sq = bc_source(i, j, P.g, P.bq);
-        const double u = P.q[P.g.plane + sq.off], v = P.q[2 * P.g.plane + sq.off];
-        tile[k] = P.q[sq.off];
-        tile[BOX + k] = sq.fx ? -u : u;
-        tile[2 * BOX + k] = sq.fy ? -v : v;
-        const SrcCell sa = bc_source(i, j, P.g, P.ba);
-        tile[3 * BOX + k] = P.aux[sa.off];
-        tile[4 * BOX + k] = P.aux[P.g.plane + sa.off];
-        const bool in = i >= 0 && j >= 0 && i < P.g.mx && j < P.g.my;
-        const long long off = in ? (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng) : 0;
-        tile[5 * BOX + k] = in ? P.lam[off] : 0.0;
-        tile[6 * BOX + k] = in ? P.lam[P.g.plane + off] : 0.0;
-        tile[7 * BOX + k] = in ? P.lam[2 * P.g.plane + off] : 0.0;
-    }
+    if (FOLD) rslot[threadIdx.x] = -1;
+    stage_box<256>(P, tile, A.bx, A.by, threadIdx.x);
     __syncthreads();
-    tile_faces(A, threadIdx.x, 256);
+    if (FOLD) receiver_slots(P, tx0, ty0, txe, tye, rslot, threadIdx.x, 256);
+    tile_faces<256>(A, threadIdx.x);
     __syncthreads();
     const int i = tx0 + (threadIdx.x & (TW - 1)), j = ty0 + (threadIdx.x / TW);
-    if (i < P.g.mx + ANG && j < P.g.my + ANG) position_write(A, i, j);
+    const bool own = i < txe && j < tye;
+    if (!FOLD) {
+        if (own) position_write(A, i, j);
+        return;
+    }
+    const bool in = i >= 0 && j >= 0 && i < P.g.mx && j < P.g.my;
+    // the aux gradient's old values load while the position is computed
+    double ga0 = 0.0, ga1 = 0.0;
+    if (own && in) {
+        const long long off = (long long)(j + P.g.ng) * P.g.ldq + (i + P.g.ng);
+        ga0 = P.gaux[off];
+        ga1 = P.gaux[P.g.plane + off];
+    }
+    double gq[3] = {0.0, 0.0, 0.0}, grho = 0.0, gc = 0.0;
+    if (own) position_raw(A, i, j, gq, grho, gc);
+    // Only tiles in the first or last row or column hold ghost positions and
+    // ring cells (the first tile of an axis starts at -ANG, the last at or
+    // before n-ANG); the test is uniform over the block.
+    double *xg = tile;   // planes 0-1 of the staged box: q, read for the last time by position_raw
+    if (bxi == 0 || bxi == ntx - 1 || byi == 0 || byi == nty - 1) {
+        __syncthreads();
+        if (own && !in) {
+            xg[threadIdx.x] = gq[0];
+            xg[TW * TH + threadIdx.x] = gq[1];
+            xg[2 * TW * TH + threadIdx.x] = gq[2];
+            xg[3 * TW * TH + threadIdx.x] = grho;
+            xg[4 * TW * TH + threadIdx.x] = gc;
+        }
+        __syncthreads();
+    }
+    if (own && in) {
+        const int k = A.at(i, j);
+        const double lam[3] = {tile[5 * BOX + k], tile[6 * BOX + k], tile[7 * BOX + k]};
+        position_finish(P, xg, tx0, ty0, i, j, lam, ga0, ga1, gq, grho, gc, rslot[threadIdx.x]);
+    }
 }
 
 // Thread t < 5 * nring folds field m = t / nring of ring cell t % nring (lam
@@ -773,10 +951,15 @@ static int vc_adjoint_launch(const ClawGrid *grid, const ClawMethod *meth, const
     P.mth[0] = meth->mthlim[0]; P.mth[1] = meth->mthlim[1];
     P.rec_off = rec_off; P.rec_tr = rec_tr; P.rec_obs = rec_obs; P.nrec = nrec;
     hipStream_t st = (hipStream_t)stream;
-    const dim3 tiles((grid->mx + 2 * adj::ANG + adj::TW - 1) / adj::TW, (grid->my + 2 * adj::ANG + adj::TH - 1) / adj::TH);
-    hipLaunchKernelGGL(adj::adjoint_tile_kernel, tiles, dim3(256), 0, st, P);
-    const long long nfold = 5 * adj::ring_count(P) + nrec;
-    hipLaunchKernelGGL(adj::adjoint_fold_kernel, dim3((unsigned)((nfold + 127) / 128)), dim3(128), 0, st, P);
+    const int ntx = adj::tiles_x(P), nty = adj::tiles_y(P);
+    const dim3 tiles((unsigned)(ntx * nty));
+    if (adj::fold_in_tile(P, ntx, nty)) {
+        hipLaunchKernelGGL(adj::adjoint_tile_kernel<true>, tiles, dim3(256), 0, st, P);
+    } else {
+        hipLaunchKernelGGL(adj::adjoint_tile_kernel<false>, tiles, dim3(256), 0, st, P);
+        const long long nfold = 5 * adj::ring_count(P) + nrec;
+        hipLaunchKernelGGL(adj::adjoint_fold_kernel, dim3((unsigned)((nfold + 127) / 128)), dim3(128), 0, st, P);
+    }
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 

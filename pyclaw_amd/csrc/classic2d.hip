@@ -76,13 +76,6 @@ struct ClassicParams {
     int wpb;        // wavefronts per block
 };
 
-// Virtual block index: consecutive workgroups are dispatched round-robin over
-// the 8 XCDs; remap so each XCD (own L2) gets a contiguous range of strips.
-__device__ __forceinline__ int xcd_block() {
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int per = nb / 8, rem = nb % 8, xcd = b % 8, loc = b / 8;
-    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
-}
 
 // Row state carried from one y-interface to the next (the previous
 // interface's A+dq, correction flux, transverse splits and its non-zero wave

@@ -40,6 +40,15 @@ template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
 template <typename T> __device__ __forceinline__ T lane_up(T v) { return dpp_move<DPP_WAVE_ROR1>(v); }
 template <typename T> __device__ __forceinline__ T lane_dn(T v) { return dpp_move<DPP_WAVE_ROL1>(v); }
 
+// Virtual block index of a 1D grid: consecutive workgroups are dispatched
+// round-robin over the 8 XCDs; remap so each XCD (own L2) gets a contiguous
+// range of blocks.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8, xcd = b % 8, loc = b / 8;
+    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
 // ---- grid description on the device ---------------------------------------
 struct GridDev {
     int mx, my, ng;

@@ -128,12 +128,6 @@ struct SharpParams {
     int H, nsx, nsy, nsyb, wpb;
 };
 
-__device__ __forceinline__ int xcd_block_s() {
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int per = nb / 8, rem = nb % 8, xcd = b % 8, loc = b / 8;
-    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
-}
-
 template <class RP, typename R>
 __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     constexpr int NE = RP::NE, NW = RP::NW;
@@ -142,7 +136,7 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     using Cache = typename RP::Cache;
 
     const int lane = threadIdx.x & (WAVE - 1);
-    const int vb = xcd_block_s();
+    const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
     if (sx >= p.nsx || sy >= p.nsy) return;

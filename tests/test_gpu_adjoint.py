@@ -201,6 +201,46 @@ def test_adjoint_step_is_finite_and_scale_invariant_at_subnormal_wave_norms(scal
         assert np.abs(a2 / scale - a1).max() <= 1e-10 * np.abs(a1).max()
 
 
+@pytest.mark.parametrize("bc", [((1, 1), (1, 1)), ((3, 3), (3, 3)), ((3, 1), (1, 3)), ((2, 2), (2, 2))])
+@pytest.mark.parametrize("shape", [(63, 37), (9, 6)])
+def test_misfit_gradient_with_receivers_on_the_boundary_ring(bc, shape):
+    """Receivers in the corners and in the two cells next to every side, where
+    the ghosts' contributions are folded back into lam: inside the tile for
+    extrapolation and wall sides (63 x 37 also moves the last tile of both
+    axes back to hold the high ghosts), through the fold launch for periodic
+    ones."""
+    from pyclaw_amd.adjoint import MisfitProblem
+    from pyclaw_amd.problems import adjoint_medium
+    mx, my = shape
+    N, every = 6, 4
+    bcl, bcu = bc
+    print("ring receivers", bc, shape, flush=True)
+    rng = np.random.default_rng(11)
+    q0, K, rho, K_true = adjoint_medium(rng, mx, my, width=0.2)
+    dx, dy = 1.0 / mx, 1.0 / my
+    dt = 0.8 * min(dx, dy) / np.sqrt(K / rho).max()
+    rec = sorted({(i, j) for i in (0, 1, mx // 2, mx - 2, mx - 1) for j in (0, 1, my // 2, my - 2, my - 1)})
+    ix = torch.tensor([r[0] for r in rec])
+    iy = torch.tensor([r[1] for r in rec])
+    kw = dict(order=2, trans=2, mthlim=(4, 4), bc_lower=bcl, bc_upper=bcu, aux_bc_lower=bcl, aux_bc_upper=bcu)
+    with torch.no_grad():
+        _, obs = A.misfit(torch.from_numpy(q0), torch.from_numpy(rho), torch.from_numpy(np.sqrt(K_true / rho)), N,
+                          dx, dy, dt, (ix, iy), torch.zeros(N, len(rec), dtype=torch.float64), **kw)
+    q0t = torch.from_numpy(q0).requires_grad_(True)
+    Kt = torch.from_numpy(K).requires_grad_(True)
+    rt = torch.from_numpy(rho).requires_grad_(True)
+    Jref, _ = A.misfit(q0t, rt, torch.sqrt(Kt / rt), N, dx, dy, dt, (ix, iy), obs, **kw)
+    refs = torch.autograd.grad(Jref, (q0t, Kt, rt))
+    P = MisfitProblem(mx, my, dx, dy, dt, N, rec, p_obs=obs.numpy(), order=2, transverse=2, limiters=4,
+                      bc_lower=bcl, bc_upper=bcu, aux_bc_lower=bcl, aux_bc_upper=bcu, every=every)
+    J, gq, gK, gr = P.gradient(q0, K, rho)
+    assert abs(float(J) - float(Jref)) <= 1e-12 * abs(float(Jref))
+    for got, ref in zip((gq, gK, gr), refs):
+        got = got.cpu().numpy()
+        ref = ref.numpy()
+        assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("every", [1, 4, 17, 40])
 def test_checkpoint_interval_does_not_change_the_gradient(every):
     """Every checkpoint interval (1: all states kept, no recomputation; larger:
