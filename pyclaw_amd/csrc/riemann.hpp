@@ -224,14 +224,23 @@ template <typename R> struct Euler5 {
         wave[4][0] = R(0); wave[4][mu] = R(0); wave[4][mv] = R(0); wave[4][3] = R(0);
         wave[4][4] = hi[4] - lo[4]; s[4] = u;
 
+        // Sums that start at +0 never become -0, so a term s * w with a wave
+        // entry w that is structurally zero adds +-0 and changes nothing (for
+        // finite data): such terms are skipped (WNZ).
         if (!P.efix) {
+            // s < 0 goes to amdq, s >= 0 to apdq; min(s, 0) and max(s, 0) put
+            // an exact no-op term in the other sum
+            R sm[NW], sp[NW];
+#pragma unroll
+            for (int mw = 0; mw < NW; ++mw) { sm[mw] = rmin<R>(s[mw], R(0)); sp[mw] = rmax<R>(s[mw], R(0)); }
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
                 R am = R(0), ap = R(0);
 #pragma unroll
                 for (int mw = 0; mw < NW; ++mw) {
-                    if (s[mw] < R(0)) am = am + s[mw] * wave[mw][m];
-                    else ap = ap + s[mw] * wave[mw][m];
+                    if (!WNZ<DIR>(m, mw)) continue;
+                    am = am + sm[mw] * wave[mw][m];
+                    ap = ap + sp[mw] * wave[mw][m];
                 }
                 amdq[m] = am; apdq[m] = ap;
             }
@@ -276,11 +285,13 @@ template <typename R> struct Euler5 {
 #pragma unroll
             for (int m = 0; m < NE; ++m) amdq[m] = sfract * wave[0][m];
             if (s[1] >= R(0)) break;
+            // here s[1] = s[2] = s[4] = u < 0: a structurally zero entry adds -0,
+            // which changes no value (not even -0 or +0)
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
-                amdq[m] = amdq[m] + s[1] * wave[1][m];
-                amdq[m] = amdq[m] + s[2] * wave[2][m];
-                amdq[m] = amdq[m] + s[4] * wave[4][m];
+                if (WNZ<DIR>(m, 1)) amdq[m] = amdq[m] + s[1] * wave[1][m];
+                if (WNZ<DIR>(m, 2)) amdq[m] = amdq[m] + s[2] * wave[2][m];
+                if (WNZ<DIR>(m, 4)) amdq[m] = amdq[m] + s[4] * wave[4][m];
             }
             R s3, s2;
             R rho2 = hi[0] - wave[3][0];
@@ -311,7 +322,8 @@ template <typename R> struct Euler5 {
         for (int m = 0; m < NE; ++m) {
             R df = R(0);
 #pragma unroll
-            for (int mw = 0; mw < NW; ++mw) df = df + s[mw] * wave[mw][m];
+            for (int mw = 0; mw < NW; ++mw)
+                if (WNZ<DIR>(m, mw)) df = df + s[mw] * wave[mw][m];
             apdq[m] = df - amdq[m];
         }
     }
@@ -342,11 +354,14 @@ template <typename R> struct Euler5 {
         wb[2][4] = R(0); sb[2] = v + a;
         wb[3][0] = R(0); wb[3][mu] = R(0); wb[3][mv] = R(0); wb[3][3] = R(0); wb[3][4] = asdq[4];
         sb[3] = v;
+        // waves 0-2 have components 0-3, wave 3 (tracer) component 4 only; the
+        // sums start at +0, so the structurally zero terms are skipped
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             R b_m = R(0), b_p = R(0);
 #pragma unroll
             for (int mw = 0; mw < 4; ++mw) {
+                if ((mw == 3) != (m == 4)) continue;
                 b_m = b_m + rmin<R>(sb[mw], R(0)) * wb[mw][m];
                 b_p = b_p + rmax<R>(sb[mw], R(0)) * wb[mw][m];
             }
@@ -423,8 +438,10 @@ template <typename R> struct ShallowRoe {
 #pragma unroll
             for (int m = 0; m < NE; ++m) amdq[m] = sfract * wave[0][m];
             if (s[1] >= R(0)) break;
+            // s[1] = u < 0: a structurally zero entry adds -0, which changes no value
 #pragma unroll
-            for (int m = 0; m < NE; ++m) amdq[m] = amdq[m] + s[1] * wave[1][m];
+            for (int m = 0; m < NE; ++m)
+                if (WNZ<DIR>(m, 1)) amdq[m] = amdq[m] + s[1] * wave[1][m];
             R s03 = hi[mu] / hr + rsqrt_exact<R>(grav * hr);
             R h3 = hr - wave[2][0];
             R hu3 = hi[mu] - wave[2][mu];
@@ -439,7 +456,8 @@ template <typename R> struct ShallowRoe {
         for (int m = 0; m < NE; ++m) {
             R df = R(0);
 #pragma unroll
-            for (int mw = 0; mw < NW; ++mw) df = df + s[mw] * wave[mw][m];
+            for (int mw = 0; mw < NW; ++mw)
+                if (WNZ<DIR>(m, mw)) df = df + s[mw] * wave[mw][m];
             apdq[m] = df - amdq[m];
         }
     }
@@ -456,11 +474,14 @@ template <typename R> struct ShallowRoe {
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
         wb[1][0] = R(0); wb[1][mu] = a2; wb[1][mv] = R(0); sb[1] = v;
         wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
+        // wave 1 has component mu only; the sums start at +0, so the
+        // structurally zero terms are skipped
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             R b_m = R(0), b_p = R(0);
 #pragma unroll
             for (int mw = 0; mw < 3; ++mw) {
+                if (mw == 1 && m != mu) continue;
                 b_m = b_m + rmin<R>(sb[mw], R(0)) * wb[mw][m];
                 b_p = b_p + rmax<R>(sb[mw], R(0)) * wb[mw][m];
             }
