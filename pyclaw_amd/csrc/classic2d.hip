@@ -130,6 +130,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     const int order = p.order, trans = p.trans;
     const auto &P = p.P;
 
+    // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
+    // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
     R cfl = R(0);
     R qprev[NE];
     R a_m2[NA], a_m1[NA], a_0[NA];
@@ -186,7 +188,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             RP::template rpn<0>(P, lo, qprev, alo, a_m1, wave, s, amdq, apdq, cx);
             if (lane >= 1 && row >= -1 && row <= g.my && i >= 0 && i <= g.mx) {
 #pragma unroll
-                for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdx * s[mw]); cfl = rmax2<R>(cfl, -dtdx * s[mw]); }
+                for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
             }
 #pragma unroll
             for (int m = 0; m < NE; ++m) { fadd[m] = R(0); cq[m] = R(0); }
@@ -258,7 +260,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         RP::template rpn<1>(P, qprev, qc, a_m1, a_0, wN, sN, amN, apN, cN);
         if (L >= 0 && L <= g.my && i >= -1 && i <= g.mx) {
 #pragma unroll
-            for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdy * sN[mw]); cfl = rmax2<R>(cfl, -dtdy * sN[mw]); }
+            for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdy * rabs(sN[mw]));
         }
         R wY[NW][NE];
 #pragma unroll
@@ -466,6 +468,8 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
     const R dtdx = R(dtd / p.dx);
     const R avx = R(0.5) * (dtdx + dtdx);
     const auto &P = p.P;
+    // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
+    // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
     R cfl = R(0);
     for (int row = R0; row < R1; ++row) {
         R hi[NE], lo[NE], ahi[NA], alo[NA];
@@ -486,7 +490,7 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
         RP::template rpn<0>(P, lo, hi, alo, ahi, wave, s, amdq, apdq, cx);
         if (i >= 0 && i <= g.mx) {
 #pragma unroll
-            for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdx * s[mw]); cfl = rmax2<R>(cfl, -dtdx * s[mw]); }
+            for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
         }
 #pragma unroll
         for (int m = 0; m < NE; ++m) fadd[m] = R(0);
@@ -559,6 +563,8 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
     const R dtdy = R(dtd / p.dy);
     const R avy = R(0.5) * (dtdy + dtdy);
     const auto &P = p.P;
+    // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
+    // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
     R cfl = R(0);
 
     auto loadq = [&](int row, R (&out)[NE]) {
@@ -599,7 +605,7 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
         RP::template rpn<1>(P, qm1, qc, aprev, a0, wN, sN, amN, apN, cN);
         if (L >= 0 && L <= g.my && valid_col) {
 #pragma unroll
-            for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdy * sN[mw]); cfl = rmax2<R>(cfl, -dtdy * sN[mw]); }
+            for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdy * rabs(sN[mw]));
         }
 #pragma unroll
         for (int mw = 0; mw < NW; ++mw) {

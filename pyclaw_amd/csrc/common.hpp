@@ -24,9 +24,11 @@ __device__ __forceinline__ double lane_get(double v, int addr) {
 // lane_up: lane L receives lane L-1's value (lane 0 receives lane 63's);
 // lane_dn: lane L receives lane L+1's value (lane 63 receives lane 0's).
 // The same values as lane_get with addresses ((lane -/+ 1) & 63) * 4.
+// A whole-wavefront rotate gives every lane a source, so no lane keeps the
+// destination's old value: the move takes none (no zeroing of the destination).
 constexpr int DPP_WAVE_ROL1 = 0x134, DPP_WAVE_ROR1 = 0x13C;
 template <int CTRL> __device__ __forceinline__ int dpp_move(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
 template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
     return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
@@ -127,19 +129,39 @@ __device__ __forceinline__ void load_aux(const R *__restrict__ a, const GridDev 
 __device__ __forceinline__ double rabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float rabs(float x) { return __builtin_fabsf(x); }
 
+// IEEE maximum/minimum (one v_max/v_min instruction; a NaN operand yields the
+// other).  Where a select a > b ? a : b differs from it only in the sign of a
+// zero result, and that zero is only ever added to a sum that starts at +0
+// (so the sign changes nothing), these are exact replacements.
+__device__ __forceinline__ double fmax_ieee(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float fmax_ieee(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double fmin_ieee(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float fmin_ieee(float a, float b) { return __builtin_fminf(a, b); }
+
 // ---- wave limiter (classic/philim.f90:4-57) ---------------------------------
 template <typename R> __device__ __forceinline__ R rmax2(R a, R b) { return a > b ? a : b; }
 template <typename R> __device__ __forceinline__ R rmin2(R a, R b) { return a < b ? a : b; }
+// The limiter's max/min: selects (the reference's results for every input,
+// NaN included) by default; the fused-multiply-add build, which does not
+// promise bitwise equality, uses IEEE max/min (one instruction each; the same
+// value for finite ratios up to the sign of a zero phi).
+#ifdef CLAW_SHARED_RECIPROCALS
+template <typename R> __device__ __forceinline__ R lmax(R a, R b) { return fmax_ieee(a, b); }
+template <typename R> __device__ __forceinline__ R lmin(R a, R b) { return fmin_ieee(a, b); }
+#else
+template <typename R> __device__ __forceinline__ R lmax(R a, R b) { return rmax2<R>(a, b); }
+template <typename R> __device__ __forceinline__ R lmin(R a, R b) { return rmin2<R>(a, b); }
+#endif
 
 template <typename R>
 __device__ __forceinline__ R philim(R a, R b, int meth) {
     R r = b / a;
     switch (meth) {
-    case 2: return rmax2<R>(rmax2<R>(R(0), rmin2<R>(R(1), R(2) * r)), rmin2<R>(R(2), r));
+    case 2: return lmax<R>(lmax<R>(R(0), lmin<R>(R(1), R(2) * r)), lmin<R>(R(2), r));
     case 3: { R ar = rabs(r); return (r + ar) / (R(1) + ar); }
-    case 4: { R c = (R(1) + r) / R(2); return rmax2<R>(R(0), rmin2<R>(rmin2<R>(c, R(2)), R(2) * r)); }
+    case 4: { R c = (R(1) + r) / R(2); return lmax<R>(R(0), lmin<R>(lmin<R>(c, R(2)), R(2) * r)); }
     case 5: return r;
-    default: return rmax2<R>(R(0), rmin2<R>(R(1), r));
+    default: return lmax<R>(R(0), lmin<R>(R(1), r));
     }
 }
 

@@ -229,10 +229,10 @@ template <typename R> struct Euler5 {
         // finite data): such terms are skipped (WNZ).
         if (!P.efix) {
             // s < 0 goes to amdq, s >= 0 to apdq; min(s, 0) and max(s, 0) put
-            // an exact no-op term in the other sum
+            // an exact no-op term (+-0) in the other sum
             R sm[NW], sp[NW];
 #pragma unroll
-            for (int mw = 0; mw < NW; ++mw) { sm[mw] = rmin<R>(s[mw], R(0)); sp[mw] = rmax<R>(s[mw], R(0)); }
+            for (int mw = 0; mw < NW; ++mw) { sm[mw] = fmin_ieee(s[mw], R(0)); sp[mw] = fmax_ieee(s[mw], R(0)); }
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
                 R am = R(0), ap = R(0);
@@ -355,15 +355,16 @@ template <typename R> struct Euler5 {
         wb[3][0] = R(0); wb[3][mu] = R(0); wb[3][mv] = R(0); wb[3][3] = R(0); wb[3][4] = asdq[4];
         sb[3] = v;
         // waves 0-2 have components 0-3, wave 3 (tracer) component 4 only; the
-        // sums start at +0, so the structurally zero terms are skipped
+        // sums start at +0, so the structurally zero terms are skipped, and
+        // the sign of a zero min(s, 0) or max(s, 0) changes nothing
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             R b_m = R(0), b_p = R(0);
 #pragma unroll
             for (int mw = 0; mw < 4; ++mw) {
                 if ((mw == 3) != (m == 4)) continue;
-                b_m = b_m + rmin<R>(sb[mw], R(0)) * wb[mw][m];
-                b_p = b_p + rmax<R>(sb[mw], R(0)) * wb[mw][m];
+                b_m = b_m + fmin_ieee(sb[mw], R(0)) * wb[mw][m];
+                b_p = b_p + fmax_ieee(sb[mw], R(0)) * wb[mw][m];
             }
             bm[m] = b_m; bp[m] = b_p;
         }
@@ -475,15 +476,16 @@ template <typename R> struct ShallowRoe {
         wb[1][0] = R(0); wb[1][mu] = a2; wb[1][mv] = R(0); sb[1] = v;
         wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
         // wave 1 has component mu only; the sums start at +0, so the
-        // structurally zero terms are skipped
+        // structurally zero terms are skipped, and the sign of a zero
+        // min(s, 0) or max(s, 0) changes nothing
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             R b_m = R(0), b_p = R(0);
 #pragma unroll
             for (int mw = 0; mw < 3; ++mw) {
                 if (mw == 1 && m != mu) continue;
-                b_m = b_m + rmin<R>(sb[mw], R(0)) * wb[mw][m];
-                b_p = b_p + rmax<R>(sb[mw], R(0)) * wb[mw][m];
+                b_m = b_m + fmin_ieee(sb[mw], R(0)) * wb[mw][m];
+                b_p = b_p + fmax_ieee(sb[mw], R(0)) * wb[mw][m];
             }
             bm[m] = b_m; bp[m] = b_p;
         }

@@ -152,6 +152,8 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     const R dtdx = R(dtd / p.dx), dtdy = R(dtd / p.dy);
     const auto &P = p.P;
     const int mode = p.mode;
+    // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
+    // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
     R cfl = R(0);
 
     // five-row window of q (rows w-2..w+2) and aux (rows w-1, w)
@@ -195,7 +197,7 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
             RP::template rpn<1>(P, qrY_prev, qlY, aw[1], aw[2], wave, s, amY, apY, c);
             if (w <= g.my && i >= -1 && i <= g.mx) {
 #pragma unroll
-                for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdy * s[mw]); cfl = rmax2<R>(cfl, -dtdy * s[mw]); }
+                for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdy * rabs(s[mw]));
             }
             RP::template rpn<1>(P, qlY, qrY, aw[2], aw[2], wave, s, am2Y, ap2Y, c);
         }
@@ -227,7 +229,7 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
                 RP::template rpn<0>(P, qrXl, qlX, alx, aw[1], wave, s, amX, apX, c);
                 if (i >= 0 && i <= g.mx && lane >= 3 && lane <= WAVE - 3) {
 #pragma unroll
-                    for (int mw = 0; mw < NW; ++mw) { cfl = rmax2<R>(cfl, dtdx * s[mw]); cfl = rmax2<R>(cfl, -dtdx * s[mw]); }
+                    for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
                 }
                 RP::template rpn<0>(P, qlX, qrX, aw[1], aw[1], wave, s, am2X, ap2X, c);
             }
