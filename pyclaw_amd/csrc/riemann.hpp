@@ -422,9 +422,10 @@ template <typename R> struct ShallowRoe {
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
         wave[1][0] = R(0); wave[1][mu] = R(0); wave[1][mv] = a2; s[1] = u;
         wave[2][0] = a3; wave[2][mu] = a3 * (u + a); wave[2][mv] = a3 * v; s[2] = u + a;
+        // depths are positive: the shared-reciprocal build multiplies by 1/h
         do {
             R sfract;
-            R s0 = lo[mu] / hl - rsqrt_exact<R>(grav * hl);
+            R s0 = (SHARED_RCP ? lo[mu] * rcp<R>(hl) : lo[mu] / hl) - rsqrt_exact<R>(grav * hl);
             if (s0 >= R(0) && s[0] > R(0)) {
 #pragma unroll
                 for (int m = 0; m < NE; ++m) amdq[m] = R(0);
@@ -432,7 +433,7 @@ template <typename R> struct ShallowRoe {
             }
             R h1 = hl + wave[0][0];
             R hu1 = lo[mu] + wave[0][mu];
-            R s1 = hu1 / h1 - rsqrt_exact<R>(grav * h1);
+            R s1 = (SHARED_RCP ? hu1 * rcp<R>(h1) : hu1 / h1) - rsqrt_exact<R>(grav * h1);
             if (s0 < R(0) && s1 > R(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
             else if (s[0] < R(0)) sfract = s[0];
             else sfract = R(0);
@@ -443,10 +444,10 @@ template <typename R> struct ShallowRoe {
 #pragma unroll
             for (int m = 0; m < NE; ++m)
                 if (WNZ<DIR>(m, 1)) amdq[m] = amdq[m] + s[1] * wave[1][m];
-            R s03 = hi[mu] / hr + rsqrt_exact<R>(grav * hr);
+            R s03 = (SHARED_RCP ? hi[mu] * rcp<R>(hr) : hi[mu] / hr) + rsqrt_exact<R>(grav * hr);
             R h3 = hr - wave[2][0];
             R hu3 = hi[mu] - wave[2][mu];
-            R s3 = hu3 / h3 + rsqrt_exact<R>(grav * h3);
+            R s3 = (SHARED_RCP ? hu3 * rcp<R>(h3) : hu3 / h3) + rsqrt_exact<R>(grav * h3);
             if (s3 < R(0) && s03 > R(0)) sfract = s3 * (s03 - s[2]) / (s03 - s3);
             else if (s[2] < R(0)) sfract = s[2];
             else break;
@@ -468,9 +469,10 @@ template <typename R> struct ShallowRoe {
         constexpr int mu = MU<DIR>, mv = MV<DIR>;
         const R u = C.u, v = C.v, a = C.a;
         R q0 = asdq[0], qu = asdq[mu], qv = asdq[mv];
-        R a1 = (R(0.5) / a) * ((v + a) * q0 - qv);
+        const R h2a = SHARED_RCP ? R(0.5) * rcp<R>(a) : R(0.5) / a;
+        R a1 = h2a * ((v + a) * q0 - qv);
         R a2 = qu - u * q0;
-        R a3 = (R(0.5) / a) * (-(v - a) * q0 + qv);
+        R a3 = h2a * (-(v - a) * q0 + qv);
         R wb[3][NE], sb[3];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
         wb[1][0] = R(0); wb[1][mu] = a2; wb[1][mv] = R(0); sb[1] = v;

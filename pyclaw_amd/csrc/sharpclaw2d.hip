@@ -87,8 +87,10 @@ __device__ __forceinline__ void weno5(R qm2, R qm1, R q0, R qp1, R qp2, R &ql, R
     R t0 = eps + sigma0, t1 = eps + sigma1, t2 = eps + sigma2;
     R t00 = t0 * t0, t11 = t1 * t1, t22 = t2 * t2;
     R omega0, omega1, omega2, omega3, omega4, omega5;
-    if constexpr (SHARED_RCP) {   // one division per smoothness indicator
-        const R i0 = R(1) / t00, i1 = R(1) / t11, i2 = R(1) / t22;
+    if constexpr (SHARED_RCP) {
+        // one reciprocal per smoothness indicator, by the hardware reciprocal
+        // and two Newton steps: (eps + sigma)^2 >= 1e-72 is positive and normal
+        const R i0 = rcp<R>(t00), i1 = rcp<R>(t11), i2 = rcp<R>(t22);
         omega0 = R(0.1) * i0; omega1 = R(0.6) * i1; omega2 = R(0.3) * i2;
         omega3 = R(0.3) * i0; omega4 = R(0.6) * i1; omega5 = R(0.1) * i2;
     } else {
@@ -107,8 +109,13 @@ __device__ __forceinline__ void weno5(R qm2, R qm1, R q0, R qp1, R qp2, R &ql, R
         - R(0.16666666666666666666666666666666667) * qm1;
     R fr5 = R(1.8333333333333333333333333333333333) * q0 - R(1.1666666666666666666666666666666667) * qm1
         + R(0.33333333333333333333333333333333333) * qm2;
-    ql = (fr0 * omega0 + fr1 * omega1 + fr2 * omega2) / (omega0 + omega1 + omega2);
-    qr = (fr3 * omega3 + fr4 * omega4 + fr5 * omega5) / (omega3 + omega4 + omega5);
+    if constexpr (SHARED_RCP) {   // the weight sums are positive and normal as well
+        ql = (fr0 * omega0 + fr1 * omega1 + fr2 * omega2) * rcp<R>(omega0 + omega1 + omega2);
+        qr = (fr3 * omega3 + fr4 * omega4 + fr5 * omega5) * rcp<R>(omega3 + omega4 + omega5);
+    } else {
+        ql = (fr0 * omega0 + fr1 * omega1 + fr2 * omega2) / (omega0 + omega1 + omega2);
+        qr = (fr3 * omega3 + fr4 * omega4 + fr5 * omega5) / (omega3 + omega4 + omega5);
+    }
 }
 
 template <class RP, typename R>
