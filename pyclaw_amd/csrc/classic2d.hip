@@ -105,7 +105,8 @@ struct CarrySlots {
 // wavefront, uniform arguments).  Used by the single-grid kernel below and by
 // the batched work-queue kernel.  cs: this wavefront's carry slots
 // (CarrySlots<RP>::COUNT x WAVE values in LDS).
-template <class RP, typename R>
+// O32: every component plane is smaller than 4 GiB (32-bit byte offsets within a plane).
+template <class RP, typename R, bool O32 = false>
 __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy, R *__restrict__ cs) {
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
@@ -123,10 +124,11 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     // of a produced cell's interface), lane 1's is
     const bool produce = lane >= 2 && lane <= XU_USEFUL + 1 && i < g.mx;
 
+    // uniform step constants in scalar registers (the whole wavefront is active here)
     const double dtd = *p.dt;
-    const R dtdx = R(dtd / p.dx), dtdy = R(dtd / p.dy);
-    const R hx = R(0.5) * dtdx, hy = R(0.5) * dtdy;
-    const R avx = R(0.5) * (dtdx + dtdx), avy = R(0.5) * (dtdy + dtdy);
+    const R dtdx = uniform(R(dtd / p.dx)), dtdy = uniform(R(dtd / p.dy));
+    const R hx = uniform(R(0.5) * dtdx), hy = uniform(R(0.5) * dtdy);
+    const R avx = uniform(R(0.5) * (dtdx + dtdx)), avy = uniform(R(0.5) * (dtdy + dtdy));
     const int order = p.order, trans = p.trans;
     const auto &P = p.P;
 
@@ -158,17 +160,17 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 #pragma unroll
     for (int a = 0; a < NA; ++a) { a_m2[a] = a_m1[a] = a_0[a] = R(1); }
 
-    load_q<NE>(p.qold, g, p.bq, i, R0 - 2, qprev);
+    load_q<NE, R, O32>(p.qold, g, p.bq, i, R0 - 2, qprev);
     if constexpr (HAS_AUX) {
-        load_aux<NA>(p.aux, g, p.ba, i, R0 - 2, a_m1);
+        load_aux<NA, R, O32>(p.aux, g, p.ba, i, R0 - 2, a_m1);
     }
 
     for (int L = R0 - 1; L <= R1 + 1; ++L) {
         // ---- 1. load the lead row L ----------------------------------------
         R qc[NE];
-        load_q<NE>(p.qold, g, p.bq, i, L, qc);
+        load_q<NE, R, O32>(p.qold, g, p.bq, i, L, qc);
         if constexpr (HAS_AUX) {
-            load_aux<NA>(p.aux, g, p.ba, i, L, a_0);
+            load_aux<NA, R, O32>(p.aux, g, p.ba, i, L, a_0);
         }
 
         // ---- 2. x-interface (i-1/2, L-1): solve, limit, transverse, x-part of cell (i, L-1)
@@ -351,7 +353,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     R v = accOld[m] + dtdx * g2l;
                     v = v + ((qadd - dtdy * (fY[m] - uF)) - dtdx * (g2 - g1));
                     v = v - dtdx * g1r;
-                    if (produce) p.qnew[m * g.plane + off] = v;
+                    if (produce) plane_at<O32>(p.qnew + m * g.plane, off) = v;
                 }
             }
 #pragma unroll
@@ -393,28 +395,28 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 // dynamic LDS (p.wpb x CarrySlots::COUNT x WAVE values).
 extern __shared__ __attribute__((aligned(16))) unsigned char classic_carry_lds[];
 
-template <class RP, typename R>
+template <class RP, typename R, bool O32>
 __device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
     const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
     if (sx >= p.nsx || sy >= p.nsy) return;
     R *carry = reinterpret_cast<R *>(classic_carry_lds);
-    unsplit_band<RP, R>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+    unsplit_band<RP, R, O32>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
 }
 
-template <class RP, typename R>
+template <class RP, typename R, bool O32>
 __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
-    unsplit_block<RP, R>(p);
+    unsplit_block<RP, R, O32>(p);
 }
 
 // The same kernel compiled for two wavefronts per SIMD: the register budget
 // halves (the FP64 Euler instantiation then spills to scratch), latency
 // hiding doubles.  The autotuner times both.
-template <class RP, typename R>
+template <class RP, typename R, bool O32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void classic_unsplit_kernel_w2(const ClassicParams<RP, R> p) {
-    unsplit_block<RP, R>(p);
+    unsplit_block<RP, R, O32>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -855,10 +857,16 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         p.nsyb = (p.nsy + wpb - 1) / wpb;
         int nblocks = p.nsx * p.nsyb;
         const size_t lds = (size_t)wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
-        if (w2)
-            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
+        // 32-bit offsets when every byte offset within a component plane fits
+        const bool o32 = (unsigned long long)grid->plane * sizeof(R) <= 0xffffffffull;
+        if (w2 && o32)
+            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, true>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
+        else if (w2)
+            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, false>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
+        else if (o32)
+            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, true>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
         else
-            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
+            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, false>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
         return check(hipGetLastError());
     }
     // split: x pass into scratch (padded layout, ghost rows included), then y pass

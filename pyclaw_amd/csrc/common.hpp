@@ -39,6 +39,21 @@ template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
     int hi = dpp_move<CTRL>((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// A value that is the same in every lane, moved to scalar registers.  FP64
+// arithmetic on uniform values runs in the vector unit, and the compiler then
+// keeps the results in vector registers; a loop-invariant uniform constant
+// costs two VGPRs per double unless it is moved.  Call with every lane of the
+// wavefront active (the first active lane's value is taken).
+__device__ __forceinline__ double uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 template <typename T> __device__ __forceinline__ T lane_up(T v) { return dpp_move<DPP_WAVE_ROR1>(v); }
 template <typename T> __device__ __forceinline__ T lane_dn(T v) { return dpp_move<DPP_WAVE_ROL1>(v); }
 
@@ -106,24 +121,44 @@ __host__ __device__ __forceinline__ SrcCell bc_source(int i, int j, const GridDe
     return r;
 }
 
-template <int NE, typename R>
+// Element e of the plane that starts at base.  O32 (every plane smaller than
+// 4 GiB): the byte offset is formed in 32 bits, so the access is a uniform
+// 64-bit base (the plane) plus one 32-bit per-lane offset, with no per-lane
+// 64-bit address.
+template <bool O32, typename R>
+__device__ __forceinline__ const R &plane_at(const R *base, long long e) {
+    if constexpr (O32)
+        return *reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) +
+                                            (unsigned)e * (unsigned)sizeof(R));
+    else
+        return base[e];
+}
+template <bool O32, typename R>
+__device__ __forceinline__ R &plane_at(R *base, long long e) {
+    if constexpr (O32)
+        return *reinterpret_cast<R *>(reinterpret_cast<char *>(base) + (unsigned)e * (unsigned)sizeof(R));
+    else
+        return base[e];
+}
+
+template <int NE, typename R, bool O32 = false>
 __device__ __forceinline__ void load_q(const R *__restrict__ q, const GridDev &g, const BCDev &bc,
                                        int i, int j, R (&out)[NE]) {
     SrcCell s = bc_source(i, j, g, bc);
 #pragma unroll
     for (int m = 0; m < NE; ++m) {
-        R v = q[m * g.plane + s.off];
+        R v = plane_at<O32>(q + m * g.plane, s.off);
         if ((m == 1 && s.fx) || (m == 2 && s.fy)) v = -v;
         out[m] = v;
     }
 }
 
-template <int NA, typename R>
+template <int NA, typename R, bool O32 = false>
 __device__ __forceinline__ void load_aux(const R *__restrict__ a, const GridDev &g, const BCDev &bc,
                                          int i, int j, R (&out)[NA]) {
     SrcCell s = bc_source(i, j, g, bc);
 #pragma unroll
-    for (int m = 0; m < NA; ++m) out[m] = a[m * g.plane + s.off];
+    for (int m = 0; m < NA; ++m) out[m] = plane_at<O32>(a + m * g.plane, s.off);
 }
 
 __device__ __forceinline__ double rabs(double x) { return __builtin_fabs(x); }
