@@ -93,6 +93,13 @@ typedef struct {
 int claw_ctx_create(int device, void **ctx);
 int claw_ctx_destroy(void *ctx);
 
+/* No counterpart in the reference.  The unsplit classic kernels address each
+   component plane through a 32-bit per-lane offset from the plane's base when
+   the plane is under 4 GiB, and through 64-bit offsets otherwise; on = 1 makes
+   this context use the 64-bit kernels for every grid (the results are the
+   same; for tests of that path on small grids).  on = 0 restores the default. */
+int claw_ctx_set_wide_offsets(void *ctx, int on);
+
 /* One classic step (unsplit or split) from qold into qnew (distinct buffers).
    Replaces ClawSolver2D.step_hyperbolic's step2/step2ds calls
    (src/pyclaw/classic/solver.py:498-547, step2.f90, step2ds.f90).

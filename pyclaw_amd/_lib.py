@@ -84,6 +84,7 @@ def lib():
 def _optional_symbols():
     vp = ctypes.c_void_p
     return {
+        "claw_ctx_set_wide_offsets": [vp, ctypes.c_int],
         "claw2d_sharpclaw_dq": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.POINTER(ClawBC),
                                 ctypes.c_int, vp, vp, vp, vp, vp, vp],
         "claw2d_sharpclaw_stage": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann),

@@ -858,7 +858,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         int nblocks = p.nsx * p.nsyb;
         const size_t lds = (size_t)wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
         // 32-bit offsets when every byte offset within a component plane fits
-        const bool o32 = (unsigned long long)grid->plane * sizeof(R) <= 0xffffffffull;
+        const bool o32 = !ctx->wide_offsets && (unsigned long long)grid->plane * sizeof(R) <= 0xffffffffull;
         if (w2 && o32)
             hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, true>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
         else if (w2)
@@ -1072,6 +1072,13 @@ int claw_ctx_create(int device, void **out) {
     if (!c) return CLAW_ENOMEM;
     c->device = device;
     *out = c;
+    return CLAW_OK;
+}
+
+int claw_ctx_set_wide_offsets(void *ctx, int on) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || (on != 0 && on != 1)) return CLAW_EINVAL;
+    c->wide_offsets = on != 0;
     return CLAW_OK;
 }
 

@@ -87,6 +87,29 @@ def test_unsplit_ragged_shapes(shape):
     assert cref == cgot
 
 
+@pytest.mark.parametrize("precision", ['f64', 'f64fma', 'f32'])
+@pytest.mark.parametrize("name", list(RP_CASES))
+def test_wide_offset_kernels_give_identical_results(name, precision):
+    """The unsplit kernels with 64-bit per-lane offsets (used for component
+    planes of 4 GiB and more) against the 32-bit-offset kernels every smaller
+    grid runs, bit for bit, and (FP64) against the oracle."""
+    print("case", name, precision, flush=True)
+    case = RP_CASES[name]
+    rng = np.random.default_rng(12)
+    q = case['make'](rng, 131, 47)
+    aux = vc_aux(rng, 131, 47) if case['naux'] else None
+    lo, hi = BC_SETS[4]
+    qbc = padded_with_bc(q, 2, lo, hi, True, ghost_rng=rng)
+    abc = padded_with_bc(aux, 2, (1, 3), (3, 1), False) if aux is not None else None
+    args = (qbc, abc, case, case['dt'], 2, 2, False, None, lo, hi, (1, 3), (3, 1))
+    narrow, c1 = gpu_classic_step(*args, precision=precision)
+    wide, c2 = gpu_classic_step(*args, precision=precision, wide_offsets=True)
+    assert np.array_equal(narrow, wide) and c1 == c2
+    if precision == 'f64':
+        ref, cref = oracle_classic_step(qbc, abc, case, case['dt'], 2, 2)
+        assert np.array_equal(ref, wide) and cref == c2
+
+
 def test_f32_error_vs_f64():
     """FP32 variant: relative L-inf error of one Euler step against the FP64 oracle."""
     case = RP_CASES['euler']
