@@ -466,9 +466,9 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
     const int R1 = min(R0 + p.H, g.my + CNG);
     const bool l0 = (lane == 0);
     const bool produce = lane >= 1 && lane <= XS_USEFUL && i < g.mx;
-    const double dtd = *p.dt;
-    const R dtdx = R(dtd / p.dx);
-    const R avx = R(0.5) * (dtdx + dtdx);
+    const double dtd = *p.dt;   // uniform constants in scalar registers (whole wavefront active)
+    const R dtdx = uniform(R(dtd / p.dx));
+    const R avx = uniform(R(0.5) * (dtdx + dtdx));
     const auto &P = p.P;
     // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
     // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
@@ -561,9 +561,9 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
     const bool valid_col = (i < g.mx + CNG);
     const int R0 = sy * p.H;
     const int R1 = min(R0 + p.H, g.my);
-    const double dtd = *p.dt;
-    const R dtdy = R(dtd / p.dy);
-    const R avy = R(0.5) * (dtdy + dtdy);
+    const double dtd = *p.dt;   // uniform constants in scalar registers (whole wavefront active)
+    const R dtdy = uniform(R(dtd / p.dy));
+    const R avy = uniform(R(0.5) * (dtdy + dtdy));
     const auto &P = p.P;
     // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
     // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave

@@ -155,8 +155,8 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     const int R1 = min(R0 + p.H, g.my);
     const bool produce = lane >= 3 && lane < 3 + SX_USEFUL && i < g.mx;
 
-    const double dtd = *p.dt;
-    const R dtdx = R(dtd / p.dx), dtdy = R(dtd / p.dy);
+    const double dtd = *p.dt;   // uniform constants in scalar registers (whole wavefront active)
+    const R dtdx = uniform(R(dtd / p.dx)), dtdy = uniform(R(dtd / p.dy));
     const auto &P = p.P;
     const int mode = p.mode;
     // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
