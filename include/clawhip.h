@@ -320,6 +320,25 @@ int claw2d_gather(const double *a, const long long *offsets, int n, double *out,
 int claw2d_misfit_source(double *a, const long long *offsets, int n, const double *traces,
                          const double *observed, void *stream);
 
+/* ---- Source terms as device operations --------------------------------------
+   Replace the host source routines of the Euler shock-bubble application,
+   apps/euler_2d/shockbubble.py:89-124 (step_Euler_radial, set as
+   solver.step_source, classic/solver.py:131-149) and :127-153
+   (dq_Euler_radial, set as solver.dq_src, sharpclaw/solver.py:279-280):
+   geometric source terms of the Euler equations with radial symmetry, radius
+   = aux component 0.  Interior cells only, q in the padded planar layout of
+   grid; num_eqn >= 4 (components 4.. are a passive tracer); gamma1 = gamma - 1.
+   FP64 results equal the reference's bit for bit. */
+
+/* One 2-stage Runge-Kutta step of length dt, in place on q. */
+int claw2d_euler_radial_step(const ClawGrid *grid, int precision, int num_eqn, void *q, const void *aux,
+                             double dt, double gamma1, void *stream);
+
+/* dq = dt * source(q) (components >= 4 zero); dq has q's layout and is a
+   different buffer. */
+int claw2d_euler_radial_dq(const ClawGrid *grid, int precision, int num_eqn, const void *q, const void *aux,
+                           void *dq, double dt, double gamma1, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ kernels (libclawhip.so, C ABI in include/clawhip.h).
 """
 from . import limiters
 from . import riemann
+from . import sources
 from .cfl import CFL
 from .classic import ClawSolver2D
 from .controller import Controller
@@ -21,4 +22,4 @@ except ImportError:  # pragma: no cover
     SharpClawSolver2D = None
 
 __all__ = ['Controller', 'Dimension', 'Patch', 'Domain', 'Grid', 'Solution', 'State', 'CFL',
-           'ClawSolver2D', 'SharpClawSolver2D', 'BC', 'CFLError', 'limiters', 'riemann']
+           'ClawSolver2D', 'SharpClawSolver2D', 'BC', 'CFLError', 'limiters', 'riemann', 'sources']

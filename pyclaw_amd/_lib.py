@@ -85,6 +85,10 @@ def _optional_symbols():
     vp = ctypes.c_void_p
     return {
         "claw_ctx_set_wide_offsets": [vp, ctypes.c_int],
+        "claw2d_euler_radial_step": [ctypes.POINTER(ClawGrid), ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_double,
+                                     ctypes.c_double, vp],
+        "claw2d_euler_radial_dq": [ctypes.POINTER(ClawGrid), ctypes.c_int, ctypes.c_int, vp, vp, vp,
+                                   ctypes.c_double, ctypes.c_double, vp],
         "claw2d_sharpclaw_dq": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.POINTER(ClawBC),
                                 ctypes.c_int, vp, vp, vp, vp, vp, vp],
         "claw2d_sharpclaw_stage": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann),

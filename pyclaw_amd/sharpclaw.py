@@ -272,9 +272,13 @@ class SharpClawSolver(Solver):
                 if c > self.cfl_max:
                     self.cfl.update_global_max(c)
                     return False
-                src = np.asarray(self.dq_src(self, sst, self.dt), dtype=np.float64)
+                src = self.dq_src(self, sst, self.dt)
                 dqi = state._q.interior(dqbuf)
-                dqi += torch.from_numpy(np.ascontiguousarray(src)).to(dqi.device, dqi.dtype)
+                if torch.is_tensor(src):   # a device source (pyclaw_amd.sources): no host copy
+                    dqi += src.to(dqi.device, dqi.dtype)
+                else:
+                    src = np.asarray(src, dtype=np.float64)
+                    dqi += torch.from_numpy(np.ascontiguousarray(src)).to(dqi.device, dqi.dtype)
                 self._combine(mode, state._q.interior(bufs[a]), state._q.interior(q), dqi,
                               state._q.interior(self._regs['s2']), state._q.interior(bufs[b]))
             self.cfl.update_global_max(c)
@@ -318,7 +322,8 @@ class SharpClawSolver(Solver):
             raise CFLError('cfl_max exceeded')
         out = state._q.interior(dqbuf).to('cpu', torch.float64).numpy().copy(order='F')
         if self.dq_src is not None:
-            out += self.dq_src(self, state, self.dt)
+            src = self.dq_src(self, state, self.dt)
+            out += src.to('cpu', torch.float64).numpy() if torch.is_tensor(src) else src
         return out
 
     def dqdt(self, state):

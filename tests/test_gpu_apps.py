@@ -20,7 +20,7 @@ RP = {'acoustics': 'rp2_acoustics', 'vc_acoustics': 'rp2_vc_acoustics', 'euler':
       'shallow': 'rp2_shallow_roe_with_efix'}
 
 
-def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False, fma=False):
+def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False, fma=False, dq_src=None):
     import pyclaw_amd as pc
     x = pc.Dimension('x', opts['lower'][0], opts['upper'][0], opts['mx'])
     y = pc.Dimension('y', opts['lower'][1], opts['upper'][1], opts['my'])
@@ -40,6 +40,8 @@ def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False,
             solver.step_source = step_source
     else:
         solver = pc.SharpClawSolver2D(rp)
+        if dq_src is not None:
+            solver.dq_src = dq_src
     solver.limiters = opts['limiters']
     solver.autotune = autotune
     solver.fma = fma
