@@ -6,7 +6,8 @@ Sources (all under the reference's apps/ directory):
   acoustics_2d_homogeneous/verify_classic.txt     p at t=0.12, 100x100, classic split
   acoustics_2d_homogeneous/verify_sharpclaw.txt   p at t=0.12, 100x100, SharpClaw
   acoustics_2d_variable/pressure_classic.txt      p at frame 20 (t=0.6), 200x200
-  acoustics_2d_variable/io_test_verification/fort.q0020, fort.a0000 (ascii frames)
+  acoustics_2d_variable/io_test_verification/fort.q0020, fort.a0000 (ascii frames,
+                                                  and the SHA-256 of fort.q0020, fort.t0020, fort.a0000)
   euler_2d/verify_shockbubble_classic.txt         density at t=0.2, 160x40
 Run in the container that has the reference: python tests/golden/make_fixtures.py
 """
@@ -38,6 +39,11 @@ def main():
     s0 = Solution()
     io.read(s0, 0, d, read_aux=True)
     out['acoustics_vc_forta0000'] = s0.state.aux
+    # SHA-256 of the reference's frame files, to pin the ascii writer byte for byte
+    import hashlib
+    for name in ('fort.q0020', 'fort.t0020', 'fort.a0000'):
+        with open(os.path.join(d, name), 'rb') as f:
+            out['sha256_' + name.replace('.', '_')] = np.array(hashlib.sha256(f.read()).hexdigest())
     np.savez_compressed(os.path.join(HERE, 'reference_fixtures.npz'), **out)
     for k, v in out.items():
         print(k, np.shape(v))

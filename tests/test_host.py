@@ -152,3 +152,25 @@ def test_ascii_frame_roundtrip(tmp_path):
     assert np.allclose(back.state.q, st.q, rtol=1e-7, atol=1e-8)
     assert np.allclose(back.state.aux, st.aux, rtol=1e-7, atol=1e-8)
     assert back.domain.grid.num_cells == [5, 3]
+
+
+def test_ascii_writer_reproduces_the_reference_frame_files(tmp_path):
+    """Frame 20 of the reference's variable-coefficient acoustics io test
+    (apps/acoustics_2d_variable/io_test_verification), written by
+    Solution.write from the values parsed out of those files: fort.q0020,
+    fort.t0020 and the aux file are byte-identical to the reference's (SHA-256
+    recorded by tests/golden/make_fixtures.py)."""
+    import hashlib
+    G = np.load(os.path.join(ROOT, 'tests', 'golden', 'reference_fixtures.npz'))
+    x = pc.Dimension('x', -1., 1., 200)
+    y = pc.Dimension('y', -1., 1., 200)
+    dom = pc.Domain([x, y])
+    st = pc.State(dom, 3, 2, device='cpu')
+    st.q[...] = G['acoustics_vc_fortq0020']
+    st.aux[...] = G['acoustics_vc_forta0000']
+    st.t = float(G['acoustics_vc_fortq0020_t'])
+    pc.Solution(st, dom).write(20, str(tmp_path), write_aux=True)
+    for mine, ref in (('fort.q0020', 'fort_q0020'), ('fort.t0020', 'fort_t0020'), ('fort.a0020', 'fort_a0000')):
+        with open(os.path.join(str(tmp_path), mine), 'rb') as f:
+            digest = hashlib.sha256(f.read()).hexdigest()
+        assert digest == str(G['sha256_' + ref]), mine
