@@ -129,6 +129,7 @@ class Controller(object):
             logging.info("Solution %s computed for time t=%f" % (frame, self.solution.t))
             for g in self.solution.state.grid.gauge_files:
                 g.flush()
+        self.solver.flush_gauge_values(self.solution)
         self.solver.teardown()
         for g in self.solution.state.grid.gauge_files:
             g.close()

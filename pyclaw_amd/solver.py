@@ -120,6 +120,7 @@ class Solver(object):
         self.auxbc = None
         self._ctx = None
         self._qalt = None
+        self._gauge_rec = None    # gauge values recorded on the device, written out by flush_gauge_values
         self._dt_dev = None
         self._cfl_dev = None
         self._static_filled = set()
@@ -288,6 +289,19 @@ class Solver(object):
             self.logger.info("Already at or beyond end time: no evolution required.")
             self.max_steps = 0
 
+        try:
+            self._step_loop(solution, tend, take_one_step, tstart)
+        finally:
+            # gauge values recorded on the device reach their files and
+            # gauge_data at the end of every evolve_to_time call
+            self.flush_gauge_values(solution)
+        if self.dt_variable and tend is not None and solution.t < tend \
+                and self.status['numsteps'] == self.max_steps:
+            raise Exception("Maximum number of timesteps have been taken")
+        return self.status
+
+    def _step_loop(self, solution, tend, take_one_step, tstart):
+        """The step loop of evolve_to_time (solver.py:629-696)."""
         for n in range(self.max_steps):
             state = solution.state
             if not take_one_step:
@@ -328,10 +342,6 @@ class Solver(object):
                 break
             if take_one_step:
                 break
-        if self.dt_variable and tend is not None and solution.t < tend \
-                and self.status['numsteps'] == self.max_steps:
-            raise Exception("Maximum number of timesteps have been taken")
-        return self.status
 
     def _hooks_active(self):
         return False
@@ -359,10 +369,23 @@ class Solver(object):
 
     # ---- gauges (solver.py:713-742) ------------------------------------------------------
     def write_gauge_values(self, solution):
+        """Record the gauge values of this step.  With the default
+        compute_gauge_values (q itself) and an FP64 device state, the values are
+        gathered into a device buffer (claw2d_gather, no host copy per step)
+        and written to the gauge files and gauge_data by flush_gauge_values, at
+        the end of every evolve_to_time call or when the buffer fills; a user
+        compute_gauge_values is called per step on host values, as in the
+        reference."""
         gauges = solution.state.grid.gauges
         if not gauges:
             return
         state = solution.state
+        buf = state._q.buf
+        if self.compute_gauge_values is default_compute_gauge_values and buf.is_cuda \
+                and buf.dtype == torch.float64:
+            self._record_gauges(solution)
+            return
+        self.flush_gauge_values(solution)
         idx = torch.tensor(gauges, dtype=torch.long)
         q = state._q.interior()
         qg = q[:, idx[:, 0].to(q.device), idx[:, 1].to(q.device)].to('cpu', torch.float64).numpy()
@@ -370,18 +393,64 @@ class Solver(object):
         if state._aux is not None:
             a = state._aux.interior()
             aux = a[:, idx[:, 0].to(a.device), idx[:, 1].to(a.device)].to('cpu', torch.float64).numpy()
-        t = solution.t
         for i in range(len(gauges)):
             p = self.compute_gauge_values(qg[:, i], None if aux is None else aux[:, i])
-            if state.keep_gauges:
-                gd = state.gauge_data
-                if len(gd) == len(gauges):
-                    gd[i] = np.vstack((gd[i], np.append(t, p)))
-                else:
-                    gd.append(np.append(t, p))
-            files = state.grid.gauge_files
-            if i < len(files):
-                files[i].write(str(t) + ' ' + ' '.join(str(j) for j in p) + '\n')
+            self._emit_gauge(state, i, solution.t, p)
+
+    def _emit_gauge(self, state, i, t, p):
+        """solver.py:727-737: keep_gauges data and one file line per gauge."""
+        if state.keep_gauges:
+            gd = state.gauge_data
+            if len(gd) == len(state.grid.gauges):
+                gd[i] = np.vstack((gd[i], np.append(t, p)))
+            else:
+                gd.append(np.append(t, p))
+        files = state.grid.gauge_files
+        if i < len(files):
+            files[i].write(str(t) + ' ' + ' '.join(str(j) for j in p) + '\n')
+
+    _GAUGE_CAP = 1024   # steps recorded on the device between flushes
+
+    def _record_gauges(self, solution):
+        state = solution.state
+        gauges = state.grid.gauges
+        buf = state._q.buf
+        ne = state.num_eqn
+        key = (tuple(map(tuple, gauges)), tuple(buf.shape), buf.device)
+        rec = self._gauge_rec
+        if rec is None or rec['key'] != key:
+            self.flush_gauge_values(solution)
+            rows, ldq = buf.shape[1], buf.shape[2]
+            plane = rows * ldq
+            # gauge-major: value g * num_eqn + m is component m at gauge g
+            offs = [m * plane + (iy + NG_STORE) * ldq + (ix + NG_STORE) for (ix, iy) in gauges for m in range(ne)]
+            rec = dict(key=key, offsets=torch.tensor(offs, dtype=torch.int64, device=buf.device),
+                       vals=torch.empty((self._GAUGE_CAP, len(offs)), dtype=torch.float64, device=buf.device),
+                       times=[], state=state)
+            self._gauge_rec = rec
+        if len(rec['times']) == self._GAUGE_CAP:
+            self.flush_gauge_values(solution)
+        k = len(rec['times'])
+        n = rec['offsets'].numel()
+        _lib.check(_lib.lib().claw2d_gather(buf.data_ptr(), rec['offsets'].data_ptr(), n,
+                                            rec['vals'][k].data_ptr(),
+                                            torch.cuda.current_stream(buf.device).cuda_stream), "claw2d_gather")
+        rec['times'].append(solution.t)
+        rec['state'] = state
+
+    def flush_gauge_values(self, solution=None):
+        """Write the gauge values recorded on the device (one host copy for all
+        recorded steps), in step order, exactly as the per-step path would."""
+        rec = self._gauge_rec
+        if rec is None or not rec['times']:
+            return
+        state = rec['state']
+        ne = state.num_eqn
+        vals = rec['vals'][:len(rec['times'])].to('cpu').numpy()
+        for r, t in enumerate(rec['times']):
+            for i in range(len(state.grid.gauges)):
+                self._emit_gauge(state, i, t, vals[r, i * ne:(i + 1) * ne])
+        rec['times'] = []
 
 
 def field_end(state, idim, ng):

@@ -208,7 +208,7 @@ class State(object):
         self.sync_to_device()
         result = State.__new__(State)
         for k, v in self.__dict__.items():
-            if k in ('_q', '_aux'):
+            if k in ('_q', '_aux', 'patch'):   # the patch (and its grid's open gauge files) is shared
                 continue
             setattr(result, k, copy.deepcopy(v, memo))
         result.patch = self.patch
