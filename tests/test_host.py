@@ -48,6 +48,18 @@ def test_invalid_arguments_return_einval_without_gpu():
     rc = L.claw2d_classic_step(ctx.handle, g, m, r, bc, _lib.F64, ctypes.c_void_p(16), ctypes.c_void_p(32),
                                None, ctypes.c_void_p(48), ctypes.c_void_p(64), None)
     assert rc == _lib.CLAW_EINVAL
+    # the context switch and the source terms validate before any launch
+    assert L.claw_ctx_set_wide_offsets(None, 1) == _lib.CLAW_EINVAL
+    assert L.claw_ctx_set_wide_offsets(ctx.handle, 2) == _lib.CLAW_EINVAL
+    assert L.claw_ctx_set_wide_offsets(ctx.handle, 0) == _lib.CLAW_OK
+    q, a, d = ctypes.c_void_p(16), ctypes.c_void_p(32), ctypes.c_void_p(48)
+    assert L.claw2d_euler_radial_step(g, _lib.F64, 5, q, None, 0.1, 0.4, None) == _lib.CLAW_EINVAL   # no aux
+    assert L.claw2d_euler_radial_step(g, _lib.F64, 3, q, a, 0.1, 0.4, None) == _lib.CLAW_EINVAL      # num_eqn < 4
+    assert L.claw2d_euler_radial_step(g, 7, 5, q, a, 0.1, 0.4, None) == _lib.CLAW_EINVAL             # precision
+    assert L.claw2d_euler_radial_step(g, _lib.F64, 5, q, a, -0.1, 0.4, None) == _lib.CLAW_EINVAL     # dt < 0
+    assert L.claw2d_euler_radial_dq(g, _lib.F64, 5, q, a, q, 0.1, 0.4, None) == _lib.CLAW_EINVAL     # dq aliases q
+    bad = _lib.ClawGrid(8, 8, 3, 0, 13, 196, 0.1, 0.1)   # ldq < mx + 2 ng
+    assert L.claw2d_euler_radial_dq(bad, _lib.F64, 5, q, a, d, 0.1, 0.4, None) == _lib.CLAW_EINVAL
 
 
 def test_dimension_and_domain_match_reference_doctests():
