@@ -55,8 +55,32 @@ template <> __device__ __forceinline__ double rcp<double>(double x) {
     return __builtin_fma(r, e, r);
 }
 
+// Square root in the shared-reciprocal build: Goldschmidt's iteration from the
+// hardware reciprocal square root y ~ 1/sqrt(x), carrying g -> sqrt(x) and
+// h -> 1/(2 sqrt(x)): one coupled step r = 1/2 - g h, g += g r, h += h r, then
+// two residual corrections g += (x - g^2) h.  Within about one ulp for normal
+// positive x (the quantities whose roots the solvers take: densities, depths,
+// squared sound speeds); 0, +inf and NaN pass through as sqrt returns them.
+// Without the library's rescaling of inputs below 2^-767 it takes 13
+// instructions instead of 18.  Other builds use the correctly rounded sqrt.
+__device__ __forceinline__ double sqrt_goldschmidt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return (x == 0.0 || x == __builtin_inf() || x != x) ? x : g;
+}
+
 template <typename R> __device__ __forceinline__ R rsqrt_exact(R x);
-template <> __device__ __forceinline__ double rsqrt_exact<double>(double x) { return __builtin_sqrt(x); }
+template <> __device__ __forceinline__ double rsqrt_exact<double>(double x) {
+    if constexpr (SHARED_RCP) return sqrt_goldschmidt(x);
+    return __builtin_sqrt(x);
+}
 template <> __device__ __forceinline__ float rsqrt_exact<float>(float x) { return __builtin_sqrtf(x); }
 
 // ---------------------------------------------------------------------------
