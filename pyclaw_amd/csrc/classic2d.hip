@@ -106,7 +106,10 @@ struct CarrySlots {
 // the batched work-queue kernel.  cs: this wavefront's carry slots
 // (CarrySlots<RP>::COUNT x WAVE values in LDS).
 // O32: every component plane is smaller than 4 GiB (32-bit byte offsets within a plane).
-template <class RP, typename R, bool O32 = false>
+// M22: the method is known to be order 2 with transverse correction waves
+// (transverse = 2), so its branches fold at compile time; otherwise p.order
+// and p.trans select at run time.
+template <class RP, typename R, bool O32 = false, bool M22 = false>
 __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy, R *__restrict__ cs) {
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
@@ -129,7 +132,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     const R dtdx = uniform(R(dtd / p.dx)), dtdy = uniform(R(dtd / p.dy));
     const R hx = uniform(R(0.5) * dtdx), hy = uniform(R(0.5) * dtdy);
     const R avx = uniform(R(0.5) * (dtdx + dtdx)), avy = uniform(R(0.5) * (dtdy + dtdy));
-    const int order = p.order, trans = p.trans;
+    const int order = M22 ? 2 : p.order, trans = M22 ? 2 : p.trans;
     const auto &P = p.P;
 
     // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
@@ -395,28 +398,28 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 // dynamic LDS (p.wpb x CarrySlots::COUNT x WAVE values).
 extern __shared__ __attribute__((aligned(16))) unsigned char classic_carry_lds[];
 
-template <class RP, typename R, bool O32>
+template <class RP, typename R, bool O32, bool M22>
 __device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
     const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
     if (sx >= p.nsx || sy >= p.nsy) return;
     R *carry = reinterpret_cast<R *>(classic_carry_lds);
-    unsplit_band<RP, R, O32>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+    unsplit_band<RP, R, O32, M22>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
 }
 
-template <class RP, typename R, bool O32>
+template <class RP, typename R, bool O32, bool M22>
 __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
-    unsplit_block<RP, R, O32>(p);
+    unsplit_block<RP, R, O32, M22>(p);
 }
 
 // The same kernel compiled for two wavefronts per SIMD: the register budget
 // halves (the FP64 Euler instantiation then spills to scratch), latency
 // hiding doubles.  The autotuner times both.
-template <class RP, typename R, bool O32>
+template <class RP, typename R, bool O32, bool M22>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void classic_unsplit_kernel_w2(const ClassicParams<RP, R> p) {
-    unsplit_block<RP, R, O32>(p);
+    unsplit_block<RP, R, O32, M22>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -859,14 +862,20 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         const size_t lds = (size_t)wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
         // 32-bit offsets when every byte offset within a component plane fits
         const bool o32 = !ctx->wide_offsets && (unsigned long long)grid->plane * sizeof(R) <= 0xffffffffull;
-        if (w2 && o32)
-            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, true>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
-        else if (w2)
-            hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, false>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
-        else if (o32)
-            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, true>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
-        else
-            hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, false>), dim3(nblocks), dim3(WAVE * wpb), lds, st, p);
+        // order 2 with transverse correction waves: the compile-time specialisation
+        const bool m22 = meth->order == 2 && meth->transverse == 2;
+        const dim3 grd(nblocks), blk(WAVE * wpb);
+        if (w2) {
+            if (o32 && m22) hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, true, true>), grd, blk, lds, st, p);
+            else if (o32) hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, true, false>), grd, blk, lds, st, p);
+            else if (m22) hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, false, true>), grd, blk, lds, st, p);
+            else hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, false, false>), grd, blk, lds, st, p);
+        } else {
+            if (o32 && m22) hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, true, true>), grd, blk, lds, st, p);
+            else if (o32) hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, true, false>), grd, blk, lds, st, p);
+            else if (m22) hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, false, true>), grd, blk, lds, st, p);
+            else hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, false, false>), grd, blk, lds, st, p);
+        }
         return check(hipGetLastError());
     }
     // split: x pass into scratch (padded layout, ghost rows included), then y pass
