@@ -106,11 +106,13 @@ struct CarrySlots {
 // the batched work-queue kernel.  cs: this wavefront's carry slots
 // (CarrySlots<RP>::COUNT x WAVE values in LDS).
 // O32: every component plane is smaller than 4 GiB (32-bit byte offsets within a plane).
-// M22: the method is known to be order 2 with transverse correction waves
-// (transverse = 2), so its branches fold at compile time; otherwise p.order
-// and p.trans select at run time.
-template <class RP, typename R, bool O32 = false, bool M22 = false>
+// SPEC: 0, p.order, p.trans and p.mthlim select at run time; 1, the method
+// is known to be order 2 with transverse correction waves (transverse = 2),
+// so its branches fold at compile time; 2, in addition every wave family is
+// limited with the MC limiter (id 4).
+template <class RP, typename R, bool O32 = false, int SPEC = 0>
 __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy, R *__restrict__ cs) {
+    constexpr bool M22 = SPEC >= 1;
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
     constexpr int NA = HAS_AUX ? RP::NA : 1;
@@ -204,7 +206,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 #pragma unroll
                     for (int m = 0; m < NE; ++m)
                         wn[m] = RP::template WNZ<0>(m, mw) ? lane_dn(wave[mw][m]) : R(0);
-                    const int mth = p.mthlim[mw];
+                    const int mth = SPEC == 2 ? 4 : p.mthlim[mw];
                     if (mth == 0) continue;
                     R wnorm2 = R(0), dotr = R(0);
 #pragma unroll
@@ -297,7 +299,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             if (order == 2) {
 #pragma unroll
                 for (int mw = 0; mw < NW; ++mw) {
-                    const int mth = p.mthlim[mw];
+                    const int mth = SPEC == 2 ? 4 : p.mthlim[mw];
                     if (mth == 0) continue;
                     R wnorm2 = R(0);
 #pragma unroll
@@ -398,28 +400,28 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 // dynamic LDS (p.wpb x CarrySlots::COUNT x WAVE values).
 extern __shared__ __attribute__((aligned(16))) unsigned char classic_carry_lds[];
 
-template <class RP, typename R, bool O32, bool M22>
+template <class RP, typename R, bool O32, int SPEC>
 __device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
     const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
     if (sx >= p.nsx || sy >= p.nsy) return;
     R *carry = reinterpret_cast<R *>(classic_carry_lds);
-    unsplit_band<RP, R, O32, M22>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+    unsplit_band<RP, R, O32, SPEC>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
 }
 
-template <class RP, typename R, bool O32, bool M22>
+template <class RP, typename R, bool O32, int SPEC>
 __global__ __launch_bounds__(256) void classic_unsplit_kernel(const ClassicParams<RP, R> p) {
-    unsplit_block<RP, R, O32, M22>(p);
+    unsplit_block<RP, R, O32, SPEC>(p);
 }
 
 // The same kernel compiled for two wavefronts per SIMD: the register budget
 // halves (the FP64 Euler instantiation then spills to scratch), latency
 // hiding doubles.  The autotuner times both.
-template <class RP, typename R, bool O32, bool M22>
+template <class RP, typename R, bool O32, int SPEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void classic_unsplit_kernel_w2(const ClassicParams<RP, R> p) {
-    unsplit_block<RP, R, O32, M22>(p);
+    unsplit_block<RP, R, O32, SPEC>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -862,20 +864,24 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         const size_t lds = (size_t)wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
         // 32-bit offsets when every byte offset within a component plane fits
         const bool o32 = !ctx->wide_offsets && (unsigned long long)grid->plane * sizeof(R) <= 0xffffffffull;
-        // order 2 with transverse correction waves: the compile-time specialisation
-        const bool m22 = meth->order == 2 && meth->transverse == 2;
-        const dim3 grd(nblocks), blk(WAVE * wpb);
-        if (w2) {
-            if (o32 && m22) hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, true, true>), grd, blk, lds, st, p);
-            else if (o32) hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, true, false>), grd, blk, lds, st, p);
-            else if (m22) hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, false, true>), grd, blk, lds, st, p);
-            else hipLaunchKernelGGL((classic_unsplit_kernel_w2<RP, R, false, false>), grd, blk, lds, st, p);
-        } else {
-            if (o32 && m22) hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, true, true>), grd, blk, lds, st, p);
-            else if (o32) hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, true, false>), grd, blk, lds, st, p);
-            else if (m22) hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, false, true>), grd, blk, lds, st, p);
-            else hipLaunchKernelGGL((classic_unsplit_kernel<RP, R, false, false>), grd, blk, lds, st, p);
+        // compile-time specialisations: order 2 with transverse correction
+        // waves, and that with the MC limiter on every wave family
+        int spec = 0;
+        if (meth->order == 2 && meth->transverse == 2) {
+            spec = 2;
+            for (int k = 0; k < RP::NW; ++k)
+                if (meth->mthlim[k] != 4) spec = 1;
         }
+        const dim3 grd(nblocks), blk(WAVE * wpb);
+#define CLAW_UNSPLIT_LAUNCH(KERNEL, O)                                                            \
+        if (spec == 2) hipLaunchKernelGGL((KERNEL<RP, R, O, 2>), grd, blk, lds, st, p);        \
+        else if (spec == 1) hipLaunchKernelGGL((KERNEL<RP, R, O, 1>), grd, blk, lds, st, p);   \
+        else hipLaunchKernelGGL((KERNEL<RP, R, O, 0>), grd, blk, lds, st, p);
+        if (w2 && o32) { CLAW_UNSPLIT_LAUNCH(classic_unsplit_kernel_w2, true) }
+        else if (w2) { CLAW_UNSPLIT_LAUNCH(classic_unsplit_kernel_w2, false) }
+        else if (o32) { CLAW_UNSPLIT_LAUNCH(classic_unsplit_kernel, true) }
+        else { CLAW_UNSPLIT_LAUNCH(classic_unsplit_kernel, false) }
+#undef CLAW_UNSPLIT_LAUNCH
         return check(hipGetLastError());
     }
     // split: x pass into scratch (padded layout, ghost rows included), then y pass
