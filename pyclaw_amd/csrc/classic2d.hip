@@ -109,7 +109,7 @@ struct CarrySlots {
 // SPEC: 0, p.order, p.trans and p.mthlim select at run time; 1, the method
 // is known to be order 2 with transverse correction waves (transverse = 2),
 // so its branches fold at compile time; 2, in addition every wave family is
-// limited with the MC limiter (id 4).
+// limited with the MC limiter (id 4) and a solver with an entropy fix has it on.
 template <class RP, typename R, bool O32 = false, int SPEC = 0>
 __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy, R *__restrict__ cs) {
     constexpr bool M22 = SPEC >= 1;
@@ -135,7 +135,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     const R hx = uniform(R(0.5) * dtdx), hy = uniform(R(0.5) * dtdy);
     const R avx = uniform(R(0.5) * (dtdx + dtdx)), avy = uniform(R(0.5) * (dtdy + dtdy));
     const int order = M22 ? 2 : p.order, trans = M22 ? 2 : p.trans;
-    const auto &P = p.P;
+    typename RP::Params P = p.P;
+    if constexpr (SPEC == 2 && RP::HAS_EFIX) P.efix = 1;
 
     // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
     // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
@@ -868,7 +869,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         // waves, and that with the MC limiter on every wave family
         int spec = 0;
         if (meth->order == 2 && meth->transverse == 2) {
-            spec = 2;
+            spec = (RP::HAS_EFIX && !rp->efix) ? 1 : 2;
             for (int k = 0; k < RP::NW; ++k)
                 if (meth->mthlim[k] != 4) spec = 1;
         }

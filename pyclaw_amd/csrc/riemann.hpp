@@ -88,6 +88,7 @@ template <> __device__ __forceinline__ float rsqrt_exact<float>(float x) { retur
 // ---------------------------------------------------------------------------
 template <typename R> struct Acoustics {
     static constexpr int NE = 3, NW = 2, NA = 0;
+    static constexpr bool HAS_EFIX = false;   // no entropy fix
     struct Params { R rho, bulk, cc, zz; };
     struct Cache {};
     static constexpr bool SPEEDS_FROM_CACHE = false;
@@ -128,6 +129,7 @@ template <typename R> struct Acoustics {
 // ---------------------------------------------------------------------------
 template <typename R> struct VCAcoustics {
     static constexpr int NE = 3, NW = 2, NA = 2;
+    static constexpr bool HAS_EFIX = false;   // no entropy fix
     struct Params { R unused; };
     struct Cache {};
     static constexpr bool SPEEDS_FROM_CACHE = false;
@@ -177,6 +179,7 @@ template <typename R> struct VCAcoustics {
 // ---------------------------------------------------------------------------
 template <typename R> struct Euler5 {
     static constexpr int NE = 5, NW = 5, NA = 0;
+    static constexpr bool HAS_EFIX = true;    // Params::efix switches the entropy fix
     struct Params { R gamma, gamma1; int efix; };
     // Roe averages kept for the transverse solves of the same interface; the
     // other quantities rpt needs are recomputed with rpn's expressions
@@ -400,6 +403,7 @@ template <typename R> struct Euler5 {
 // ---------------------------------------------------------------------------
 template <typename R> struct ShallowRoe {
     static constexpr int NE = 3, NW = 3, NA = 0;
+    static constexpr bool HAS_EFIX = false;   // the entropy fix is always on
     struct Params { R grav; };
     struct Cache { R u, v, a; };
     static constexpr bool SPEEDS_FROM_CACHE = false;

@@ -64,7 +64,7 @@ def padded_with_bc(interior, ng, bc_lower, bc_upper, negate, ghost_rng=None):
 
 def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False, mthlim=None,
                      bc_lower=(1, 1), bc_upper=(1, 1), abc_lower=(1, 1), abc_upper=(1, 1),
-                     dx=0.1, dy=0.07, precision='f64', wide_offsets=False):
+                     dx=0.1, dy=0.07, precision='f64', wide_offsets=False, efix=1):
     """Run claw2d_classic_step on the GPU.  qbc_ref/auxbc_ref are reference-layout
     padded arrays (ghost width 2 for the classic step); they are copied into the
     device layout with NG ghost layers (the outermost layer is padding).
@@ -95,7 +95,7 @@ def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False
         m.mthlim[k] = v
     r = _lib.ClawRiemann()
     r.kind = case['rp']
-    r.efix = 1
+    r.efix = efix
     vals = {O.RP_ACOUSTICS: ['rho', 'bulk', 'cc', 'zz'], O.RP_EULER: ['gamma', 'gamma1'],
             O.RP_SHALLOW: ['grav'], O.RP_VC_ACOUSTICS: []}[case['rp']]
     for k, key in enumerate(vals):
@@ -143,7 +143,7 @@ def _rp_struct(case):
     from pyclaw_amd import _lib
     r = _lib.ClawRiemann()
     r.kind = case['rp']
-    r.efix = 1
+    r.efix = efix
     vals = {O.RP_ACOUSTICS: ['rho', 'bulk', 'cc', 'zz'], O.RP_EULER: ['gamma', 'gamma1'],
             O.RP_SHALLOW: ['grav'], O.RP_VC_ACOUSTICS: []}[case['rp']]
     for k, key in enumerate(vals):

@@ -110,6 +110,24 @@ def test_wide_offset_kernels_give_identical_results(name, precision):
         assert np.array_equal(ref, wide) and cref == c2
 
 
+@pytest.mark.parametrize("trans", [1, 2])
+def test_euler_without_entropy_fix(trans):
+    """efix = 0 (rpn2_euler_5wave.f's plain Roe splitting) against the oracle
+    with the same switch; with transverse 2 and MC this is the method-
+    specialised kernel's fallback (its top level assumes the fix on)."""
+    import copy
+    case = copy.deepcopy(RP_CASES['euler'])
+    case['pd']['efix'] = 0
+    rng = np.random.default_rng(21)
+    q = case['make'](rng, 83, 41)
+    qbc = padded_with_bc(q, 2, (1, 1), (1, 1), True)
+    ref, cref = oracle_classic_step(qbc, None, case, case['dt'], 2, trans)
+    got, cgot = gpu_classic_step(qbc, None, case, case['dt'], 2, trans, efix=0)
+    assert np.array_equal(ref, got) and cref == cgot
+    with_fix, _ = gpu_classic_step(qbc, None, RP_CASES['euler'], case['dt'], 2, trans)
+    assert not np.array_equal(with_fix, got)   # the switch reaches the kernel
+
+
 def test_f32_error_vs_f64():
     """FP32 variant: relative L-inf error of one Euler step against the FP64 oracle."""
     case = RP_CASES['euler']
