@@ -934,6 +934,19 @@ int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const
     default: return launch_classic<ShallowRoe<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
     }
 }
+
+// FP32 step (CLAW_F32), built here: FP32 makes no bitwise promise, so it takes
+// this build's contraction and shared reciprocals.
+int classic_step_f32(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
+                     const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
+                     hipStream_t st) {
+    switch (rp->kind) {
+    case CLAW_RP_ACOUSTICS: return launch_classic<Acoustics<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    case CLAW_RP_VC_ACOUSTICS: return launch_classic<VCAcoustics<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    case CLAW_RP_EULER: return launch_classic<Euler5<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    default: return launch_classic<ShallowRoe<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    }
+}
 #else
 
 // ---- batched runs: host side -----------------------------------------------
@@ -1074,6 +1087,9 @@ namespace fused {
 int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
                      const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
                      hipStream_t st);
+int classic_step_f32(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
+                     const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
+                     hipStream_t st);
 }  // namespace fused
 }  // namespace clawhip
 
@@ -1142,20 +1158,21 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
     default: return CLAW_EINVAL;
     }
     if (meth->num_waves != nw_expected) return CLAW_EINVAL;
-    const bool f32 = (precision == CLAW_F32);
     if (precision == CLAW_F64_FMA)
         return fused::classic_step_f64(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
-    if (precision != CLAW_F32 && precision != CLAW_F64) return CLAW_EINVAL;
-#define DISPATCH(RPT)                                                                                      \
-    return f32 ? launch_classic<RPT<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st) \
-               : launch_classic<RPT<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st)
+    if (precision == CLAW_F32)
+        return fused::classic_step_f32(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+    if (precision != CLAW_F64) return CLAW_EINVAL;
     switch (rp->kind) {
-    case CLAW_RP_ACOUSTICS: DISPATCH(Acoustics);
-    case CLAW_RP_VC_ACOUSTICS: DISPATCH(VCAcoustics);
-    case CLAW_RP_EULER: DISPATCH(Euler5);
-    default: DISPATCH(ShallowRoe);
+    case CLAW_RP_ACOUSTICS:
+        return launch_classic<Acoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+    case CLAW_RP_VC_ACOUSTICS:
+        return launch_classic<VCAcoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+    case CLAW_RP_EULER:
+        return launch_classic<Euler5<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+    default:
+        return launch_classic<ShallowRoe<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
     }
-#undef DISPATCH
 }
 
 int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,

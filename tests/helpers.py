@@ -143,7 +143,7 @@ def _rp_struct(case):
     from pyclaw_amd import _lib
     r = _lib.ClawRiemann()
     r.kind = case['rp']
-    r.efix = efix
+    r.efix = int(case['pd'].get('efix', 1))
     vals = {O.RP_ACOUSTICS: ['rho', 'bulk', 'cc', 'zz'], O.RP_EULER: ['gamma', 'gamma1'],
             O.RP_SHALLOW: ['grav'], O.RP_VC_ACOUSTICS: []}[case['rp']]
     for k, key in enumerate(vals):
