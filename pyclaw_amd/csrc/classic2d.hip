@@ -725,7 +725,8 @@ struct BatchStepParams {
     int mthlim[8];
 };
 
-template <class RP, typename R>
+// O32: every instance's component planes are under 4 GiB (32-bit in-plane offsets).
+template <class RP, typename R, bool O32>
 __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<RP, R> b) {
     const int lane = threadIdx.x & (WAVE - 1);
     // blockDim.x / WAVE wavefronts, each with its carry slots in dynamic LDS
@@ -753,7 +754,7 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<R
 #pragma unroll
         for (int q = 0; q < 8; ++q) p.mthlim[q] = b.mthlim[q];
         p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0;
-        unsplit_band<RP, R>(p, local / I.nsy, local % I.nsy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+        unsplit_band<RP, R, O32>(p, local / I.nsy, local % I.nsy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
     }
 }
 
@@ -984,9 +985,14 @@ static int batch_step_launch(BatchPlan *pl, void *qpool, const void *auxpool, co
     b.order = pl->meth.order; b.trans = pl->meth.transverse; b.ng = pl->ng;
     for (int k = 0; k < 8; ++k) b.mthlim[k] = pl->meth.mthlim[k];
     const size_t lds = (size_t)pl->wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
+    bool o32 = true;
+    for (const BatchInstDev &I : pl->hinst)
+        if ((unsigned long long)I.plane * sizeof(R) > 0xffffffffull) o32 = false;
     if (pl->blocks == 0) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<RP, R>, WAVE * pl->wpb, lds) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, o32 ? batch_step_kernel<RP, R, true>
+                                                                       : batch_step_kernel<RP, R, false>,
+                                                         WAVE * pl->wpb, lds) !=
             hipSuccess)
             per_cu = 1;
         int dev = 0;
@@ -996,7 +1002,10 @@ static int batch_step_launch(BatchPlan *pl, void *qpool, const void *auxpool, co
         // no more waves than work items
         pl->blocks = std::min(pl->blocks, (pl->nitems + pl->wpb - 1) / pl->wpb);
     }
-    hipLaunchKernelGGL((batch_step_kernel<RP, R>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
+    if (o32)
+        hipLaunchKernelGGL((batch_step_kernel<RP, R, true>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<RP, R, false>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 
