@@ -284,6 +284,49 @@ int claw2d_step_update(int n, double cfl_max, double cfl_desired, double dt_max,
                        double *cfl, double *t, double *cflmax, int32_t *parity,
                        int32_t *naccepted, int32_t *nrejected, void *stream);
 
+/* ---- evolve_to_time's step loop on the device -------------------------------
+   Replaces the host loop of src/pyclaw/solver.py:629-696 (dt clipped to tend,
+   accept if cfl <= cfl_max else restore q, dt = min(dt_max,
+   dt*cfl_desired/cfl), t += dt, status counters) for runs with no per-step
+   host work, so that many steps are queued between host synchronisations.
+   The state is double-buffered: iteration k reads cur = states[k%2] and
+   cfl[k%2] and writes next = states[(k+1)%2] and cfl[(k+1)%2] = 0; its step
+   reads dt from &cur->dt and accumulates into cfl[k%2].  Iterations after the
+   loop's end (t >= tend, or max_steps iterations) leave every field as is. */
+typedef struct {
+    double cfl_max, cfl_desired, dt_max, tend, tstart;
+    int32_t max_steps, pad;
+} ClawLoopRule;
+
+typedef struct {
+    double dt;           /* the next step's dt (clipped to tend) */
+    double t;            /* solution time */
+    double cflmax, dtmin, dtmax;   /* status['cflmax'/'dtmin'/'dtmax'] */
+    double cfl_last;     /* Courant number of the last iteration */
+    int32_t naccepted;   /* status['numsteps'] */
+    int32_t niter;       /* iterations taken (accepted + rejected) */
+    int32_t done;        /* 1 once t >= tend or niter == max_steps */
+    int32_t pad;
+} ClawLoopState;
+
+/* One iteration's bookkeeping after the step qold -> qnew: qnew := qold
+   (nbytes, the whole padded buffer) when the step is rejected or the loop has
+   ended, so the state is always in qnew afterwards. */
+int claw2d_loop_advance(const ClawLoopRule *rule, const ClawLoopState *cur, ClawLoopState *next,
+                        const double *cfl_cur, double *cfl_next, const void *qold, void *qnew,
+                        long long nbytes, void *stream);
+
+/* niter iterations of that loop for the classic step, launched from here:
+   iteration i steps q[(k0+i)%2] -> q[(k0+i+1)%2] (q = {qa, qb}) with
+   claw2d_classic_step_ex(..., CLAW_STEP_ACCUMULATE_CFL) reading dt from
+   states[(k0+i)%2] and accumulating into cfl[(k0+i)%2], then
+   claw2d_loop_advance.  states: two ClawLoopState, cfl: two doubles, device
+   memory.  No host synchronisation. */
+int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                        const ClawBC *bc, int precision, void *qa, void *qb, const void *aux,
+                        const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
+                        int niter, void *stream);
+
 /* ---- Discrete adjoint of the classic step (variable-coefficient acoustics) ---
    No counterpart in the reference, which has no adjoint.  The forward map is
    claw2d_classic_step with CLAW_RP_VC_ACOUSTICS (aux = (rho, c), c > 0 in

@@ -44,6 +44,19 @@ class ClawBC(ctypes.Structure):
                 ("aux_lower", ctypes.c_int32 * 2), ("aux_upper", ctypes.c_int32 * 2)]
 
 
+class ClawLoopRule(ctypes.Structure):
+    _fields_ = [("cfl_max", ctypes.c_double), ("cfl_desired", ctypes.c_double), ("dt_max", ctypes.c_double),
+                ("tend", ctypes.c_double), ("tstart", ctypes.c_double), ("max_steps", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
+class ClawLoopState(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("t", ctypes.c_double), ("cflmax", ctypes.c_double),
+                ("dtmin", ctypes.c_double), ("dtmax", ctypes.c_double), ("cfl_last", ctypes.c_double),
+                ("naccepted", ctypes.c_int32), ("niter", ctypes.c_int32), ("done", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
 class ClawError(RuntimeError):
     pass
 
@@ -85,6 +98,10 @@ def _optional_symbols():
     vp = ctypes.c_void_p
     return {
         "claw_ctx_set_wide_offsets": [vp, ctypes.c_int],
+        "claw2d_loop_advance": [ctypes.POINTER(ClawLoopRule), vp, vp, vp, vp, vp, vp, ctypes.c_longlong, vp],
+        "claw2d_classic_loop": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod), ctypes.POINTER(ClawRiemann),
+                                ctypes.POINTER(ClawBC), ctypes.c_int, vp, vp, vp, ctypes.POINTER(ClawLoopRule), vp, vp,
+                                ctypes.c_longlong, ctypes.c_int, ctypes.c_int, vp],
         "claw2d_euler_radial_step": [ctypes.POINTER(ClawGrid), ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_double,
                                      ctypes.c_double, vp],
         "claw2d_euler_radial_dq": [ctypes.POINTER(ClawGrid), ctypes.c_int, ctypes.c_int, vp, vp, vp,

@@ -146,19 +146,10 @@ class ClawSolver2D(ClawSolver):
     def teardown(self):
         self._ctx = None
 
-    def step_hyperbolic(self, solution):
-        """classic/solver.py:498-547 on the GPU: ghost handling, one fused
-        kernel (unsplit) or two passes (split), Courant number back to the host."""
-        state = solution.states[0]
-        if not self._is_set_up:
-            self.setup(solution)
-        state.sync_to_device()
-        bc = self._kernel_bcs(state)
-        grid = self._grid_struct(state)
-        prec = _lib.F32 if state.dtype == torch.float32 else (_lib.F64_FMA if self.fma else _lib.F64)
-        self._dt_dev.fill_(float(self.dt))
-        aux = state._aux.buf.data_ptr() if state._aux is not None else None
-        stream = torch.cuda.current_stream().cuda_stream
+    def _precision(self, state):
+        return _lib.F32 if state.dtype == torch.float32 else (_lib.F64_FMA if self.fma else _lib.F64)
+
+    def _autotune_once(self, state, grid, bc, prec, aux, stream):
         key = (grid.mx, grid.my, prec, self._method_struct.order, self._method_struct.transverse,
                self._method_struct.split, self._rp_struct.kind)
         if self.autotune and key not in self._tuned:
@@ -169,6 +160,43 @@ class ClawSolver2D(ClawSolver):
                                                     ctypes.byref(band))
             _lib.check(rc, "claw2d_classic_autotune")
             self._tuned.add(key)
+
+    def _loop_launcher(self, state):
+        """Iterations of the device step loop: the fused kernel(s) of
+        step_hyperbolic reading dt from the device and accumulating the Courant
+        number there, each followed by the loop's bookkeeping, all launched by
+        claw2d_classic_loop."""
+        if not self._is_set_up or self._hooks_active():
+            return None
+        state.sync_to_device()
+        bc = self._kernel_bcs(state)
+        grid = self._grid_struct(state)
+        prec = self._precision(state)
+        aux = state._aux.buf.data_ptr() if state._aux is not None else None
+        self._dt_dev.fill_(float(self.dt))
+        self._autotune_once(state, grid, bc, prec, aux, torch.cuda.current_stream().cuda_stream)
+        fn = _lib.lib().claw2d_classic_loop
+        ctx, m, r = self._ctx.handle, self._method_struct, self._rp_struct
+
+        def launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream):
+            rc = fn(ctx, grid, m, r, bc, prec, qa, qb, aux, rule, states, cfl, nbytes, k0, niter, stream)
+            _lib.check(rc, "claw2d_classic_loop")
+        return launch
+
+    def step_hyperbolic(self, solution):
+        """classic/solver.py:498-547 on the GPU: ghost handling, one fused
+        kernel (unsplit) or two passes (split), Courant number back to the host."""
+        state = solution.states[0]
+        if not self._is_set_up:
+            self.setup(solution)
+        state.sync_to_device()
+        bc = self._kernel_bcs(state)
+        grid = self._grid_struct(state)
+        prec = self._precision(state)
+        self._dt_dev.fill_(float(self.dt))
+        aux = state._aux.buf.data_ptr() if state._aux is not None else None
+        stream = torch.cuda.current_stream().cuda_stream
+        self._autotune_once(state, grid, bc, prec, aux, stream)
         rc = _lib.lib().claw2d_classic_step(self._ctx.handle, grid, self._method_struct, self._rp_struct, bc,
                                             prec, state._q.buf.data_ptr(), self._qalt.data_ptr(), aux,
                                             self._dt_dev.data_ptr(), self._cfl_dev.data_ptr(), stream)

@@ -783,6 +783,54 @@ __global__ void batch_update_kernel(int n, double cfl_max, double cfl_desired, d
     cfl[k] = 0.0;
 }
 
+// The step loop of solver.py:629-696 for one grid (claw2d_loop_advance).
+// Every thread decides from cur/cfl_cur, which no thread of this launch
+// writes; thread 0 of block 0 alone writes next and cfl_next.  Python's
+// min(a, b) is "b < a ? b : a" and max(a, b) is "b > a ? b : a": kept so, with
+// the same operation order, the bookkeeping equals the host loop's bit for bit.
+__global__ __launch_bounds__(256) void loop_advance_kernel(ClawLoopRule r, const ClawLoopState *cur,
+                                                           ClawLoopState *next, const double *cfl_cur,
+                                                           double *cfl_next, const uint4 *qold, uint4 *qnew,
+                                                           long long n16, const unsigned *qold4, unsigned *qnew4,
+                                                           int ntail) {
+    const ClawLoopState s = *cur;
+    const double c = *cfl_cur;
+    const bool restore = s.done || !(c <= r.cfl_max);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ClawLoopState n = s;
+        if (!s.done) {
+            const bool ok = c <= r.cfl_max;
+            n.niter = s.niter + 1;
+            n.cfl_last = c;
+            if (ok) {
+                n.cflmax = s.cflmax > c ? s.cflmax : c;
+                n.t = s.t + s.dt;
+                n.naccepted = s.naccepted + 1;
+            }
+            double d = r.dt_max;
+            if (c > 0.0) {
+                const double x = s.dt * r.cfl_desired / c;
+                d = x < r.dt_max ? x : r.dt_max;
+                n.dtmin = s.dtmin < d ? s.dtmin : d;
+                n.dtmax = s.dtmax > d ? s.dtmax : d;
+            }
+            if (n.t >= r.tend || n.niter >= r.max_steps) {
+                n.done = 1;
+            } else {
+                if (n.t + d > r.tend && r.tstart < r.tend) d = r.tend - n.t;
+                if (r.tend - n.t - d < 1.e-14 * n.t) d = r.tend - n.t;
+            }
+            n.dt = d;
+        }
+        *next = n;
+        *cfl_next = 0.0;
+    }
+    if (!restore) return;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) qnew[k] = qold[k];
+    if (blockIdx.x == 0 && (int)threadIdx.x < ntail) qnew4[threadIdx.x] = qold4[threadIdx.x];
+}
+
 __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *counter = 0;
@@ -1415,6 +1463,42 @@ int claw2d_batch_steps(void *plan, void *q_pool, const void *aux_pool, int nstep
         pl->key_par[0] = cfl_max; pl->key_par[1] = cfl_desired; pl->key_par[2] = dt_max;
     }
     return hipGraphLaunch(pl->exec, st) == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+int claw2d_loop_advance(const ClawLoopRule *rule, const ClawLoopState *cur, ClawLoopState *next,
+                        const double *cfl_cur, double *cfl_next, const void *qold, void *qnew, long long nbytes,
+                        void *stream) {
+    if (!rule || !cur || !next || !cfl_cur || !cfl_next || !qold || !qnew) return CLAW_EINVAL;
+    if ((const void *)cur == (const void *)next || cfl_cur == cfl_next || qold == qnew) return CLAW_EINVAL;
+    if (nbytes <= 0 || nbytes % 4 || ((uintptr_t)qold | (uintptr_t)qnew) % 16) return CLAW_EINVAL;
+    const long long n16 = nbytes / 16;
+    const int ntail = (int)(nbytes % 16) / 4;
+    // a small grid-stride launch: accepted steps (nearly all) only read two
+    // scalars per block; the copy runs on rejected and trailing iterations
+    const long long want = (n16 + 255) / 256;
+    const int nb = (int)std::max<long long>(1, std::min<long long>(want, 64));
+    hipLaunchKernelGGL(loop_advance_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, *rule, cur, next, cfl_cur,
+                       cfl_next, (const uint4 *)qold, (uint4 *)qnew, n16,
+                       (const unsigned *)qold + n16 * 4, (unsigned *)qnew + n16 * 4, ntail);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
+int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                        const ClawBC *bc, int precision, void *qa, void *qb, const void *aux,
+                        const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
+                        int niter, void *stream) {
+    if (!states || !cfl || !qa || !qb || niter < 0 || k0 < 0) return CLAW_EINVAL;
+    void *q[2] = {qa, qb};
+    for (int i = 0; i < niter; ++i) {
+        const int a = (k0 + i) & 1;
+        int rc = claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, q[a], q[1 - a], aux, &states[a].dt,
+                                        &cfl[a], CLAW_STEP_ACCUMULATE_CFL, stream);
+        if (rc == CLAW_OK)
+            rc = claw2d_loop_advance(rule, &states[a], &states[1 - a], &cfl[a], &cfl[1 - a], q[a], q[1 - a], nbytes,
+                                     stream);
+        if (rc != CLAW_OK) return rc;
+    }
+    return CLAW_OK;
 }
 
 int claw2d_step_update(int n, double cfl_max, double cfl_desired, double dt_max, double *dt, double *cfl, double *t,

@@ -12,7 +12,9 @@ Behaviour follows the reference's Solver (src/pyclaw/solver.py:25-742):
 On the GPU the step writes into a second buffer, so the reference's q_backup
 copy is replaced by keeping the previous buffer until the step is accepted.
 """
+import ctypes
 import logging
+import math
 
 import numpy as np
 import torch
@@ -129,6 +131,9 @@ class Solver(object):
         # shapes and keeps the fastest for the solver's context; results do not change.
         self.autotune = False
         self._tuned = set()
+        # evolve_to_time queues its steps with the accept/reject bookkeeping on
+        # the device where no step needs the host; results do not change.
+        self.device_loop = True
         if riemann_solver is not None:
             self.rp = riemann_solver
             self.num_eqn = riemann_solver.num_eqn
@@ -290,7 +295,11 @@ class Solver(object):
             self.max_steps = 0
 
         try:
-            self._step_loop(solution, tend, take_one_step, tstart)
+            launch = self._device_loop_launcher(solution, take_one_step)
+            if launch is not None:
+                self._device_step_loop(solution, tend, tstart, launch)
+            else:
+                self._step_loop(solution, tend, take_one_step, tstart)
         finally:
             # gauge values recorded on the device reach their files and
             # gauge_data at the end of every evolve_to_time call
@@ -342,6 +351,81 @@ class Solver(object):
                 break
             if take_one_step:
                 break
+
+    def _device_loop_launcher(self, solution, take_one_step):
+        """The device step of _loop_launcher, or None for the host loop.  The
+        step loop runs on the device when no step needs the host: a
+        variable dt, no gauges, no per-step debug log, no user hooks or ghost
+        cells filled by user functions each step, and a solver with a device
+        step (_loop_launcher)."""
+        if not self.device_loop or take_one_step or not self.dt_variable or self.max_steps <= 0:
+            return None
+        state = solution.state
+        if state.grid.gauges or self._hooks_active() or self.logger.isEnabledFor(logging.DEBUG):
+            return None
+        if not hasattr(_lib.lib(), 'claw2d_classic_loop'):
+            return None
+        q_custom = any(b == BC.custom for b in self.bc_lower + self.bc_upper)
+        a_custom = state.num_aux > 0 and any(b == BC.custom for b in self.aux_bc_lower + self.aux_bc_upper)
+        if (q_custom and not self.custom_bc_static) or a_custom:
+            return None
+        return self._loop_launcher(state)
+
+    def _loop_launcher(self, state):
+        """A function launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream)
+        that queues niter iterations of the device step loop (claw2d_loop_advance
+        after each step; iteration k0 + i reads q[(k0+i)%2], states[(k0+i)%2],
+        cfl[(k0+i)%2]), or None where the solver has none."""
+        return None
+
+    def _device_step_loop(self, solution, tend, tstart, launch):
+        """_step_loop with the bookkeeping of solver.py:629-696 on the device
+        (claw2d_loop_advance after each step): steps are queued in chunks of the
+        number left at the current dt, with one host synchronisation per chunk.
+        Solution, status, dt and Courant number end as the host loop leaves them."""
+        state = solution.state
+        dev = state._q.buf.device
+        t = solution.t
+        dt = self.dt
+        # the first step's clip to tend (solver.py:637-641); the device clips the rest
+        if t + dt > tend and tstart < tend:
+            dt = tend - t
+        if tend - t - dt < 1.e-14 * t:
+            dt = tend - t
+        st = _lib.ClawLoopState(dt, t, self.status['cflmax'], self.status['dtmin'], self.status['dtmax'],
+                                self.cfl.get_cached_max(), 0, 0, 0, 0)
+        rule = _lib.ClawLoopRule(self.cfl_max, self.cfl_desired, self.dt_max, tend, tstart, self.max_steps, 0)
+        size = ctypes.sizeof(_lib.ClawLoopState)
+        sbuf = torch.zeros(2 * size, dtype=torch.uint8, device=dev)
+        sbuf[:size].copy_(torch.frombuffer(bytearray(st), dtype=torch.uint8))
+        cfl = torch.zeros(2, dtype=torch.float64, device=dev)
+        bufs = [state._q.buf, self._qalt]
+        qp = [b.data_ptr() for b in bufs]
+        nbytes = bufs[0].numel() * bufs[0].element_size()
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        rref = ctypes.byref(rule)
+        k = 0
+        while True:
+            left = self.max_steps - st.niter
+            est = int(math.ceil((tend - st.t) / st.dt)) if st.dt > 0 else 1
+            n = max(1, min(left, est, 1024))
+            launch(k, n, qp[0], qp[1], rref, sbuf.data_ptr(), cfl.data_ptr(), nbytes, stream)
+            k += n
+            a = k & 1
+            st = _lib.ClawLoopState.from_buffer_copy(sbuf[a * size:(a + 1) * size].cpu().numpy().tobytes())
+            if st.done:
+                break
+        if k & 1:
+            self._swap_in(state)
+        else:
+            state._device_updated()
+        solution.t = st.t
+        self.dt = st.dt
+        self.cfl.update_global_max(st.cfl_last)
+        self.status['cflmax'] = st.cflmax
+        self.status['dtmin'] = st.dtmin
+        self.status['dtmax'] = st.dtmax
+        self.status['numsteps'] += st.naccepted
 
     def _hooks_active(self):
         return False

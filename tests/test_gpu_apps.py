@@ -21,6 +21,15 @@ RP = {'acoustics': 'rp2_acoustics', 'vc_acoustics': 'rp2_vc_acoustics', 'euler':
 
 
 def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False, fma=False, dq_src=None):
+    claw = amd_controller(q0, aux, opts, step_source, user_bc_lower, autotune, fma, dq_src)
+    status = claw.run()
+    return [f.state.q.copy() for f in claw.frames], status
+
+
+def amd_controller(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False, fma=False, dq_src=None,
+                   **solver_attrs):
+    """The application's Controller, set up and not yet run; solver_attrs are
+    set on the solver last."""
     import pyclaw_amd as pc
     x = pc.Dimension('x', opts['lower'][0], opts['upper'][0], opts['mx'])
     y = pc.Dimension('y', opts['lower'][1], opts['upper'][1], opts['my'])
@@ -61,8 +70,9 @@ def amd_run(q0, aux, opts, step_source=None, user_bc_lower=None, autotune=False,
     claw.keep_copy = True
     claw.tfinal = opts['tfinal']
     claw.num_output_times = opts['num_output_times']
-    status = claw.run()
-    return [f.state.q.copy() for f in claw.frames], status
+    for k, v in solver_attrs.items():
+        setattr(solver, k, v)
+    return claw
 
 
 def _compare_frames(got, ref):

@@ -173,7 +173,7 @@ def test_autotuned_band_heights_give_identical_results():
     dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
     sh = torch.cuda.current_stream().cuda_stream
     outs, cfls = [], []
-    for tuned in (False, True, (5, 1), (5, 2), (70, 2), (36, 2, 1), (7, 1, 2), (3, 2, 1)):
+    for tuned in (False, True, (5, 1), (5, 2), (70, 2), (36, 2, 1), (7, 1, 2), (3, 2, 1), (1, 1, 1), (2, 2, 4)):
         ctx = _lib.Context(0)
         cfl = torch.zeros((), dtype=torch.float64, device=dev)
         qb = torch.zeros_like(qa)

@@ -58,6 +58,21 @@ def test_invalid_arguments_return_einval_without_gpu():
     assert L.claw2d_euler_radial_step(g, 7, 5, q, a, 0.1, 0.4, None) == _lib.CLAW_EINVAL             # precision
     assert L.claw2d_euler_radial_step(g, _lib.F64, 5, q, a, -0.1, 0.4, None) == _lib.CLAW_EINVAL     # dt < 0
     assert L.claw2d_euler_radial_dq(g, _lib.F64, 5, q, a, q, 0.1, 0.4, None) == _lib.CLAW_EINVAL     # dq aliases q
+    # the device step loop's bookkeeping
+    rule = _lib.ClawLoopRule(1.0, 0.9, 1e99, 1.0, 0.0, 10, 0)
+    s0, s1, c0, c1 = ctypes.c_void_p(256), ctypes.c_void_p(512), ctypes.c_void_p(768), ctypes.c_void_p(776)
+    qa, qb = ctypes.c_void_p(4096), ctypes.c_void_p(8192)
+    adv = L.claw2d_loop_advance
+    assert adv(None, s0, s1, c0, c1, qa, qb, 64, None) == _lib.CLAW_EINVAL
+    assert adv(rule, s0, s0, c0, c1, qa, qb, 64, None) == _lib.CLAW_EINVAL        # cur == next
+    assert adv(rule, s0, s1, c0, c0, qa, qb, 64, None) == _lib.CLAW_EINVAL        # cfl slots alias
+    assert adv(rule, s0, s1, c0, c1, qa, qa, 64, None) == _lib.CLAW_EINVAL        # qold == qnew
+    assert adv(rule, s0, s1, c0, c1, qa, qb, 62, None) == _lib.CLAW_EINVAL        # not whole 4-byte words
+    assert adv(rule, s0, s1, c0, c1, qa, ctypes.c_void_p(8200), 64, None) == _lib.CLAW_EINVAL   # misaligned
+    assert L.claw2d_classic_loop(ctx.handle, g, m, r, bc, _lib.F64, qa, qb, None, rule, s0, c0, 64, 0, -1,
+                                 None) == _lib.CLAW_EINVAL
+    assert L.claw2d_classic_loop(ctx.handle, g, m, r, bc, _lib.F64, qa, qb, None, rule, s0, c0, 64, 0, 0,
+                                 None) == _lib.CLAW_OK                           # nothing to launch
     bad = _lib.ClawGrid(8, 8, 3, 0, 13, 196, 0.1, 0.1)   # ldq < mx + 2 ng
     assert L.claw2d_euler_radial_dq(bad, _lib.F64, 5, q, a, d, 0.1, 0.4, None) == _lib.CLAW_EINVAL
 
