@@ -295,7 +295,9 @@ int claw2d_step_update(int n, double cfl_max, double cfl_desired, double dt_max,
    loop's end (t >= tend, or max_steps iterations) leave every field as is. */
 typedef struct {
     double cfl_max, cfl_desired, dt_max, tend, tstart;
-    int32_t max_steps, pad;
+    int32_t max_steps;
+    int32_t ncfl;   /* Courant numbers per step (0 or 1; 10 for the SSP104 stages, of which the
+                       first above cfl_max decides, else the last: sharpclaw/solver.py:221-251) */
 } ClawLoopRule;
 
 typedef struct {
@@ -311,7 +313,8 @@ typedef struct {
 
 /* One iteration's bookkeeping after the step qold -> qnew: qnew := qold
    (nbytes, the whole padded buffer) when the step is rejected or the loop has
-   ended, so the state is always in qnew afterwards. */
+   ended, so the state is always in qnew afterwards.  cfl_cur and cfl_next
+   hold ncfl values each. */
 int claw2d_loop_advance(const ClawLoopRule *rule, const ClawLoopState *cur, ClawLoopState *next,
                         const double *cfl_cur, double *cfl_next, const void *qold, void *qnew,
                         long long nbytes, void *stream);
@@ -326,6 +329,14 @@ int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth,
                         const ClawBC *bc, int precision, void *qa, void *qb, const void *aux,
                         const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
                         int niter, void *stream);
+
+/* The same for SharpClaw: whole SSP104 steps (claw2d_sharpclaw_ssp104_step,
+   stage registers s1a/s1b/s2) with rule->ncfl = 10 and cfl holding 2 x 10
+   doubles (iteration i uses cfl + 10*((k0+i)%2)). */
+int claw2d_sharpclaw_loop(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                          int precision, void *qa, void *qb, void *s1a, void *s1b, void *s2, const void *aux,
+                          const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
+                          int niter, void *stream);
 
 /* ---- Discrete adjoint of the classic step (variable-coefficient acoustics) ---
    No counterpart in the reference, which has no adjoint.  The forward map is

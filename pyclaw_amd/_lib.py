@@ -47,7 +47,7 @@ class ClawBC(ctypes.Structure):
 class ClawLoopRule(ctypes.Structure):
     _fields_ = [("cfl_max", ctypes.c_double), ("cfl_desired", ctypes.c_double), ("dt_max", ctypes.c_double),
                 ("tend", ctypes.c_double), ("tstart", ctypes.c_double), ("max_steps", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("ncfl", ctypes.c_int32)]
 
 
 class ClawLoopState(ctypes.Structure):
@@ -99,6 +99,9 @@ def _optional_symbols():
     return {
         "claw_ctx_set_wide_offsets": [vp, ctypes.c_int],
         "claw2d_loop_advance": [ctypes.POINTER(ClawLoopRule), vp, vp, vp, vp, vp, vp, ctypes.c_longlong, vp],
+        "claw2d_sharpclaw_loop": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.POINTER(ClawBC),
+                                  ctypes.c_int, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ClawLoopRule), vp, vp,
+                                  ctypes.c_longlong, ctypes.c_int, ctypes.c_int, vp],
         "claw2d_classic_loop": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod), ctypes.POINTER(ClawRiemann),
                                 ctypes.POINTER(ClawBC), ctypes.c_int, vp, vp, vp, ctypes.POINTER(ClawLoopRule), vp, vp,
                                 ctypes.c_longlong, ctypes.c_int, ctypes.c_int, vp],

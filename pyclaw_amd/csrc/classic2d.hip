@@ -794,7 +794,10 @@ __global__ __launch_bounds__(256) void loop_advance_kernel(ClawLoopRule r, const
                                                            long long n16, const unsigned *qold4, unsigned *qnew4,
                                                            int ntail) {
     const ClawLoopState s = *cur;
-    const double c = *cfl_cur;
+    const int nc = r.ncfl > 1 ? r.ncfl : 1;
+    double c = cfl_cur[nc - 1];
+    for (int k = 0; k < nc - 1; ++k)
+        if (cfl_cur[k] > r.cfl_max) { c = cfl_cur[k]; break; }
     const bool restore = s.done || !(c <= r.cfl_max);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ClawLoopState n = s;
@@ -823,7 +826,7 @@ __global__ __launch_bounds__(256) void loop_advance_kernel(ClawLoopRule r, const
             n.dt = d;
         }
         *next = n;
-        *cfl_next = 0.0;
+        for (int k = 0; k < nc; ++k) cfl_next[k] = 0.0;
     }
     if (!restore) return;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -1470,6 +1473,9 @@ int claw2d_loop_advance(const ClawLoopRule *rule, const ClawLoopState *cur, Claw
                         void *stream) {
     if (!rule || !cur || !next || !cfl_cur || !cfl_next || !qold || !qnew) return CLAW_EINVAL;
     if ((const void *)cur == (const void *)next || cfl_cur == cfl_next || qold == qnew) return CLAW_EINVAL;
+    if (rule->ncfl < 0 || rule->ncfl > 16) return CLAW_EINVAL;
+    const int nc = rule->ncfl > 1 ? rule->ncfl : 1;
+    if (cfl_next < cfl_cur + nc && cfl_cur < cfl_next + nc) return CLAW_EINVAL;
     if (nbytes <= 0 || nbytes % 4 || ((uintptr_t)qold | (uintptr_t)qnew) % 16) return CLAW_EINVAL;
     const long long n16 = nbytes / 16;
     const int ntail = (int)(nbytes % 16) / 4;
@@ -1488,6 +1494,7 @@ int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth,
                         const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
                         int niter, void *stream) {
     if (!states || !cfl || !qa || !qb || niter < 0 || k0 < 0) return CLAW_EINVAL;
+    if (!rule || rule->ncfl > 1) return CLAW_EINVAL;
     void *q[2] = {qa, qb};
     for (int i = 0; i < niter; ++i) {
         const int a = (k0 + i) & 1;

@@ -576,5 +576,23 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
     return CLAW_OK;
 }
 
+int claw2d_sharpclaw_loop(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
+                          int precision, void *qa, void *qb, void *s1a, void *s1b, void *s2, const void *aux,
+                          const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
+                          int niter, void *stream) {
+    if (!rule || rule->ncfl != 10 || !states || !cfl || !qa || !qb || niter < 0 || k0 < 0) return CLAW_EINVAL;
+    void *q[2] = {qa, qb};
+    for (int i = 0; i < niter; ++i) {
+        const int a = (k0 + i) & 1;
+        double *ca = cfl + 10 * a, *cb = cfl + 10 * (1 - a);
+        int rc = claw2d_sharpclaw_ssp104_step(ctx, grid, rp, bc, precision, q[a], q[1 - a], s1a, s1b, s2, aux,
+                                              &states[a].dt, ca, stream);
+        if (rc == CLAW_OK)
+            rc = claw2d_loop_advance(rule, &states[a], &states[1 - a], ca, cb, q[a], q[1 - a], nbytes, stream);
+        if (rc != CLAW_OK) return rc;
+    }
+    return CLAW_OK;
+}
+
 }  // extern "C"
 #endif  // CLAW_SHARP_MAIN

@@ -166,6 +166,34 @@ class SharpClawSolver(Solver):
             ok = self._step_staged(state)
         return ok
 
+    def _loop_launcher(self, state):
+        """Iterations of the device step loop: whole SSP104 steps
+        (claw2d_sharpclaw_ssp104_step), each followed by the loop's bookkeeping
+        with the stage rule of _apply_cfl_rule, launched by
+        claw2d_sharpclaw_loop.  None where a step needs the host (another
+        integrator, dq_src, a before_step hook, user boundary functions)."""
+        has_custom = any(b == 0 for b in self.bc_lower + self.bc_upper) and not self.custom_bc_static
+        if (not self._is_set_up or self.time_integrator != 'SSP104' or self.dq_src is not None
+                or self.call_before_step_each_stage or self.before_step is not before_step or has_custom):
+            return None
+        state.sync_to_device()
+        if self.autotune:
+            self._tune(state)
+        bc = self._kernel_bcs(state)
+        grid = self._grid_struct(state)
+        prec = self._precision(state)
+        aux = state._aux.buf.data_ptr() if state._aux is not None else None
+        fn = _lib.lib().claw2d_sharpclaw_loop
+        ctx, r = self._ctx.handle, self._rp_struct
+        regs = [self._regs[k].data_ptr() for k in ('s1a', 's1b', 's2')]
+
+        def launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream):
+            rc = fn(ctx, grid, r, bc, prec, qa, qb, regs[0], regs[1], regs[2], aux, rule, states, cfl, nbytes,
+                    k0, niter, stream)
+            _lib.check(rc, "claw2d_sharpclaw_loop")
+        launch.ncfl = 10
+        return launch
+
     def _precision(self, state):
         if state.dtype == torch.float32:
             return _lib.F32

@@ -374,8 +374,9 @@ class Solver(object):
     def _loop_launcher(self, state):
         """A function launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream)
         that queues niter iterations of the device step loop (claw2d_loop_advance
-        after each step; iteration k0 + i reads q[(k0+i)%2], states[(k0+i)%2],
-        cfl[(k0+i)%2]), or None where the solver has none."""
+        after each step; iteration k0 + i reads q[(k0+i)%2], states[(k0+i)%2] and
+        the launch.ncfl (default 1) Courant numbers at cfl[ncfl*((k0+i)%2)]), or
+        None where the solver has none."""
         return None
 
     def _device_step_loop(self, solution, tend, tstart, launch):
@@ -394,11 +395,12 @@ class Solver(object):
             dt = tend - t
         st = _lib.ClawLoopState(dt, t, self.status['cflmax'], self.status['dtmin'], self.status['dtmax'],
                                 self.cfl.get_cached_max(), 0, 0, 0, 0)
-        rule = _lib.ClawLoopRule(self.cfl_max, self.cfl_desired, self.dt_max, tend, tstart, self.max_steps, 0)
+        ncfl = getattr(launch, 'ncfl', 1)
+        rule = _lib.ClawLoopRule(self.cfl_max, self.cfl_desired, self.dt_max, tend, tstart, self.max_steps, ncfl)
         size = ctypes.sizeof(_lib.ClawLoopState)
         sbuf = torch.zeros(2 * size, dtype=torch.uint8, device=dev)
         sbuf[:size].copy_(torch.frombuffer(bytearray(st), dtype=torch.uint8))
-        cfl = torch.zeros(2, dtype=torch.float64, device=dev)
+        cfl = torch.zeros(2 * ncfl, dtype=torch.float64, device=dev)
         bufs = [state._q.buf, self._qalt]
         qp = [b.data_ptr() for b in bufs]
         nbytes = bufs[0].numel() * bufs[0].element_size()

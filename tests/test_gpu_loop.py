@@ -42,11 +42,16 @@ def _cases():
     out['acoustics_rejected_first_step'] = (q0, aux, big)
     out['vc_acoustics'] = apps.acoustics_variable('classic')
     out['acoustics_unsplit'] = (q0, aux, dict(opts, dimensional_split=False, transverse=2))
+    q0, aux, opts = apps.acoustics_homogeneous('sharpclaw')
+    out['sharpclaw_acoustics'] = (q0, aux, opts)
+    out['sharpclaw_acoustics_rejected_first_step'] = (q0, aux, dict(opts, dt_initial=0.1))
+    out['sharpclaw_vc_acoustics'] = apps.acoustics_variable('sharpclaw')
     return out
 
 
 @pytest.mark.parametrize("name", ['acoustics', 'acoustics_rejected_first_step', 'vc_acoustics',
-                                  'acoustics_unsplit'])
+                                  'acoustics_unsplit', 'sharpclaw_acoustics',
+                                  'sharpclaw_acoustics_rejected_first_step', 'sharpclaw_vc_acoustics'])
 @pytest.mark.parametrize("fma", [False, True])
 def test_device_loop_equals_host_loop(name, fma):
     print("loop:", name, "fma", fma, flush=True)
@@ -68,7 +73,7 @@ def test_device_loop_with_static_custom_inflow():
     _same(_run((q0, aux, opts), True, **kw), _run((q0, aux, opts), False, **kw))
 
 
-@pytest.mark.parametrize("name", ['acoustics', 'acoustics_rejected_first_step'])
+@pytest.mark.parametrize("name", ['acoustics', 'acoustics_rejected_first_step', 'sharpclaw_acoustics'])
 def test_device_loop_stops_at_max_steps(name):
     """max_steps iterations end the loop short of tend.  solver.py:697-699
     raises only when every one of them was accepted; with a rejected step the
@@ -79,14 +84,15 @@ def test_device_loop_stops_at_max_steps(name):
     dev = _run(case, True, max_steps=3)
     host = _run(case, False, max_steps=3)
     assert dev[3] < opts['tfinal']
-    if name == 'acoustics':
+    if name in ('acoustics', 'sharpclaw_acoustics'):
         assert dev[5] is not None and 'Maximum number of timesteps' in dev[5]
     else:
         assert dev[5] is None
     _same(dev, host)
 
 
-def test_device_loop_is_used():
+@pytest.mark.parametrize("kind", ['classic', 'sharpclaw'])
+def test_device_loop_is_used(kind):
     """The device loop runs (no host fallback) for the homogeneous acoustics app."""
     import pyclaw_amd.solver as S
     calls = []
@@ -97,7 +103,7 @@ def test_device_loop_is_used():
         return orig(self, *a, **k)
     S.Solver._device_step_loop = spy
     try:
-        q0, aux, opts = apps.acoustics_homogeneous('classic')
+        q0, aux, opts = apps.acoustics_homogeneous(kind)
         amd_controller(q0, aux, opts).run()
     finally:
         S.Solver._device_step_loop = orig
