@@ -409,8 +409,9 @@ class Solver(object):
         k = 0
         while True:
             left = self.max_steps - st.niter
-            est = int(math.ceil((tend - st.t) / st.dt)) if st.dt > 0 else 1
-            n = max(1, min(left, est, 1024))
+            x = (tend - st.t) / st.dt if st.dt > 0 else 1.0
+            est = int(math.ceil(x)) if x < 1024 else 1024      # steps left at the current dt
+            n = max(1, min(left, est))
             launch(k, n, qp[0], qp[1], rref, sbuf.data_ptr(), cfl.data_ptr(), nbytes, stream)
             k += n
             a = k & 1
