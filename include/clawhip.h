@@ -298,6 +298,9 @@ typedef struct {
     int32_t max_steps;
     int32_t ncfl;   /* Courant numbers per step (0 or 1; 10 for the SSP104 stages, of which the
                        first above cfl_max decides, else the last: sharpclaw/solver.py:221-251) */
+    int32_t dt_variable;   /* 0: fixed dt, t = tstart + n*dt, and a step above cfl_max ends the
+                              loop with failed = 1, its result kept (solver.py:655-677) */
+    int32_t pad;
 } ClawLoopRule;
 
 typedef struct {
@@ -307,8 +310,8 @@ typedef struct {
     double cfl_last;     /* Courant number of the last iteration */
     int32_t naccepted;   /* status['numsteps'] */
     int32_t niter;       /* iterations taken (accepted + rejected) */
-    int32_t done;        /* 1 once t >= tend or niter == max_steps */
-    int32_t pad;
+    int32_t done;        /* 1 once t >= tend or niter == max_steps (or failed) */
+    int32_t failed;      /* fixed dt only: the last step's Courant number was above cfl_max */
 } ClawLoopState;
 
 /* One iteration's bookkeeping after the step qold -> qnew: qnew := qold

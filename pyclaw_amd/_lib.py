@@ -47,14 +47,14 @@ class ClawBC(ctypes.Structure):
 class ClawLoopRule(ctypes.Structure):
     _fields_ = [("cfl_max", ctypes.c_double), ("cfl_desired", ctypes.c_double), ("dt_max", ctypes.c_double),
                 ("tend", ctypes.c_double), ("tstart", ctypes.c_double), ("max_steps", ctypes.c_int32),
-                ("ncfl", ctypes.c_int32)]
+                ("ncfl", ctypes.c_int32), ("dt_variable", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class ClawLoopState(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("t", ctypes.c_double), ("cflmax", ctypes.c_double),
                 ("dtmin", ctypes.c_double), ("dtmax", ctypes.c_double), ("cfl_last", ctypes.c_double),
                 ("naccepted", ctypes.c_int32), ("niter", ctypes.c_int32), ("done", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("failed", ctypes.c_int32)]
 
 
 class ClawError(RuntimeError):

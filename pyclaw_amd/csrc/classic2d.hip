@@ -798,26 +798,32 @@ __global__ __launch_bounds__(256) void loop_advance_kernel(ClawLoopRule r, const
     double c = cfl_cur[nc - 1];
     for (int k = 0; k < nc - 1; ++k)
         if (cfl_cur[k] > r.cfl_max) { c = cfl_cur[k]; break; }
-    const bool restore = s.done || !(c <= r.cfl_max);
+    // a rejected step is undone with variable dt; with fixed dt its result stays
+    const bool restore = s.done || (!(c <= r.cfl_max) && r.dt_variable);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ClawLoopState n = s;
         if (!s.done) {
             const bool ok = c <= r.cfl_max;
             n.niter = s.niter + 1;
             n.cfl_last = c;
+            if (ok || !r.dt_variable) n.cflmax = s.cflmax > c ? s.cflmax : c;
             if (ok) {
-                n.cflmax = s.cflmax > c ? s.cflmax : c;
-                n.t = s.t + s.dt;
+                n.t = r.dt_variable ? s.t + s.dt : r.tstart + (double)(s.niter + 1) * s.dt;
                 n.naccepted = s.naccepted + 1;
+            } else if (!r.dt_variable) {
+                n.failed = 1;
             }
-            double d = r.dt_max;
-            if (c > 0.0) {
-                const double x = s.dt * r.cfl_desired / c;
-                d = x < r.dt_max ? x : r.dt_max;
-                n.dtmin = s.dtmin < d ? s.dtmin : d;
-                n.dtmax = s.dtmax > d ? s.dtmax : d;
+            double d = s.dt;
+            if (r.dt_variable) {
+                d = r.dt_max;
+                if (c > 0.0) {
+                    const double x = s.dt * r.cfl_desired / c;
+                    d = x < r.dt_max ? x : r.dt_max;
+                    n.dtmin = s.dtmin < d ? s.dtmin : d;
+                    n.dtmax = s.dtmax > d ? s.dtmax : d;
+                }
             }
-            if (n.t >= r.tend || n.niter >= r.max_steps) {
+            if (n.failed || n.t >= r.tend || n.niter >= r.max_steps) {
                 n.done = 1;
             } else {
                 if (n.t + d > r.tend && r.tstart < r.tend) d = r.tend - n.t;

@@ -173,7 +173,9 @@ class SharpClawSolver(Solver):
         claw2d_sharpclaw_loop.  None where a step needs the host (another
         integrator, dq_src, a before_step hook, user boundary functions)."""
         has_custom = any(b == 0 for b in self.bc_lower + self.bc_upper) and not self.custom_bc_static
-        if (not self._is_set_up or self.time_integrator != 'SSP104' or self.dq_src is not None
+        # fixed dt stays on the host: an aborted step leaves q as it was, where
+        # the device loop with fixed dt keeps the classic step's result
+        if (not self._is_set_up or not self.dt_variable or self.time_integrator != 'SSP104' or self.dq_src is not None
                 or self.call_before_step_each_stage or self.before_step is not before_step or has_custom):
             return None
         state.sync_to_device()

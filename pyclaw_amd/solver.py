@@ -354,11 +354,10 @@ class Solver(object):
 
     def _device_loop_launcher(self, solution, take_one_step):
         """The device step of _loop_launcher, or None for the host loop.  The
-        step loop runs on the device when no step needs the host: a
-        variable dt, no gauges, no per-step debug log, no user hooks or ghost
+        step loop runs on the device when no step needs the host: no gauges, no per-step debug log, no user hooks or ghost
         cells filled by user functions each step, and a solver with a device
         step (_loop_launcher)."""
-        if not self.device_loop or take_one_step or not self.dt_variable or self.max_steps <= 0:
+        if not self.device_loop or take_one_step or self.max_steps <= 0:
             return None
         state = solution.state
         if state.grid.gauges or self._hooks_active() or self.logger.isEnabledFor(logging.DEBUG):
@@ -396,7 +395,8 @@ class Solver(object):
         st = _lib.ClawLoopState(dt, t, self.status['cflmax'], self.status['dtmin'], self.status['dtmax'],
                                 self.cfl.get_cached_max(), 0, 0, 0, 0)
         ncfl = getattr(launch, 'ncfl', 1)
-        rule = _lib.ClawLoopRule(self.cfl_max, self.cfl_desired, self.dt_max, tend, tstart, self.max_steps, ncfl)
+        rule = _lib.ClawLoopRule(self.cfl_max, self.cfl_desired, self.dt_max, tend, tstart, self.max_steps, ncfl,
+                                 1 if self.dt_variable else 0, 0)
         size = ctypes.sizeof(_lib.ClawLoopState)
         sbuf = torch.zeros(2 * size, dtype=torch.uint8, device=dev)
         sbuf[:size].copy_(torch.frombuffer(bytearray(st), dtype=torch.uint8))
@@ -429,6 +429,8 @@ class Solver(object):
         self.status['dtmin'] = st.dtmin
         self.status['dtmax'] = st.dtmax
         self.status['numsteps'] += st.naccepted
+        if st.failed:
+            raise Exception('CFL too large, giving up!')
 
     def _hooks_active(self):
         return False

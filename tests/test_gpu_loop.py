@@ -108,3 +108,40 @@ def test_device_loop_is_used(kind):
     finally:
         S.Solver._device_step_loop = orig
     assert len(calls) == opts['num_output_times']
+
+
+def _fixed(opts, steps_per_frame=None, dt=None):
+    """Fixed-dt options: dt divides the output interval."""
+    interval = opts['tfinal'] / opts['num_output_times']
+    if dt is None:
+        dt = interval / steps_per_frame
+    return dict(opts, dt_initial=dt, dt_variable=False)
+
+
+@pytest.mark.parametrize("fma", [False, True])
+def test_device_loop_with_fixed_dt(fma):
+    """dt_variable = False: t = tstart + n dt (solver.py:655-659), the clip of
+    the last step to each output time, no dt rule."""
+    print("loop: fixed dt fma", fma, flush=True)
+    q0, aux, opts = apps.acoustics_homogeneous('classic')
+    interval = opts['tfinal'] / opts['num_output_times']
+    k = int(np.ceil(interval / opts['dt_initial']))
+    case = (q0, aux, _fixed(opts, steps_per_frame=k))
+    kw = dict(dt_variable=False, fma=fma)
+    dev = _run(case, True, **kw)
+    host = _run(case, False, **kw)
+    assert dev[5] is None
+    _same(dev, host)
+
+
+def test_device_loop_fixed_dt_above_cfl_max_fails_like_the_host_loop():
+    """A fixed dt above the stable one: the first step's Courant number is above
+    cfl_max, both loops raise 'CFL too large' with that step's result kept."""
+    print("loop: fixed dt CFL failure", flush=True)
+    q0, aux, opts = apps.acoustics_homogeneous('classic')
+    big = 2.0 * opts['dt_initial'] / 0.45          # Courant number about 2 (dt_initial is 0.45 dx / c)
+    case = (q0, aux, dict(opts, dt_initial=big, tfinal=4 * big, num_output_times=1))
+    dev = _run(case, True, dt_variable=False)
+    host = _run(case, False, dt_variable=False)
+    assert dev[5] is not None and 'CFL too large' in dev[5]
+    _same(dev, host)
