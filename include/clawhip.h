@@ -314,6 +314,17 @@ typedef struct {
     int32_t failed;      /* fixed dt only: the last step's Courant number was above cfl_max */
 } ClawLoopState;
 
+/* Gauges recorded by the loop entry points below (solver.py:713-742 with the
+   default compute_gauge_values, FP64): after each accepted iteration,
+   vals[row*n + k] = q[offsets[k]] of the new state and times[row] = t, with
+   row = naccepted - 1 - base (rows outside [0, cap) are not written). */
+typedef struct {
+    const long long *offsets;
+    double *vals;
+    double *times;
+    int32_t n, base, cap, pad;
+} ClawLoopGauges;
+
 /* One iteration's bookkeeping after the step qold -> qnew: qnew := qold
    (nbytes, the whole padded buffer) when the step is rejected or the loop has
    ended, so the state is always in qnew afterwards.  cfl_cur and cfl_next
@@ -326,12 +337,13 @@ int claw2d_loop_advance(const ClawLoopRule *rule, const ClawLoopState *cur, Claw
    iteration i steps q[(k0+i)%2] -> q[(k0+i+1)%2] (q = {qa, qb}) with
    claw2d_classic_step_ex(..., CLAW_STEP_ACCUMULATE_CFL) reading dt from
    states[(k0+i)%2] and accumulating into cfl[(k0+i)%2], then
-   claw2d_loop_advance.  states: two ClawLoopState, cfl: two doubles, device
-   memory.  No host synchronisation. */
+   claw2d_loop_advance, then (gauges != NULL, FP64 only) the gauge record.
+   states: two ClawLoopState, cfl: two doubles, device memory.  No host
+   synchronisation. */
 int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
                         const ClawBC *bc, int precision, void *qa, void *qb, const void *aux,
                         const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
-                        int niter, void *stream);
+                        int niter, const ClawLoopGauges *gauges, void *stream);
 
 /* The same for SharpClaw: whole SSP104 steps (claw2d_sharpclaw_ssp104_step,
    stage registers s1a/s1b/s2) with rule->ncfl = 10 and cfl holding 2 x 10
@@ -339,7 +351,7 @@ int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth,
 int claw2d_sharpclaw_loop(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
                           int precision, void *qa, void *qb, void *s1a, void *s1b, void *s2, const void *aux,
                           const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
-                          int niter, void *stream);
+                          int niter, const ClawLoopGauges *gauges, void *stream);
 
 /* ---- Discrete adjoint of the classic step (variable-coefficient acoustics) ---
    No counterpart in the reference, which has no adjoint.  The forward map is

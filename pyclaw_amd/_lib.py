@@ -57,6 +57,11 @@ class ClawLoopState(ctypes.Structure):
                 ("failed", ctypes.c_int32)]
 
 
+class ClawLoopGauges(ctypes.Structure):
+    _fields_ = [("offsets", ctypes.c_void_p), ("vals", ctypes.c_void_p), ("times", ctypes.c_void_p),
+                ("n", ctypes.c_int32), ("base", ctypes.c_int32), ("cap", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
 class ClawError(RuntimeError):
     pass
 
@@ -101,10 +106,10 @@ def _optional_symbols():
         "claw2d_loop_advance": [ctypes.POINTER(ClawLoopRule), vp, vp, vp, vp, vp, vp, ctypes.c_longlong, vp],
         "claw2d_sharpclaw_loop": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawRiemann), ctypes.POINTER(ClawBC),
                                   ctypes.c_int, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ClawLoopRule), vp, vp,
-                                  ctypes.c_longlong, ctypes.c_int, ctypes.c_int, vp],
+                                  ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ClawLoopGauges), vp],
         "claw2d_classic_loop": [vp, ctypes.POINTER(ClawGrid), ctypes.POINTER(ClawMethod), ctypes.POINTER(ClawRiemann),
                                 ctypes.POINTER(ClawBC), ctypes.c_int, vp, vp, vp, ctypes.POINTER(ClawLoopRule), vp, vp,
-                                ctypes.c_longlong, ctypes.c_int, ctypes.c_int, vp],
+                                ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ClawLoopGauges), vp],
         "claw2d_euler_radial_step": [ctypes.POINTER(ClawGrid), ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_double,
                                      ctypes.c_double, vp],
         "claw2d_euler_radial_dq": [ctypes.POINTER(ClawGrid), ctypes.c_int, ctypes.c_int, vp, vp, vp,

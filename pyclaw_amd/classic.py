@@ -178,8 +178,8 @@ class ClawSolver2D(ClawSolver):
         fn = _lib.lib().claw2d_classic_loop
         ctx, m, r = self._ctx.handle, self._method_struct, self._rp_struct
 
-        def launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream):
-            rc = fn(ctx, grid, m, r, bc, prec, qa, qb, aux, rule, states, cfl, nbytes, k0, niter, stream)
+        def launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream, gauges=None):
+            rc = fn(ctx, grid, m, r, bc, prec, qa, qb, aux, rule, states, cfl, nbytes, k0, niter, gauges, stream)
             _lib.check(rc, "claw2d_classic_loop")
         return launch
 

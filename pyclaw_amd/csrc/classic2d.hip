@@ -840,6 +840,25 @@ __global__ __launch_bounds__(256) void loop_advance_kernel(ClawLoopRule r, const
     if (blockIdx.x == 0 && (int)threadIdx.x < ntail) qnew4[threadIdx.x] = qold4[threadIdx.x];
 }
 
+// Gauge values of an accepted iteration (ClawLoopGauges): one wavefront.
+__global__ __launch_bounds__(64) void loop_gauge_kernel(const ClawLoopState *cur, const ClawLoopState *next,
+                                                        const double *q, ClawLoopGauges G) {
+    const int acc0 = cur->naccepted, acc1 = next->naccepted;
+    if (acc1 == acc0) return;            // rejected, or after the loop's end
+    const int row = acc1 - 1 - G.base;
+    if (row < 0 || row >= G.cap) return;
+    for (int k = threadIdx.x; k < G.n; k += blockDim.x) G.vals[(long long)row * G.n + k] = q[G.offsets[k]];
+    if (threadIdx.x == 0) G.times[row] = next->t;
+}
+
+int loop_gauges(const ClawLoopGauges *G, int precision, const ClawLoopState *cur, const ClawLoopState *next,
+                const void *q, hipStream_t st) {
+    if (!G || G->n <= 0) return CLAW_OK;
+    if (precision == CLAW_F32 || !G->offsets || !G->vals || !G->times || G->cap <= 0) return CLAW_EINVAL;
+    hipLaunchKernelGGL(loop_gauge_kernel, dim3(1), dim3(64), 0, st, cur, next, (const double *)q, *G);
+    return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
+}
+
 __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *counter = 0;
@@ -1498,8 +1517,9 @@ int claw2d_loop_advance(const ClawLoopRule *rule, const ClawLoopState *cur, Claw
 int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
                         const ClawBC *bc, int precision, void *qa, void *qb, const void *aux,
                         const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
-                        int niter, void *stream) {
+                        int niter, const ClawLoopGauges *gauges, void *stream) {
     if (!states || !cfl || !qa || !qb || niter < 0 || k0 < 0) return CLAW_EINVAL;
+    if (gauges && gauges->n > 0 && precision == CLAW_F32) return CLAW_EINVAL;
     if (!rule || rule->ncfl > 1) return CLAW_EINVAL;
     void *q[2] = {qa, qb};
     for (int i = 0; i < niter; ++i) {
@@ -1509,6 +1529,7 @@ int claw2d_classic_loop(void *ctx, const ClawGrid *grid, const ClawMethod *meth,
         if (rc == CLAW_OK)
             rc = claw2d_loop_advance(rule, &states[a], &states[1 - a], &cfl[a], &cfl[1 - a], q[a], q[1 - a], nbytes,
                                      stream);
+        if (rc == CLAW_OK) rc = loop_gauges(gauges, precision, &states[a], &states[1 - a], q[1 - a], (hipStream_t)stream);
         if (rc != CLAW_OK) return rc;
     }
     return CLAW_OK;

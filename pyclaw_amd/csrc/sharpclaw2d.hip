@@ -576,11 +576,21 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
     return CLAW_OK;
 }
 
+}  // extern "C"
+namespace clawhip {
+namespace exact {   // defined with the classic loop (classic2d.hip)
+int loop_gauges(const ClawLoopGauges *G, int precision, const ClawLoopState *cur, const ClawLoopState *next,
+                const void *q, hipStream_t st);
+}
+}
+extern "C" {
+
 int claw2d_sharpclaw_loop(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
                           int precision, void *qa, void *qb, void *s1a, void *s1b, void *s2, const void *aux,
                           const ClawLoopRule *rule, ClawLoopState *states, double *cfl, long long nbytes, int k0,
-                          int niter, void *stream) {
+                          int niter, const ClawLoopGauges *gauges, void *stream) {
     if (!rule || rule->ncfl != 10 || !states || !cfl || !qa || !qb || niter < 0 || k0 < 0) return CLAW_EINVAL;
+    if (gauges && gauges->n > 0 && precision == CLAW_F32) return CLAW_EINVAL;
     void *q[2] = {qa, qb};
     for (int i = 0; i < niter; ++i) {
         const int a = (k0 + i) & 1;
@@ -589,6 +599,7 @@ int claw2d_sharpclaw_loop(void *ctx, const ClawGrid *grid, const ClawRiemann *rp
                                               &states[a].dt, ca, stream);
         if (rc == CLAW_OK)
             rc = claw2d_loop_advance(rule, &states[a], &states[1 - a], ca, cb, q[a], q[1 - a], nbytes, stream);
+        if (rc == CLAW_OK) rc = loop_gauges(gauges, precision, &states[a], &states[1 - a], q[1 - a], (hipStream_t)stream);
         if (rc != CLAW_OK) return rc;
     }
     return CLAW_OK;

@@ -189,9 +189,9 @@ class SharpClawSolver(Solver):
         ctx, r = self._ctx.handle, self._rp_struct
         regs = [self._regs[k].data_ptr() for k in ('s1a', 's1b', 's2')]
 
-        def launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream):
+        def launch(k0, niter, qa, qb, rule, states, cfl, nbytes, stream, gauges=None):
             rc = fn(ctx, grid, r, bc, prec, qa, qb, regs[0], regs[1], regs[2], aux, rule, states, cfl, nbytes,
-                    k0, niter, stream)
+                    k0, niter, gauges, stream)
             _lib.check(rc, "claw2d_sharpclaw_loop")
         launch.ncfl = 10
         return launch

@@ -360,7 +360,10 @@ class Solver(object):
         if not self.device_loop or take_one_step or self.max_steps <= 0:
             return None
         state = solution.state
-        if state.grid.gauges or self._hooks_active() or self.logger.isEnabledFor(logging.DEBUG):
+        if self._hooks_active() or self.logger.isEnabledFor(logging.DEBUG):
+            return None
+        if state.grid.gauges and not (self.compute_gauge_values is default_compute_gauge_values
+                                      and state._q.buf.dtype == torch.float64):
             return None
         if not hasattr(_lib.lib(), 'claw2d_classic_loop'):
             return None
@@ -406,16 +409,39 @@ class Solver(object):
         nbytes = bufs[0].numel() * bufs[0].element_size()
         stream = torch.cuda.current_stream(dev).cuda_stream
         rref = ctypes.byref(rule)
+        gauges = state.grid.gauges
+        G = None
+        if gauges:
+            # recorded after each accepted step on the device, written out per chunk
+            self.flush_gauge_values(solution)
+            ne, rows, ldq = state.num_eqn, bufs[0].shape[1], bufs[0].shape[2]
+            offs = torch.tensor([m * rows * ldq + (iy + NG_STORE) * ldq + (ix + NG_STORE)
+                                 for (ix, iy) in gauges for m in range(ne)], dtype=torch.int64, device=dev)
+            vals = torch.empty((self._GAUGE_CAP, offs.numel()), dtype=torch.float64, device=dev)
+            times = torch.empty(self._GAUGE_CAP, dtype=torch.float64, device=dev)
+            G = _lib.ClawLoopGauges(offs.data_ptr(), vals.data_ptr(), times.data_ptr(), offs.numel(), 0,
+                                    self._GAUGE_CAP, 0)
         k = 0
         while True:
             left = self.max_steps - st.niter
             x = (tend - st.t) / st.dt if st.dt > 0 else 1.0
             est = int(math.ceil(x)) if x < 1024 else 1024      # steps left at the current dt
-            n = max(1, min(left, est))
-            launch(k, n, qp[0], qp[1], rref, sbuf.data_ptr(), cfl.data_ptr(), nbytes, stream)
+            n = max(1, min(left, est, self._GAUGE_CAP))
+            base = st.naccepted
+            if G is not None:
+                G.base = base
+            launch(k, n, qp[0], qp[1], rref, sbuf.data_ptr(), cfl.data_ptr(), nbytes, stream,
+                   None if G is None else ctypes.byref(G))
             k += n
             a = k & 1
             st = _lib.ClawLoopState.from_buffer_copy(sbuf[a * size:(a + 1) * size].cpu().numpy().tobytes())
+            if G is not None and st.naccepted > base:
+                m = st.naccepted - base
+                tv = times[:m].cpu().numpy()
+                vv = vals[:m].cpu().numpy()
+                for r in range(m):
+                    for i in range(len(gauges)):
+                        self._emit_gauge(state, i, float(tv[r]), vv[r, i * ne:(i + 1) * ne])
             if st.done:
                 break
         if k & 1:

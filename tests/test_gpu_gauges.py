@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 GAUGES = [(0.0, 0.0), (0.31, -0.42), (-0.77, 0.5), (0.5, 0.9)]
 
 
-def _run(outdir, custom):
+def _run(outdir, custom, device_loop=True):
     import pyclaw_amd as pc
     q0, aux, opts = apps.acoustics_variable('classic')
     x = pc.Dimension('x', opts['lower'][0], opts['upper'][0], opts['mx'])
@@ -37,6 +37,7 @@ def _run(outdir, custom):
     solver.bc_lower, solver.bc_upper = list(opts['bc_lower']), list(opts['bc_upper'])
     solver.aux_bc_lower = list(opts.get('aux_bc_lower', (1, 1)))
     solver.aux_bc_upper = list(opts.get('aux_bc_upper', (1, 1)))
+    solver.device_loop = device_loop
     if custom:
         solver.compute_gauge_values = lambda q, a: q      # a user function: the per-step host path
     claw = pc.Controller()
@@ -56,9 +57,13 @@ def _run(outdir, custom):
     return files, [np.array(g) for g in state.gauge_data], claw
 
 
-def test_device_recorded_gauges_equal_the_per_step_path(tmp_path):
-    print("gauges: vc acoustics classic", flush=True)
-    dev_files, dev_data, claw = _run(tmp_path / 'dev', custom=False)
+@pytest.mark.parametrize("device_loop", [True, False])
+def test_device_recorded_gauges_equal_the_per_step_path(tmp_path, device_loop):
+    """Default compute_gauge_values: recorded inside the device step loop
+    (device_loop) or gathered on the device after each host-loop step; a user
+    compute_gauge_values: the per-step host path."""
+    print("gauges: vc acoustics classic, device loop", device_loop, flush=True)
+    dev_files, dev_data, claw = _run(tmp_path / 'dev', custom=False, device_loop=device_loop)
     host_files, host_data, _ = _run(tmp_path / 'host', custom=True)
     assert len(dev_files) == len(GAUGES)
     assert dev_files == host_files

@@ -70,15 +70,18 @@ def test_invalid_arguments_return_einval_without_gpu():
     assert adv(rule, s0, s1, c0, c1, qa, qb, 62, None) == _lib.CLAW_EINVAL        # not whole 4-byte words
     assert adv(rule, s0, s1, c0, c1, qa, ctypes.c_void_p(8200), 64, None) == _lib.CLAW_EINVAL   # misaligned
     assert L.claw2d_classic_loop(ctx.handle, g, m, r, bc, _lib.F64, qa, qb, None, rule, s0, c0, 64, 0, -1,
-                                 None) == _lib.CLAW_EINVAL
+                                 None, None) == _lib.CLAW_EINVAL
     assert L.claw2d_sharpclaw_loop(ctx.handle, g, r, bc, _lib.F64, qa, qb, qa, qa, qa, None, rule, s0, c0, 64, 0, 0,
-                                   None) == _lib.CLAW_EINVAL                     # the SSP104 loop needs ncfl = 10
+                                   None, None) == _lib.CLAW_EINVAL                     # the SSP104 loop needs ncfl = 10
     rule10 = _lib.ClawLoopRule(1.0, 0.9, 1e99, 1.0, 0.0, 10, 10)
     assert adv(rule10, s0, s1, c0, c1, qa, qb, 64, None) == _lib.CLAW_EINVAL      # 10-value slots overlap
     assert adv(_lib.ClawLoopRule(1.0, 0.9, 1e99, 1.0, 0.0, 10, 17), s0, s1, c0, ctypes.c_void_p(1024), qa, qb, 64,
                None) == _lib.CLAW_EINVAL
     assert L.claw2d_classic_loop(ctx.handle, g, m, r, bc, _lib.F64, qa, qb, None, rule, s0, c0, 64, 0, 0,
-                                 None) == _lib.CLAW_OK                           # nothing to launch
+                                 None, None) == _lib.CLAW_OK                     # nothing to launch
+    G = _lib.ClawLoopGauges(None, None, None, 3, 0, 16, 0)
+    assert L.claw2d_classic_loop(ctx.handle, g, m, r, bc, _lib.F32, qa, qb, None, rule, s0, c0, 64, 0, 0,
+                                 ctypes.byref(G), None) == _lib.CLAW_EINVAL     # gauges are recorded in FP64
     bad = _lib.ClawGrid(8, 8, 3, 0, 13, 196, 0.1, 0.1)   # ldq < mx + 2 ng
     assert L.claw2d_euler_radial_dq(bad, _lib.F64, 5, q, a, d, 0.1, 0.4, None) == _lib.CLAW_EINVAL
 
