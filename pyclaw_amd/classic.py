@@ -173,8 +173,9 @@ class ClawSolver2D(ClawSolver):
         grid = self._grid_struct(state)
         prec = self._precision(state)
         aux = state._aux.buf.data_ptr() if state._aux is not None else None
-        self._dt_dev.fill_(float(self.dt))
-        self._autotune_once(state, grid, bc, prec, aux, torch.cuda.current_stream().cuda_stream)
+        if self.autotune:
+            self._dt_dev.fill_(float(self.dt))
+            self._autotune_once(state, grid, bc, prec, aux, torch.cuda.current_stream().cuda_stream)
         fn = _lib.lib().claw2d_classic_loop
         ctx, m, r = self._ctx.handle, self._method_struct, self._rp_struct
 

@@ -134,6 +134,7 @@ class Solver(object):
         # evolve_to_time queues its steps with the accept/reject bookkeeping on
         # the device where no step needs the host; results do not change.
         self.device_loop = True
+        self._loop_ws = None
         if riemann_solver is not None:
             self.rp = riemann_solver
             self.num_eqn = riemann_solver.num_eqn
@@ -401,9 +402,17 @@ class Solver(object):
         rule = _lib.ClawLoopRule(self.cfl_max, self.cfl_desired, self.dt_max, tend, tstart, self.max_steps, ncfl,
                                  1 if self.dt_variable else 0, 0)
         size = ctypes.sizeof(_lib.ClawLoopState)
-        sbuf = torch.zeros(2 * size, dtype=torch.uint8, device=dev)
-        sbuf[:size].copy_(torch.frombuffer(bytearray(st), dtype=torch.uint8))
-        cfl = torch.zeros(2 * ncfl, dtype=torch.float64, device=dev)
+        ws = self._loop_ws
+        if ws is None or ws['dev'] != dev or ws['ncfl'] != ncfl:
+            # kept between calls: the loop state, the Courant slots and a pinned upload buffer
+            ws = dict(dev=dev, ncfl=ncfl, sbuf=torch.zeros(2 * size, dtype=torch.uint8, device=dev),
+                      cfl=torch.zeros(2 * ncfl, dtype=torch.float64, device=dev),
+                      host=torch.empty(size, dtype=torch.uint8, pin_memory=True), gauges=None)
+            self._loop_ws = ws
+        sbuf, cfl = ws['sbuf'], ws['cfl']
+        ctypes.memmove(ws['host'].data_ptr(), ctypes.addressof(st), size)
+        sbuf[:size].copy_(ws['host'], non_blocking=True)
+        cfl.zero_()
         bufs = [state._q.buf, self._qalt]
         qp = [b.data_ptr() for b in bufs]
         nbytes = bufs[0].numel() * bufs[0].element_size()
@@ -415,10 +424,14 @@ class Solver(object):
             # recorded after each accepted step on the device, written out per chunk
             self.flush_gauge_values(solution)
             ne, rows, ldq = state.num_eqn, bufs[0].shape[1], bufs[0].shape[2]
-            offs = torch.tensor([m * rows * ldq + (iy + NG_STORE) * ldq + (ix + NG_STORE)
-                                 for (ix, iy) in gauges for m in range(ne)], dtype=torch.int64, device=dev)
-            vals = torch.empty((self._GAUGE_CAP, offs.numel()), dtype=torch.float64, device=dev)
-            times = torch.empty(self._GAUGE_CAP, dtype=torch.float64, device=dev)
+            gkey = (tuple(map(tuple, gauges)), tuple(bufs[0].shape))
+            if ws['gauges'] is None or ws['gauges'][0] != gkey:
+                offs = torch.tensor([m * rows * ldq + (iy + NG_STORE) * ldq + (ix + NG_STORE)
+                                     for (ix, iy) in gauges for m in range(ne)], dtype=torch.int64, device=dev)
+                ws['gauges'] = (gkey, offs, torch.empty((self._GAUGE_CAP, offs.numel()), dtype=torch.float64,
+                                                        device=dev),
+                                torch.empty(self._GAUGE_CAP, dtype=torch.float64, device=dev))
+            _, offs, vals, times = ws['gauges']
             G = _lib.ClawLoopGauges(offs.data_ptr(), vals.data_ptr(), times.data_ptr(), offs.numel(), 0,
                                     self._GAUGE_CAP, 0)
         k = 0
