@@ -16,9 +16,9 @@ pytestmark = pytest.mark.gpu
 GAUGES = [(0.0, 0.0), (0.31, -0.42), (-0.77, 0.5), (0.5, 0.9)]
 
 
-def _run(outdir, custom, device_loop=True):
+def _run(outdir, custom, device_loop=True, kind='classic'):
     import pyclaw_amd as pc
-    q0, aux, opts = apps.acoustics_variable('classic')
+    q0, aux, opts = apps.acoustics_variable(kind)
     x = pc.Dimension('x', opts['lower'][0], opts['upper'][0], opts['mx'])
     y = pc.Dimension('y', opts['lower'][1], opts['upper'][1], opts['my'])
     domain = pc.Domain([x, y])
@@ -28,9 +28,12 @@ def _run(outdir, custom, device_loop=True):
     state.aux[...] = aux
     state.keep_gauges = True
     domain.grid.add_gauges(GAUGES)
-    solver = pc.ClawSolver2D(getattr(pc.riemann, RP[opts['rp']]))
-    solver.dimensional_split = opts.get('dimensional_split', True)
-    solver.transverse_waves = opts.get('transverse', 1)
+    if kind == 'classic':
+        solver = pc.ClawSolver2D(getattr(pc.riemann, RP[opts['rp']]))
+        solver.dimensional_split = opts.get('dimensional_split', True)
+        solver.transverse_waves = opts.get('transverse', 1)
+    else:
+        solver = pc.SharpClawSolver2D(getattr(pc.riemann, RP[opts['rp']]))
     solver.limiters = opts['limiters']
     solver.cfl_max, solver.cfl_desired = opts['cfl_max'], opts['cfl_desired']
     solver.dt_initial = opts['dt_initial']
@@ -57,14 +60,14 @@ def _run(outdir, custom, device_loop=True):
     return files, [np.array(g) for g in state.gauge_data], claw
 
 
-@pytest.mark.parametrize("device_loop", [True, False])
-def test_device_recorded_gauges_equal_the_per_step_path(tmp_path, device_loop):
+@pytest.mark.parametrize("kind,device_loop", [('classic', True), ('classic', False), ('sharpclaw', True)])
+def test_device_recorded_gauges_equal_the_per_step_path(tmp_path, kind, device_loop):
     """Default compute_gauge_values: recorded inside the device step loop
     (device_loop) or gathered on the device after each host-loop step; a user
     compute_gauge_values: the per-step host path."""
-    print("gauges: vc acoustics classic, device loop", device_loop, flush=True)
-    dev_files, dev_data, claw = _run(tmp_path / 'dev', custom=False, device_loop=device_loop)
-    host_files, host_data, _ = _run(tmp_path / 'host', custom=True)
+    print("gauges: vc acoustics", kind, "device loop", device_loop, flush=True)
+    dev_files, dev_data, claw = _run(tmp_path / 'dev', custom=False, device_loop=device_loop, kind=kind)
+    host_files, host_data, _ = _run(tmp_path / 'host', custom=True, kind=kind)
     assert len(dev_files) == len(GAUGES)
     assert dev_files == host_files
     assert len(dev_data) == len(host_data) == len(GAUGES)
