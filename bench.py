@@ -59,7 +59,8 @@ def parse():
     ap.add_argument('--autotune', action='store_true', help="autotune the launch shape even where a default is pinned")
     ap.add_argument('--cpu-sample-steps', type=int, default=3)
     ap.add_argument('--workload', default='euler',
-                    choices=['euler', 'batched', 'sharpclaw', 'acoustics', 'acoustics_app', 'adjoint', 'copy'],
+                    choices=['euler', 'batched', 'sharpclaw', 'sharpclaw_app', 'acoustics', 'acoustics_app', 'adjoint',
+                             'copy'],
                     help="euler: the headline line; the others measure the other configurations")
     ap.add_argument('--instances', type=int, default=512)
     ap.add_argument('--adjoint-steps', type=int, default=2000)
@@ -627,6 +628,68 @@ def bench_acoustics_app(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sharpclaw_app(args):
+    """SharpClaw shallow water 1024x1024 periodic (WENO5 + SSP104, Roe, g=1)
+    as an application: one Controller.run output interval of about 500 steps at
+    Courant number 0.5 through the drop-in API, FP64.  Milliseconds per
+    accepted step with the step loop on the device (the default) and on the
+    host, and the relative change of total h, hu, hv."""
+    import time
+    import torch
+    import pyclaw_amd as pc
+    from pyclaw_amd.problems import shallow_hump
+    mx = my = 1024
+    rng = np.random.default_rng(7)
+    x0 = 0.25 + 0.5 * rng.random(2)
+    q0 = shallow_hump(mx, my, x0)
+    smax = float(np.max(np.sqrt(q0[0])))
+    tfinal = 500 * 0.5 * (1.0 / mx) / smax
+
+    def run(device_loop):
+        x = pc.Dimension('x', 0., 1., mx)
+        y = pc.Dimension('y', 0., 1., my)
+        dom = pc.Domain([x, y])
+        st = pc.State(dom, 3)
+        st.problem_data['grav'] = 1.0
+        st.q[...] = q0
+        s = pc.SharpClawSolver2D(pc.riemann.rp2_shallow_roe_with_efix)
+        s.cfl_max, s.cfl_desired = 0.6, 0.5
+        s.dt_initial = 0.5 * (1.0 / mx) / smax
+        s.bc_lower = [pc.BC.periodic, pc.BC.periodic]
+        s.bc_upper = [pc.BC.periodic, pc.BC.periodic]
+        s.device_loop = device_loop
+        claw = pc.Controller()
+        claw.solution = pc.Solution(st, dom)
+        claw.solver = s
+        claw.output_format = None
+        claw.keep_copy = True
+        claw.tfinal = tfinal
+        claw.num_output_times = 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        claw.run()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, s.status['numsteps'], claw.frames[-1].state.q
+
+    run(True), run(False)
+    res = {mode: run(mode) for mode in (True, False)}
+    el, n, qf = res[True]
+    tot0, tot1 = q0.reshape(3, -1).sum(axis=1), qf.reshape(3, -1).sum(axis=1)
+    scale = np.array([tot0[0], tot0[0], tot0[0]])
+    ms = el / n * 1e3
+    out = {"metric": "ms per accepted step (application loop)", "value": ms, "unit": "ms/step", "n_gpus": 1,
+           "steps": n, "warmup": 1, "ms_per_step": ms, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "SharpClaw shallow water 1024x1024 periodic, WENO5 + SSP104, Controller.run "
+                                  "of about 500 steps at CFL 0.5", "global_batch": 1, "parallelism": "single GPU"},
+           "extra": {"gcell_updates_per_s": mx * my / (ms * 1e-3) / 1e9,
+                     "host_loop_ms_per_step": res[False][0] / res[False][1] * 1e3,
+                     "device_loop_speedup": res[False][0] / el,
+                     "frames_identical": bool(np.array_equal(qf, res[False][2])),
+                     "relative_change_of_totals_h_hu_hv": (np.abs(tot1 - tot0) / scale).tolist()}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_adjoint(args):
     """Gradient of the receiver misfit of 2D vc acoustics at 512x512 (2000 steps
     at dt = 0.8 dx / max c, MC limiter, trans_inc, extrapolation boundaries;
@@ -752,7 +815,8 @@ def bench_copy(args):
 
 
 OTHER = {'batched': bench_batched, 'sharpclaw': bench_sharpclaw, 'acoustics': bench_acoustics,
-         'acoustics_app': bench_acoustics_app, 'adjoint': bench_adjoint, 'copy': bench_copy}
+         'acoustics_app': bench_acoustics_app, 'sharpclaw_app': bench_sharpclaw_app, 'adjoint': bench_adjoint,
+         'copy': bench_copy}
 
 
 if __name__ == '__main__':

@@ -321,9 +321,18 @@ template <> Euler5<float>::Params make_sharp_params<Euler5<float>, float>(const 
 template <> ShallowRoe<double>::Params make_sharp_params<ShallowRoe<double>, double>(const ClawRiemann *rp) { return {rp->p[0]}; }
 template <> ShallowRoe<float>::Params make_sharp_params<ShallowRoe<float>, float>(const ClawRiemann *rp) { return {(float)rp->p[0]}; }
 
-static int sharp_band_height(int rows, int strips_x) {
+static int sharp_band_height(int rows, int strips_x, int &w) {
+    const long long work = (long long)rows * strips_x;
+    if (work >= 7LL * 2048) {
+        // enough 7-row bands for two or more wavefronts per SIMD: the build for
+        // three per SIMD with bands of about 3072 wavefronts (1024x1024: 7 rows,
+        // what the tuner picks there)
+        w = 3;
+        const int h = (int)std::max<long long>(7, (work + 3071) / 3072);
+        return std::min(h, std::max(rows, 1));
+    }
     long long target = 2048;
-    int h = (int)std::max<long long>(16, ((long long)rows * strips_x + target - 1) / target);
+    int h = (int)std::max<long long>(16, (work + target - 1) / target);
     return std::min(h, std::max(rows, 1));
 }
 
@@ -349,7 +358,7 @@ static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int
         auto it = ctx->band.find(key);
         if (it != ctx->band.end()) { w = it->second.second; return std::min(it->second.first, grid->my); }
     }
-    return sharp_band_height(grid->my, strips);
+    return sharp_band_height(grid->my, strips, w);
 }
 
 template <class RP, typename R>
