@@ -28,6 +28,25 @@ RP_ACOUSTICS, RP_VC_ACOUSTICS, RP_EULER, RP_SHALLOW = 0, 1, 2, 3
 _lib = None
 _ref_classic = None
 _ref_sharp = None
+_threads = 1
+
+
+def set_threads(n):
+    """Threads for step2 / step2ds / sharp_dq (OpenMP over row and column bands;
+    results are bit-identical to one thread).  Returns the previous value."""
+    global _threads
+    old, _threads = _threads, max(1, int(n))
+    return old
+
+
+def host_threads(cap=16):
+    """The host's usable cores (the process's CPU affinity), at most cap."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(cap, n))
+
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -52,6 +71,13 @@ def lib():
         L.orc_step2ds.restype = ctypes.c_double
         L.orc_step2ds.argtypes = [ctypes.c_int] * 8 + [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                                        ctypes.c_double, _ip, _ip, ctypes.c_int, ctypes.c_int]
+        L.orc_step2_mt.restype = ctypes.c_double
+        L.orc_step2_mt.argtypes = L.orc_step2.argtypes + [ctypes.c_int]
+        L.orc_step2ds_mt.restype = ctypes.c_double
+        L.orc_step2ds_mt.argtypes = L.orc_step2ds.argtypes + [ctypes.c_int]
+        L.orc_sharp_dq_mt.restype = ctypes.c_double
+        L.orc_sharp_dq_mt.argtypes = [ctypes.c_int] * 7 + [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                                           ctypes.c_double, ctypes.c_int]
         L.orc_fill_bc.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip,
                                   ctypes.c_int]
         L.orc_sharp_dq.restype = ctypes.c_double
@@ -128,9 +154,9 @@ def step2(rp, qbc, auxbc, dx, dy, dt, method, mthlim, num_waves, use_fwave=False
     meth = np.asarray(method, dtype=np.int32).copy()
     meth[6] = na
     lim = np.asarray(mthlim, dtype=np.int32)
-    cfl = lib().orc_step2(rp, max(mx, my), ne, num_waves, na, ng, mx, my, _ptr(qold), _ptr(qnew),
-                          _ptr(aux), dx, dy, dt, meth.ctypes.data_as(_ip), lim.ctypes.data_as(_ip),
-                          1 if use_fwave else 0)
+    cfl = lib().orc_step2_mt(rp, max(mx, my), ne, num_waves, na, ng, mx, my, _ptr(qold), _ptr(qnew),
+                             _ptr(aux), dx, dy, dt, meth.ctypes.data_as(_ip), lim.ctypes.data_as(_ip),
+                             1 if use_fwave else 0, _threads)
     return qnew, cfl
 
 
@@ -143,9 +169,9 @@ def step2ds(rp, qold, qnew, auxbc, dx, dy, dt, method, mthlim, num_waves, ids, u
     meth = np.asarray(method, dtype=np.int32).copy()
     meth[6] = na
     lim = np.asarray(mthlim, dtype=np.int32)
-    return lib().orc_step2ds(rp, max(mx, my), ne, num_waves, na, ng, mx, my, _ptr(qold), _ptr(qnew),
-                             _ptr(aux), dx, dy, dt, meth.ctypes.data_as(_ip), lim.ctypes.data_as(_ip),
-                             ids, 1 if use_fwave else 0)
+    return lib().orc_step2ds_mt(rp, max(mx, my), ne, num_waves, na, ng, mx, my, _ptr(qold), _ptr(qnew),
+                                _ptr(aux), dx, dy, dt, meth.ctypes.data_as(_ip), lim.ctypes.data_as(_ip),
+                                ids, 1 if use_fwave else 0, _threads)
 
 
 def sharp_dq(rp, qbc, auxbc, dx, dy, dt, num_waves):
@@ -156,7 +182,8 @@ def sharp_dq(rp, qbc, auxbc, dx, dy, dt, num_waves):
     mx, my = nxp - 2 * ng, nyp - 2 * ng
     aux, na = _aux_or_dummy(auxbc, nxp, nyp)
     dq = np.zeros_like(q, order="F")
-    cfl = lib().orc_sharp_dq(rp, ne, num_waves, na, ng, mx, my, _ptr(q), _ptr(aux), _ptr(dq), dx, dy, dt)
+    cfl = lib().orc_sharp_dq_mt(rp, ne, num_waves, na, ng, mx, my, _ptr(q), _ptr(aux), _ptr(dq), dx, dy, dt,
+                                _threads)
     return dq[:, ng:-ng, ng:-ng].copy(order="F"), cfl
 
 
@@ -202,6 +229,42 @@ def rpn2(rp, ixy, qlo, qhi, auxlo=None, auxhi=None, num_waves=None):
     # interface between slice cells c-1 and c lives at Fortran index c (array index c-1+mbc)
     idx = np.arange(k) * 2 + 1 + mbc
     return wave[:, :, idx], s[:, idx], amdq[:, idx], apdq[:, idx]
+
+
+def rpt2(rp, ixy, qlo, qhi, asdq, imp=1, aux1=None, aux2=None, aux3=None, num_waves=None):
+    """Transverse solve at single interfaces, called as flux2.f90:184-185 calls
+    it: qlo/qhi (meqn, k) are the states left and right of k interfaces,
+    asdq (meqn, k) the fluctuation being split, imp 1 (amdq, goes to the left
+    cell) or 2 (apdq, right cell).  aux1/aux2/aux3 are (naux, 2, k): for each
+    interface the aux of its (left, right) cell in the row below, the row
+    itself and the row above.  Returns (bmasdq, bpasdq), each (meqn, k)."""
+    qlo = np.asarray(qlo, dtype=np.float64); qhi = np.asarray(qhi, dtype=np.float64)
+    ne, k = qlo.shape
+    nw = num_waves
+    mbc = 1
+    n = 2 * k
+    q = np.zeros((ne, n + 2 * mbc), order='F')
+    q[:, mbc:mbc + n:2] = qlo
+    q[:, mbc + 1:mbc + n:2] = qhi
+    idx = np.arange(k) * 2 + 1 + mbc
+    a = np.zeros((ne, n + 2 * mbc), order='F')
+    a[:, idx] = asdq
+    auxs = []
+    for ax in (aux1, aux2, aux3):
+        if ax is None:
+            arr = np.ones((2, n + 2 * mbc), order='F')
+        else:
+            ax = np.asarray(ax, dtype=np.float64)
+            arr = np.ones((ax.shape[0], n + 2 * mbc), order='F')
+            arr[:, mbc:mbc + n:2] = ax[:, 0]
+            arr[:, mbc + 1:mbc + n:2] = ax[:, 1]
+        auxs.append(arr)
+    na = auxs[0].shape[0]
+    bm = np.zeros((ne, n + 2 * mbc), order='F')
+    bp = np.zeros((ne, n + 2 * mbc), order='F')
+    lib().orc_rpt2(rp, ixy, n, ne, nw, mbc, n, _ptr(q), _ptr(q), _ptr(auxs[0]), _ptr(auxs[1]), _ptr(auxs[2]),
+                   imp, _ptr(a), _ptr(bm), _ptr(bp), na)
+    return bm[:, idx], bp[:, idx]
 
 
 # ---------------------------------------------------------------------------

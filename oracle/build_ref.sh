@@ -1,6 +1,8 @@
 #!/bin/bash
 # Compiles the reference's own 2D Fortran kernels, where they lie under
-# /root/reference, into oracle/_ref/ (test infrastructure only; never shipped).
+# /root/reference, into oracle/_ref/ (test infrastructure only).  oracle/_ref/ is
+# git-ignored; like the other built libraries it is part of the tree copied for
+# GPU runs, where nothing loads it: the tests that use it are CPU tests.
 #   libref_classic.so   : classic/{step2,step2ds,flux2,limiter,philim}.f90 +
 #                         development/rp_approaches/{rpn2,rpt2}_euler_5wave.f
 #   libref_sharpclaw.so : sharpclaw/{ClawParams,weno,reconstruct,tfluct,evec,

@@ -204,9 +204,10 @@ void orc_rpt2_vc_acoustics_(const int *ixy, const int *maxm, const int *meqn,
 /* rpt2_euler_5wave.f (transverse) -- same expressions, same order.   */
 /* The Roe averages are kept per slice (reference: common /comroe/).  */
 /* ------------------------------------------------------------------ */
-static double *roe_u2v2, *roe_u, *roe_v, *roe_enth, *roe_a, *roe_g1a2, *roe_euv;
-static int roe_cap;
-static int roe_off;
+/* per thread: the multi-threaded sweeps run one slice per thread at a time */
+static __thread double *roe_u2v2, *roe_u, *roe_v, *roe_enth, *roe_a, *roe_g1a2, *roe_euv;
+static __thread int roe_cap;
+static __thread int roe_off;
 
 static void roe_reserve(int n, int mbc)
 {
@@ -718,13 +719,29 @@ static void free_work(flux_work_t *wk)
     free(wk->cqxx); free(wk->bmasdq); free(wk->bpasdq);
 }
 
-/* Shared sweep driver.  mode 0: unsplit step2; 1: step2ds ids=1; 2: step2ds ids=2 */
-static double classic_sweeps(int mode, int maxm, int meqn, int mwaves, int num_aux, int mbc,
-                             int mx, int my, const double *qold, double *qnew, const double *aux,
-                             double dx, double dy, double dt, const int *method,
-                             const int *mthlim, int use_fwave, rpn2_fn rpn2, rpt2_fn rpt2)
+/* Shared sweep driver.  mode 0: unsplit step2; 1: step2ds ids=1; 2: step2ds ids=2.
+ * dir 1 runs the x sweeps (slices are rows j, step2.f90:92-165 / step2ds.f90:96-167) over
+ * slices s0..s1; dir 2 the y sweeps (slices are columns i, step2.f90:173-248 /
+ * step2ds.f90:177-252).  Only updates of rows (dir 1) or columns (dir 2) w0..w1 are
+ * applied, so that disjoint write ranges can run on different threads: a range that
+ * also runs the one slice on each side of it receives, for each of its cells, the
+ * contributions of the slices j-1, j, j+1 in the reference's order (step2ds slices
+ * update only their own row or column). */
+typedef struct {
+    int mode, maxm, meqn, mwaves, num_aux, mbc, mx, my;
+    const double *qold; double *qnew; const double *aux;
+    double dx, dy, dt;
+    const int *method, *mthlim;
+    int use_fwave;
+    rpn2_fn rpn2; rpt2_fn rpt2;
+} sweep_args_t;
+
+static double sweep_dir(const sweep_args_t *A, int dir, int s0, int s1, int w0, int w1)
 {
-    const int ne = meqn;
+    const int mode = A->mode, maxm = A->maxm, ne = A->meqn, mwaves = A->mwaves, num_aux = A->num_aux;
+    const int mbc = A->mbc, mx = A->mx, my = A->my;
+    const double *qold = A->qold, *aux = A->aux;
+    double *qnew = A->qnew;
     const int na = num_aux > 0 ? num_aux : 1;
     size_t n = (size_t)(maxm + 2 * mbc);
     double *q1d = calloc(n * ne, sizeof(double));
@@ -739,17 +756,17 @@ static double classic_sweeps(int mode, int maxm, int meqn, int mwaves, int num_a
     flux_work_t wk;
     alloc_work(&wk, maxm, mbc, ne, mwaves);
     double cfl = 0.0, cfl1d;
-    const double dtdx = dt / dx, dtdy = dt / dy;
+    const double dtdx = A->dt / A->dx, dtdy = A->dt / A->dy;
     for (int i = 1 - mbc; i <= maxm + mbc; ++i) {
         dtdx1d[i + mbc - 1] = dtdx;
         dtdy1d[i + mbc - 1] = dtdy;
     }
 #define G(m, k, i) gadd[(((i) + mbc - 1) * 2 + (k)) * ne + (m)]
 #define AUXF(ma, i, j) (QF(aux, na, i, j))[ma]
+#define WR(r) ((r) >= w0 && (r) <= w1)
 
-    if (mode == 0 || mode == 1) {
-        int j0 = (mode == 0) ? 0 : 1 - mbc, j1 = (mode == 0) ? my + 1 : my + mbc;
-        for (int j = j0; j <= j1; ++j) {
+    if (dir == 1) {
+        for (int j = s0; j <= s1; ++j) {
             for (int i = 1 - mbc; i <= mx + mbc; ++i)
                 for (int m = 0; m < ne; ++m) V(q1d, ne, m, i) = QF(qold, ne, i, j)[m];
             if (num_aux > 0) {
@@ -760,29 +777,30 @@ static double classic_sweeps(int mode, int maxm, int meqn, int mwaves, int num_a
                         if (mode == 0 || j != my + mbc) V(aux3, na, ma, i) = AUXF(ma, i, j + 1);
                     }
             }
-            flux2(1, maxm, ne, mwaves, na, mbc, mx, q1d, dtdx1d, aux1, aux2, aux3, method, mthlim,
-                  qadd, fadd, gadd, &cfl1d, &wk, rpn2, rpt2, use_fwave);
+            flux2(1, maxm, ne, mwaves, na, mbc, mx, q1d, dtdx1d, aux1, aux2, aux3, A->method, A->mthlim,
+                  qadd, fadd, gadd, &cfl1d, &wk, A->rpn2, A->rpt2, A->use_fwave);
             cfl = dmax1(cfl, cfl1d);
             if (mode == 0) {
                 for (int m = 0; m < ne; ++m)
                     for (int i = 1; i <= mx; ++i) {
-                        QF(qnew, ne, i, j)[m] = QF(qnew, ne, i, j)[m] + V(qadd, ne, m, i)
-                            - dtdx * (V(fadd, ne, m, i + 1) - V(fadd, ne, m, i))
-                            - dtdy * (G(m, 1, i) - G(m, 0, i));
-                        QF(qnew, ne, i, j - 1)[m] = QF(qnew, ne, i, j - 1)[m] - dtdy * G(m, 0, i);
-                        QF(qnew, ne, i, j + 1)[m] = QF(qnew, ne, i, j + 1)[m] + dtdy * G(m, 1, i);
+                        if (WR(j))
+                            QF(qnew, ne, i, j)[m] = QF(qnew, ne, i, j)[m] + V(qadd, ne, m, i)
+                                - dtdx * (V(fadd, ne, m, i + 1) - V(fadd, ne, m, i))
+                                - dtdy * (G(m, 1, i) - G(m, 0, i));
+                        if (WR(j - 1))
+                            QF(qnew, ne, i, j - 1)[m] = QF(qnew, ne, i, j - 1)[m] - dtdy * G(m, 0, i);
+                        if (WR(j + 1))
+                            QF(qnew, ne, i, j + 1)[m] = QF(qnew, ne, i, j + 1)[m] + dtdy * G(m, 1, i);
                     }
-            } else {
+            } else if (WR(j)) {
                 for (int i = 1; i <= mx; ++i)
                     for (int m = 0; m < ne; ++m)
                         QF(qnew, ne, i, j)[m] = QF(qnew, ne, i, j)[m] + V(qadd, ne, m, i)
                             - dtdx * (V(fadd, ne, m, i + 1) - V(fadd, ne, m, i));
             }
         }
-    }
-    if (mode == 0 || mode == 2) {
-        int i0 = (mode == 0) ? 0 : 1 - mbc, i1 = (mode == 0) ? mx + 1 : mx + mbc;
-        for (int i = i0; i <= i1; ++i) {
+    } else {
+        for (int i = s0; i <= s1; ++i) {
             for (int j = 1 - mbc; j <= my + mbc; ++j)
                 for (int m = 0; m < ne; ++m) V(q1d, ne, m, j) = QF(qold, ne, i, j)[m];
             if (num_aux > 0) {
@@ -793,19 +811,22 @@ static double classic_sweeps(int mode, int maxm, int meqn, int mwaves, int num_a
                         if (mode == 0 || i != mx + mbc) V(aux3, na, ma, j) = AUXF(ma, i + 1, j);
                     }
             }
-            flux2(2, maxm, ne, mwaves, na, mbc, my, q1d, dtdy1d, aux1, aux2, aux3, method, mthlim,
-                  qadd, fadd, gadd, &cfl1d, &wk, rpn2, rpt2, use_fwave);
+            flux2(2, maxm, ne, mwaves, na, mbc, my, q1d, dtdy1d, aux1, aux2, aux3, A->method, A->mthlim,
+                  qadd, fadd, gadd, &cfl1d, &wk, A->rpn2, A->rpt2, A->use_fwave);
             cfl = dmax1(cfl, cfl1d);
             if (mode == 0) {
                 for (int m = 0; m < ne; ++m)
                     for (int j = 1; j <= my; ++j) {
-                        QF(qnew, ne, i, j)[m] = QF(qnew, ne, i, j)[m] + (V(qadd, ne, m, j)
-                            - dtdy * (V(fadd, ne, m, j + 1) - V(fadd, ne, m, j))
-                            - dtdx * (G(m, 1, j) - G(m, 0, j)));
-                        QF(qnew, ne, i - 1, j)[m] = QF(qnew, ne, i - 1, j)[m] - dtdx * G(m, 0, j);
-                        QF(qnew, ne, i + 1, j)[m] = QF(qnew, ne, i + 1, j)[m] + dtdx * G(m, 1, j);
+                        if (WR(i))
+                            QF(qnew, ne, i, j)[m] = QF(qnew, ne, i, j)[m] + (V(qadd, ne, m, j)
+                                - dtdy * (V(fadd, ne, m, j + 1) - V(fadd, ne, m, j))
+                                - dtdx * (G(m, 1, j) - G(m, 0, j)));
+                        if (WR(i - 1))
+                            QF(qnew, ne, i - 1, j)[m] = QF(qnew, ne, i - 1, j)[m] - dtdx * G(m, 0, j);
+                        if (WR(i + 1))
+                            QF(qnew, ne, i + 1, j)[m] = QF(qnew, ne, i + 1, j)[m] + dtdx * G(m, 1, j);
                     }
-            } else {
+            } else if (WR(i)) {
                 for (int j = 1; j <= my; ++j)
                     for (int m = 0; m < ne; ++m)
                         QF(qnew, ne, i, j)[m] = QF(qnew, ne, i, j)[m] + V(qadd, ne, m, j)
@@ -815,9 +836,53 @@ static double classic_sweeps(int mode, int maxm, int meqn, int mwaves, int num_a
     }
 #undef G
 #undef AUXF
+#undef WR
     free_work(&wk);
     free(q1d); free(qadd); free(fadd); free(gadd); free(dtdx1d); free(dtdy1d);
     free(aux1); free(aux2); free(aux3);
+    return cfl;
+}
+
+/* Slices of each sweep: step2 runs rows/columns 0..n+1 (step2.f90:92,173); step2ds all
+ * of them, ghosts included (step2ds.f90:96,177).  Updated rows/columns lie in 1-mbc..n+mbc. */
+static void sweep_slices(int mode, int mbc, int n, int *s0, int *s1)
+{
+    *s0 = (mode == 0) ? 0 : 1 - mbc;
+    *s1 = (mode == 0) ? n + 1 : n + mbc;
+}
+
+static double classic_sweeps(const sweep_args_t *A, int nthreads)
+{
+    double cfl = 0.0;
+    for (int dir = 1; dir <= 2; ++dir) {
+        if ((A->mode == 1 && dir == 2) || (A->mode == 2 && dir == 1)) continue;
+        const int n = dir == 1 ? A->my : A->mx;
+        int s0, s1;
+        sweep_slices(A->mode, A->mbc, n, &s0, &s1);
+        const int w0 = 1 - A->mbc, w1 = n + A->mbc;
+        if (nthreads <= 1) {
+            cfl = dmax1(cfl, sweep_dir(A, dir, s0, s1, w0, w1));
+            continue;
+        }
+        /* disjoint write ranges of about equal size; each also runs the slice on either side */
+        const int nw = w1 - w0 + 1;
+        const int nb = nthreads < nw ? nthreads : nw;
+        double c = 0.0;
+#pragma omp parallel for num_threads(nb) schedule(static, 1) reduction(max : c)
+        for (int b = 0; b < nb; ++b) {
+            const int a0 = w0 + (int)((long long)nw * b / nb), a1 = w0 + (int)((long long)nw * (b + 1) / nb) - 1;
+            /* step2 slices update the rows/columns on either side of them, step2ds
+             * slices only their own (and in the aliased ids=2 call a neighbour's
+             * column may already be updated, so it must not be re-read) */
+            const int h = A->mode == 0 ? 1 : 0;
+            int t0 = a0 - h < s0 ? s0 : a0 - h, t1 = a1 + h > s1 ? s1 : a1 + h;
+            if (t0 <= t1) {
+                const double cb = sweep_dir(A, dir, t0, t1, a0, a1);
+                c = c > cb ? c : cb;
+            }
+        }
+        cfl = dmax1(cfl, c);
+    }
     return cfl;
 }
 
@@ -833,23 +898,42 @@ static void pick_rp(int rp, rpn2_fn *n, rpt2_fn *t)
 }
 
 /* One unsplit step: qnew (which must equal qold on entry) is updated in place. */
+double orc_step2_mt(int rp, int maxm, int meqn, int mwaves, int num_aux, int mbc, int mx, int my,
+                    const double *qold, double *qnew, const double *aux, double dx, double dy,
+                    double dt, const int *method, const int *mthlim, int use_fwave, int nthreads)
+{
+    sweep_args_t A = {0, maxm, meqn, mwaves, num_aux, mbc, mx, my, qold, qnew, aux, dx, dy, dt,
+                      method, mthlim, use_fwave, 0, 0};
+    pick_rp(rp, &A.rpn2, &A.rpt2);
+    return classic_sweeps(&A, nthreads);
+}
+
 double orc_step2(int rp, int maxm, int meqn, int mwaves, int num_aux, int mbc, int mx, int my,
                  const double *qold, double *qnew, const double *aux, double dx, double dy,
                  double dt, const int *method, const int *mthlim, int use_fwave)
 {
-    rpn2_fn n; rpt2_fn t; pick_rp(rp, &n, &t);
-    return classic_sweeps(0, maxm, meqn, mwaves, num_aux, mbc, mx, my, qold, qnew, aux, dx, dy,
-                          dt, method, mthlim, use_fwave, n, t);
+    return orc_step2_mt(rp, maxm, meqn, mwaves, num_aux, mbc, mx, my, qold, qnew, aux, dx, dy, dt,
+                        method, mthlim, use_fwave, 1);
 }
 
-/* One call of step2ds (ids = 1 or 2). */
+/* One call of step2ds (ids = 1 or 2).  With ids = 2 qold may alias qnew: each
+ * column is read before any write to it, and only by its own slice. */
+double orc_step2ds_mt(int rp, int maxm, int meqn, int mwaves, int num_aux, int mbc, int mx, int my,
+                      const double *qold, double *qnew, const double *aux, double dx, double dy,
+                      double dt, const int *method, const int *mthlim, int ids, int use_fwave, int nthreads)
+{
+    sweep_args_t A = {ids, maxm, meqn, mwaves, num_aux, mbc, mx, my, qold, qnew, aux, dx, dy, dt,
+                      method, mthlim, use_fwave, 0, 0};
+    pick_rp(rp, &A.rpn2, &A.rpt2);
+    return classic_sweeps(&A, nthreads);
+}
+
 double orc_step2ds(int rp, int maxm, int meqn, int mwaves, int num_aux, int mbc, int mx, int my,
                    const double *qold, double *qnew, const double *aux, double dx, double dy,
                    double dt, const int *method, const int *mthlim, int ids, int use_fwave)
 {
-    rpn2_fn n; rpt2_fn t; pick_rp(rp, &n, &t);
-    return classic_sweeps(ids, maxm, meqn, mwaves, num_aux, mbc, mx, my, qold, qnew, aux, dx, dy,
-                          dt, method, mthlim, use_fwave, n, t);
+    return orc_step2ds_mt(rp, maxm, meqn, mwaves, num_aux, mbc, mx, my, qold, qnew, aux, dx, dy, dt,
+                          method, mthlim, ids, use_fwave, 1);
 }
 
 /* Direct access to one Riemann solver over a slice (for unit tests). */
@@ -1037,10 +1121,14 @@ static double sharp_flux1(const double *q1d, double *dq1d, const double *aux, do
     return cfl;
 }
 
-/* SharpClaw dq for the whole patch.  q, aux padded (mbc ghosts, already filled);
-   dq padded, overwritten (zeroed first).  Returns the CFL number. */
-double orc_sharp_dq(int rp, int meqn, int mwaves, int num_aux, int mbc, int mx, int my,
-                    const double *q, const double *aux, double *dq, double dx, double dy, double dt)
+/* SharpClaw dq for the whole patch (sharpclaw/flux2.f90:2-101).  q, aux padded (mbc
+   ghosts, already filled); dq padded, overwritten (zeroed first).  Returns the CFL
+   number.  Each x slice (row j) adds only to its own row of dq and each y slice to
+   its own column, so the slices of one direction run on nthreads threads; the y
+   slices start after every x slice has added its part, as in the reference. */
+double orc_sharp_dq_mt(int rp, int meqn, int mwaves, int num_aux, int mbc, int mx, int my,
+                       const double *q, const double *aux, double *dq, double dx, double dy, double dt,
+                       int nthreads)
 {
     rpn2_fn rpn2; rpt2_fn rpt2u; pick_rp(rp, &rpn2, &rpt2u); (void)rpt2u;
     const int ne = meqn, na = num_aux > 0 ? num_aux : 1;
@@ -1048,31 +1136,50 @@ double orc_sharp_dq(int rp, int meqn, int mwaves, int num_aux, int mbc, int mx, 
     size_t tot = (size_t)(mx + 2 * mbc) * (my + 2 * mbc) * ne;
     memset(dq, 0, tot * sizeof(double));
     size_t nmax = (size_t)(maxnx + 2 * mbc);
-    double *q1d = calloc(nmax * ne, sizeof(double)), *dq1d = calloc(nmax * ne, sizeof(double));
-    double *a1d = calloc(nmax * na, sizeof(double));
     double cfl = 0.0;
-    for (int j = 0; j <= my + 1; ++j) {
-        for (int i = 1 - mbc; i <= mx + mbc; ++i) {
-            for (int m = 0; m < ne; ++m) V(q1d, ne, m, i) = QF(q, ne, i, j)[m];
-            if (num_aux > 0) for (int m = 0; m < num_aux; ++m) V(a1d, na, m, i) = QF(aux, na, i, j)[m];
+    if (nthreads < 1) nthreads = 1;
+    for (int dir = 1; dir <= 2; ++dir) {
+        const int n0 = dir == 1 ? my : mx;
+        double c = 0.0;
+#pragma omp parallel num_threads(nthreads) reduction(max : c)
+        {
+            double *q1d = calloc(nmax * ne, sizeof(double)), *dq1d = calloc(nmax * ne, sizeof(double));
+            double *a1d = calloc(nmax * na, sizeof(double));
+#pragma omp for schedule(static)
+            for (int k = 0; k <= n0 + 1; ++k) {
+                double cs;
+                if (dir == 1) {
+                    const int j = k;
+                    for (int i = 1 - mbc; i <= mx + mbc; ++i) {
+                        for (int m = 0; m < ne; ++m) V(q1d, ne, m, i) = QF(q, ne, i, j)[m];
+                        if (num_aux > 0) for (int m = 0; m < num_aux; ++m) V(a1d, na, m, i) = QF(aux, na, i, j)[m];
+                    }
+                    cs = sharp_flux1(q1d, dq1d, a1d, dt, dx, 1, num_aux, ne, mwaves, mx, mbc, maxnx, rpn2);
+                    for (int i = 1; i <= mx; ++i)
+                        for (int m = 0; m < ne; ++m) QF(dq, ne, i, j)[m] = QF(dq, ne, i, j)[m] + V(dq1d, ne, m, i);
+                } else {
+                    const int i = k;
+                    for (int j = 1 - mbc; j <= my + mbc; ++j) {
+                        for (int m = 0; m < ne; ++m) V(q1d, ne, m, j) = QF(q, ne, i, j)[m];
+                        if (num_aux > 0) for (int m = 0; m < num_aux; ++m) V(a1d, na, m, j) = QF(aux, na, i, j)[m];
+                    }
+                    cs = sharp_flux1(q1d, dq1d, a1d, dt, dy, 2, num_aux, ne, mwaves, my, mbc, maxnx, rpn2);
+                    for (int j = 1; j <= my; ++j)
+                        for (int m = 0; m < ne; ++m) QF(dq, ne, i, j)[m] = QF(dq, ne, i, j)[m] + V(dq1d, ne, m, j);
+                }
+                c = c > cs ? c : cs;
+            }
+            free(q1d); free(dq1d); free(a1d);
         }
-        double c = sharp_flux1(q1d, dq1d, a1d, dt, dx, 1, num_aux, ne, mwaves, mx, mbc, maxnx, rpn2);
         cfl = dmax1(cfl, c);
-        for (int i = 1; i <= mx; ++i)
-            for (int m = 0; m < ne; ++m) QF(dq, ne, i, j)[m] = QF(dq, ne, i, j)[m] + V(dq1d, ne, m, i);
     }
-    for (int i = 0; i <= mx + 1; ++i) {
-        for (int j = 1 - mbc; j <= my + mbc; ++j) {
-            for (int m = 0; m < ne; ++m) V(q1d, ne, m, j) = QF(q, ne, i, j)[m];
-            if (num_aux > 0) for (int m = 0; m < num_aux; ++m) V(a1d, na, m, j) = QF(aux, na, i, j)[m];
-        }
-        double c = sharp_flux1(q1d, dq1d, a1d, dt, dy, 2, num_aux, ne, mwaves, my, mbc, maxnx, rpn2);
-        cfl = dmax1(cfl, c);
-        for (int j = 1; j <= my; ++j)
-            for (int m = 0; m < ne; ++m) QF(dq, ne, i, j)[m] = QF(dq, ne, i, j)[m] + V(dq1d, ne, m, j);
-    }
-    free(q1d); free(dq1d); free(a1d);
     return cfl;
+}
+
+double orc_sharp_dq(int rp, int meqn, int mwaves, int num_aux, int mbc, int mx, int my,
+                    const double *q, const double *aux, double *dq, double dx, double dy, double dt)
+{
+    return orc_sharp_dq_mt(rp, meqn, mwaves, num_aux, mbc, mx, my, q, aux, dq, dx, dy, dt, 1);
 }
 
 /* WENO5 reconstruction over a slice q(meqn, 1:n) for k = 3..n-2 (unit tests). */

@@ -5,9 +5,21 @@ constructors, attribute names and cell-centre formula
 (centers[i] = lower + (i+0.5)*delta, geometry.py:439-446), so problem set-up
 code written for the reference runs unchanged.
 """
+import math
 import os
 
 import numpy as np
+
+
+def py2_round(v):
+    """Python 2's round() to an integer: the nearest integer, halves away from
+    zero (the reference is Python 2 code; Python 3's round takes halves to
+    even).  v - floor(v) is exact for every finite double, so the test against
+    0.5 is exact."""
+    if v < 0:
+        return -py2_round(-v)
+    f = math.floor(v)
+    return int(f) + (1 if v - f >= 0.5 else 0)
 
 
 class Dimension(object):
@@ -134,10 +146,13 @@ class Grid(object):
     p_edges = c_edges
 
     def add_gauges(self, gauge_coords):
-        """Cell indices of gauges inside the grid (geometry.py:336-355)."""
+        """Cell indices of gauges inside the grid (geometry.py:336-355), rounded as
+        Python 2 rounds.  As in the reference, a gauge in the last half cell of a
+        dimension gets index num_cells; writing its values then raises IndexError
+        (solver.py:722-723)."""
         for gauge in gauge_coords:
             if all(self.lower[n] <= gauge[n] < self.upper[n] for n in range(self.num_dim)):
-                idx = [int(round((gauge[n] - self.lower[n]) / self.delta[n])) for n in range(self.num_dim)]
+                idx = [py2_round((gauge[n] - self.lower[n]) / self.delta[n]) for n in range(self.num_dim)]
                 self.gauge_file_names.append('gauge' + '_'.join(str(c) for c in gauge) + '.txt')
                 self.gauges.append(idx)
 

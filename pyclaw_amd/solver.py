@@ -366,6 +366,9 @@ class Solver(object):
         if state.grid.gauges and not (self.compute_gauge_values is default_compute_gauge_values
                                       and state._q.buf.dtype == torch.float64):
             return None
+        if state.grid.gauges and not gauge_cells_inside(state):
+            # the host loop raises the reference's IndexError after the first step
+            return None
         if not hasattr(_lib.lib(), 'claw2d_classic_loop'):
             return None
         q_custom = any(b == BC.custom for b in self.bc_lower + self.bc_upper)
@@ -508,6 +511,7 @@ class Solver(object):
         if not gauges:
             return
         state = solution.state
+        check_gauge_cells(state)
         buf = state._q.buf
         if self.compute_gauge_values is default_compute_gauge_values and buf.is_cuda \
                 and buf.dtype == torch.float64:
@@ -535,7 +539,7 @@ class Solver(object):
                 gd.append(np.append(t, p))
         files = state.grid.gauge_files
         if i < len(files):
-            files[i].write(str(t) + ' ' + ' '.join(str(j) for j in p) + '\n')
+            files[i].write(py2_str(t) + ' ' + ' '.join(py2_str(j) for j in p) + '\n')
 
     _GAUGE_CAP = 1024   # steps recorded on the device between flushes
 
@@ -579,6 +583,35 @@ class Solver(object):
             for i in range(len(state.grid.gauges)):
                 self._emit_gauge(state, i, t, vals[r, i * ne:(i + 1) * ne])
         rec['times'] = []
+
+
+def py2_str(x):
+    """str(x) as Python 2 writes a float (12 significant digits, '.0' added to
+    integral values) -- the reference writes gauge lines with str() under
+    Python 2 (solver.py:736-737), and so did numpy's float64 there.  Other
+    values use str()."""
+    if isinstance(x, (float, np.floating)) and not isinstance(x, np.float16):
+        s = '%.12g' % float(x)
+        if all(ch in '-0123456789' for ch in s):
+            s += '.0'
+        return s
+    return str(x)
+
+
+def gauge_cells_inside(state):
+    mx, my = state.patch.num_cells_global[0], state.patch.num_cells_global[1]
+    return all(0 <= ix < mx and 0 <= iy < my for (ix, iy) in state.grid.gauges)
+
+
+def check_gauge_cells(state):
+    """The reference reads q[:, ix, iy] for each gauge (solver.py:722-723), so a
+    gauge index outside the grid raises numpy's IndexError; the device buffer
+    is padded, so the same check is made here before any read."""
+    shape = state.patch.num_cells_global
+    for g in state.grid.gauges:
+        for axis, (i, n) in enumerate(zip(g, shape)):
+            if not 0 <= i < n:
+                raise IndexError("index %d is out of bounds for axis %d with size %d" % (i, axis + 1, n))
 
 
 def field_end(state, idim, ng):

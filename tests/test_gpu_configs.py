@@ -1,0 +1,234 @@
+"""The benchmark configurations on their own workloads, through the drop-in API
+(Controller / ClawSolver2D / SharpClawSolver2D) on the GPU, against the oracle:
+
+  C1  homogeneous acoustics 200x200, ring pulse about a seeded centre, walls
+      low / extrapolation high, unsplit MC, CFL 0.9 to t=0.6
+      (apps/acoustics_2d_homogeneous/acoustics.py:82-92): every frame equal.
+  C2  plain shock-bubble (apps/euler_2d/shockbubble.py:17-87, no source term),
+      unsplit MC on 5 waves, transverse correction waves, CFL 0.8 to t=0.2 at
+      640x160: reference arithmetic frame for frame equal, fused build within
+      1e-12; 4096x1024: three steps of the default (large-grid) launch shape and
+      of a pinned shape equal to the oracle.
+  C3  SharpClaw shallow-water hump, periodic, WENO5 + SSP104 (sharpclaw/solver.py:221-251):
+      128x128 frames equal to the oracle; 1024x1024 for about 500 steps with
+      total h, hu, hv kept to 1e-13 relative to total h (compensated sums), and
+      the large-grid default launch shape's dq equal to a pinned shape's and the
+      oracle's.
+The oracle runs on all the host's cores (bit-identical to one thread).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import apps
+import oracle as O
+from helpers import NG, _bc_struct, _dev_grid, _rp_struct, from_device_interior, gpu_classic_step, \
+    RP_CASES, to_device_padded
+from test_fixtures_oracle import oracle_run
+from test_gpu_apps import amd_controller
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _oracle_threads():
+    old = O.set_threads(O.host_threads())
+    yield
+    O.set_threads(old)
+
+
+def _frames(claw):
+    claw.run()
+    return [f.state.q.copy() for f in claw.frames]
+
+
+def _compare(got, ref):
+    assert len(got) == len(ref)
+    for k, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a, b), "frame %d differs: max %g" % (k, np.abs(a - b).max())
+
+
+def _rel_per_field(a, b):
+    return [float(np.abs(a[m] - b[m]).max() / max(np.abs(b[m]).max(), 1e-300)) for m in range(a.shape[0])]
+
+
+@pytest.mark.parametrize("trans", [1, 2])
+@pytest.mark.parametrize("device_loop", [True, False])
+def test_config1_acoustics_200_to_t06(trans, device_loop):
+    print("config 1: acoustics 200x200 to t=0.6, transverse", trans, "device loop", device_loop, flush=True)
+    q0, aux, opts = apps.config1_acoustics(seed=1, transverse=trans)
+    claw = amd_controller(q0, aux, opts, device_loop=device_loop)
+    got = _frames(claw)
+    ref, run = oracle_run(q0, aux, opts)
+    assert run.nsteps > 100
+    _compare(got, ref)
+    assert claw.solution.t == run.t
+
+
+def test_config2_shockbubble_640x160_to_t02():
+    print("config 2: shock-bubble 640x160 to t=0.2 (exact and fused builds)", flush=True)
+    q0, aux, opts = apps.config2_shockbubble(640, 160)
+    kw = dict(user_bc_lower=apps.shockbubble_inflow_numpy)
+    ref, run = oracle_run(q0, aux, opts, **kw)
+    print("oracle steps", run.nsteps, flush=True)
+    claw = amd_controller(q0, aux, opts, custom_bc_static=True, **kw)
+    got = _frames(claw)
+    _compare(got, ref)
+    assert claw.solution.t == run.t
+    claw = amd_controller(q0, aux, opts, fma=True, custom_bc_static=True, **kw)
+    fused = _frames(claw)
+    assert len(fused) == len(ref)
+    worst = max(max(_rel_per_field(a, b)) for a, b in zip(fused, ref))
+    print("fused build vs oracle, worst frame relative L-inf per field: %.3g" % worst, flush=True)
+    assert worst <= 1e-12
+    assert any(not np.array_equal(a, b) for a, b in zip(fused, ref))
+
+
+def _launch(ctx, mx, my, case, prec):
+    from pyclaw_amd import _lib
+    band, w, wpb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    m = _lib.ClawMethod()
+    m.order, m.transverse, m.split, m.num_waves = 2, 2, 0, case['nw']
+    for k in range(case['nw']):
+        m.mthlim[k] = 4
+    _lib.check(_lib.lib().claw2d_classic_get_launch(ctx.handle, _dev_grid(mx, my, 1.0, 1.0), m, _rp_struct(case),
+                                                   prec, ctypes.byref(band), ctypes.byref(w), ctypes.byref(wpb)),
+               'claw2d_classic_get_launch')
+    return band.value, w.value, wpb.value
+
+
+def test_config2_shockbubble_4096x1024_steps_default_and_pinned_shape():
+    """Three steps at the headline size with the default launch shape (the
+    large-grid default: two-waves build, one-wavefront blocks, ~3926-wavefront
+    bands) and one step with a pinned shape, each equal to the oracle."""
+    from pyclaw_amd import _lib
+    from pyclaw_amd.problems import euler_max_speed, shock_state
+    print("config 2: shock-bubble 4096x1024, 3 exact steps", flush=True)
+    mx, my = 4096, 1024
+    q, dx, dy = apps.sb_init(mx, my)
+    dt = 0.8 * min(dx, dy) / euler_max_speed(q)
+    case = RP_CASES['euler']
+    ctx = _lib.Context(0)
+    assert _launch(ctx, mx, my, case, _lib.F64) == (18, 2, 1)
+    inflow = np.array(shock_state())
+    inflow = np.array([inflow[0], inflow[0] * inflow[1], 0.0, inflow[2], 0.0])
+
+    def padded(qi):
+        qbc = np.zeros((5, mx + 4, my + 4), order='F')
+        qbc[:, 2:-2, 2:-2] = qi
+        qbc[:, :2, :] = inflow[:, None, None]
+        O.fill_bc(qbc, 2, [99, 3], [1, 1], True)
+        return qbc
+
+    pinned = None
+    for k in range(3):
+        qbc = padded(q)
+        ref, cref = O.step2(O.RP_EULER, qbc, None, dx, dy, dt, O.method_array(2, 2, False), O.mthlim_array(4, 5), 5)
+        ref = ref[:, 2:-2, 2:-2]
+        got, cgot = gpu_classic_step(qbc, None, case, dt, 2, 2, bc_lower=(0, 3), bc_upper=(1, 1), dx=dx, dy=dy)
+        assert np.array_equal(got, ref) and cgot == cref, "step %d max %g" % (k, np.abs(got - ref).max())
+        if k == 0:
+            pinned = _pinned_step(qbc, case, dt, dx, dy, (36, 1, 4))
+            assert np.array_equal(pinned[0], ref) and pinned[1] == cref
+        print("step", k, "equal, cfl", cgot, flush=True)
+        q = ref
+
+
+def _pinned_step(qbc, case, dt, dx, dy, shape):
+    """gpu_classic_step with a launch shape set on the context first."""
+    import torch
+    from pyclaw_amd import _lib
+    ne, nxp, nyp = qbc.shape
+    mx, my = nxp - 4, nyp - 4
+    pad = np.zeros((ne, mx + 2 * NG, my + 2 * NG), order='F')
+    pad[:, NG - 2:NG - 2 + nxp, NG - 2:NG - 2 + nyp] = qbc
+    qd = to_device_padded(pad)
+    qn = torch.zeros_like(qd)
+    m = _lib.ClawMethod()
+    m.order, m.transverse, m.split, m.num_waves = 2, 2, 0, case['nw']
+    for k in range(case['nw']):
+        m.mthlim[k] = 4
+    grid = _dev_grid(mx, my, dx, dy)
+    ctx = _lib.Context(0)
+    _lib.check(_lib.lib().claw2d_classic_set_launch_ex(ctx.handle, grid, m, _rp_struct(case), _lib.F64, *shape),
+               'claw2d_classic_set_launch_ex')
+    assert _launch(ctx, mx, my, case, _lib.F64) == shape
+    dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
+    cfl = torch.zeros((), dtype=torch.float64, device='cuda')
+    _lib.check(_lib.lib().claw2d_classic_step(ctx.handle, grid, m, _rp_struct(case), _bc_struct((0, 3), (1, 1)),
+                                              _lib.F64, qd.data_ptr(), qn.data_ptr(), None, dtd.data_ptr(),
+                                              cfl.data_ptr(), torch.cuda.current_stream().cuda_stream),
+               'claw2d_classic_step')
+    torch.cuda.synchronize()
+    return from_device_interior(qn, mx, my), float(cfl.item())
+
+
+@pytest.mark.parametrize("device_loop", [True, False])
+def test_config3_shallow_128_frames_equal_oracle(device_loop):
+    print("config 3: shallow water 128x128 SSP104, device loop", device_loop, flush=True)
+    q0, aux, opts = apps.config3_shallow(128, nsteps=20, num_output_times=2)
+    claw = amd_controller(q0, aux, opts, device_loop=device_loop)
+    got = _frames(claw)
+    ref, run = oracle_run(q0, aux, opts)
+    assert run.nsteps >= 15
+    _compare(got, ref)
+    assert claw.solution.t == run.t
+
+
+def test_config3_shallow_1024_conservation_500_steps():
+    print("config 3: shallow water 1024x1024, about 500 SSP104 steps", flush=True)
+    q0, aux, opts = apps.config3_shallow(1024, nsteps=500)
+    claw = amd_controller(q0, aux, opts)
+    got = _frames(claw)
+    n = claw.solver.status['numsteps']
+    qf = got[-1]
+    assert n >= 450 and np.all(np.isfinite(qf))
+    tot0 = [math.fsum(q0[m].ravel()) for m in range(3)]
+    tot1 = [math.fsum(qf[m].ravel()) for m in range(3)]
+    rel = [abs(tot1[m] - tot0[m]) / tot0[0] for m in range(3)]
+    print("steps", n, "relative change of total h, hu, hv (over total h):", rel, flush=True)
+    assert max(rel) <= 1e-13
+
+
+def test_config3_sharpclaw_large_grid_default_shape():
+    """claw2d_sharpclaw_dq at 1024x1024: the default launch shape for large grids
+    (three-waves build, 7-row bands) against a pinned 16-row one-wave shape and
+    the oracle's dq, bit for bit."""
+    import torch
+    from pyclaw_amd import _lib
+    print("config 3: sharpclaw dq 1024x1024 default vs pinned shape vs oracle", flush=True)
+    q0, aux, opts = apps.config3_shallow(1024, nsteps=1)
+    case = RP_CASES['shallow']
+    mx = my = 1024
+    dx = 1.0 / mx
+    pad = np.zeros((3, mx + 2 * NG, my + 2 * NG), order='F')
+    pad[:, NG:-NG, NG:-NG] = q0
+    O.fill_bc(pad, NG, [2, 2], [2, 2], True)
+    O.set_params(O.RP_SHALLOW, dict(grav=1.0))
+    dt = opts['dt_initial']
+    ref, cref = O.sharp_dq(O.RP_SHALLOW, pad, None, dx, dx, dt, 3)
+    grid, rp, bc = _dev_grid(mx, my, dx, dx), _rp_struct(case), _bc_struct((2, 2), (2, 2))
+    qd = to_device_padded(pad)
+    dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
+    out = []
+    for shape in (None, (16, 1, 4)):
+        ctx = _lib.Context(0)
+        band, w, wpb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        if shape is not None:
+            _lib.check(_lib.lib().claw2d_sharpclaw_set_launch_ex(ctx.handle, grid, rp, _lib.F64, *shape),
+                       'claw2d_sharpclaw_set_launch_ex')
+        _lib.check(_lib.lib().claw2d_sharpclaw_get_launch(ctx.handle, grid, rp, _lib.F64, ctypes.byref(band),
+                                                          ctypes.byref(w), ctypes.byref(wpb)),
+                   'claw2d_sharpclaw_get_launch')
+        assert (band.value, w.value, wpb.value) == (shape if shape is not None else (7, 3, 4))
+        dq = torch.zeros_like(qd)
+        cfl = torch.zeros((), dtype=torch.float64, device='cuda')
+        _lib.check(_lib.lib().claw2d_sharpclaw_dq(ctx.handle, grid, rp, bc, _lib.F64, qd.data_ptr(), dq.data_ptr(),
+                                                  None, dtd.data_ptr(), cfl.data_ptr(),
+                                                  torch.cuda.current_stream().cuda_stream), 'claw2d_sharpclaw_dq')
+        torch.cuda.synchronize()
+        out.append((from_device_interior(dq, mx, my), float(cfl.item())))
+    for got, cgot in out:
+        assert np.array_equal(got, ref) and cgot == cref

@@ -210,3 +210,69 @@ def test_ascii_writer_reproduces_the_reference_frame_files(tmp_path):
         with open(os.path.join(str(tmp_path), mine), 'rb') as f:
             digest = hashlib.sha256(f.read()).hexdigest()
         assert digest == str(G['sha256_' + ref]), mine
+
+
+def test_gauge_indices_round_as_python2():
+    """geometry.py:349-350 rounds with Python 2's round (halves away from zero);
+    on power-of-two spacings gauges at exact cell edges hit the halves."""
+    from pyclaw_amd.geometry import py2_round
+    assert [py2_round(v) for v in (0.5, 1.5, 2.5, 3.5, 0.49999999999999994, 2.4999999999999996, 7.0)] == \
+        [1, 2, 3, 4, 0, 2, 7]
+    assert py2_round(-2.5) == -3
+    x = pc.Dimension('x', 0.0, 1.0, 4)     # delta 0.25
+    y = pc.Dimension('y', 0.0, 2.0, 1024)  # delta 2**-9
+    dom = pc.Domain([x, y])
+    st = pc.State(dom, 3)
+    st.grid.add_gauges([(0.125, 2.0 ** -10), (0.375, 3 * 2.0 ** -10), (0.2, 0.0)])
+    # Python 3's round would give [0, 0], [2, 2], [1, 0]
+    assert st.grid.gauges == [[1, 1], [2, 2], [1, 0]]
+
+
+def test_gauge_in_last_half_cell_raises_index_error():
+    """A gauge in the last half cell gets index num_cells (geometry.py:349-350);
+    the reference then raises IndexError reading q[:, ix, iy] (solver.py:722-723)."""
+    from pyclaw_amd.solver import check_gauge_cells, gauge_cells_inside
+    x = pc.Dimension('x', 0.0, 1.0, 10)
+    y = pc.Dimension('y', 0.0, 1.0, 8)
+    st = pc.State(pc.Domain([x, y]), 3)
+    st.grid.add_gauges([(0.5, 0.5)])
+    check_gauge_cells(st)
+    assert gauge_cells_inside(st)
+    st.grid.add_gauges([(0.97, 0.5)])
+    assert st.grid.gauges[-1] == [10, 4]
+    assert not gauge_cells_inside(st)
+    with pytest.raises(IndexError, match="out of bounds for axis 1 with size 10"):
+        check_gauge_cells(st)
+    st2 = pc.State(pc.Domain([pc.Dimension('x', 0.0, 1.0, 10), pc.Dimension('y', 0.0, 1.0, 8)]), 3)
+    st2.grid.add_gauges([(0.5, 0.99)])
+    with pytest.raises(IndexError, match="axis 2 with size 8"):
+        check_gauge_cells(st2)
+
+
+def test_gauge_lines_use_python2_str():
+    """The reference writes str(t) and str(value) (solver.py:736-737) under
+    Python 2: 12 significant digits, '.0' after integral values."""
+    from pyclaw_amd.solver import py2_str
+    cases = {0.1 + 0.2: '0.3', 1.0 / 3.0: '0.333333333333', 100.0: '100.0', 1e-5: '1e-05',
+             123456789012345.0: '1.23456789012e+14', -0.0: '-0.0', 2.0 ** 60: '1.15292150461e+18',
+             float('inf'): 'inf', 0.6000000000000001: '0.6', 5e-324: '4.94065645841e-324'}
+    for v, s in cases.items():
+        assert py2_str(v) == s, (v, py2_str(v), s)
+        assert py2_str(np.float64(v)) == s
+    assert py2_str(float('nan')) == 'nan'
+    assert py2_str(3) == '3'
+    # a line as the solver writes it
+    solver = pc.ClawSolver2D(pc.riemann.rp2_acoustics)
+
+    class F(object):
+        def __init__(self):
+            self.lines = []
+
+        def write(self, s):
+            self.lines.append(s)
+    st = pc.State(pc.Domain([pc.Dimension('x', 0.0, 1.0, 4), pc.Dimension('y', 0.0, 1.0, 4)]), 3)
+    st.grid.gauges = [[0, 0]]
+    f = F()
+    st.grid.gauge_files = [f]
+    solver._emit_gauge(st, 0, 0.1 + 0.2, np.array([1.0, 2.0 / 3.0, -1e-20]))
+    assert f.lines == ['0.3 1.0 0.666666666667 -1e-20\n']

@@ -309,3 +309,50 @@ def test_ssp104_fourth_order():
     assert 20 < e1 / e2 < 45      # ratio 2^5 = 32 for a fourth-order method
     # strong stability region contains [-6, 0] (SSP coefficient 6 for 10 stages)
     assert abs(R(-6.0)) <= 1.0 + 1e-12
+
+
+# ---------------------------------------------------------------------------
+# 3. the multi-threaded sweeps equal the single-threaded ones bit for bit
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["acoustics", "vc_acoustics", "euler", "shallow"])
+@pytest.mark.parametrize("threads", [2, 5, 8])
+def test_threaded_oracle_is_bit_identical(name, threads):
+    """Row (column) bands that recompute one slice on each side give every cell
+    its contributions in the reference's order: step2, both step2ds calls (the
+    second aliased as classic/solver.py:529-531 calls it) and SharpClaw dq."""
+    rng = np.random.default_rng(400 + threads)
+    mx, my = 37, 23
+    aux = None
+    if name == "euler":
+        rp, nw, pd, make = O.RP_EULER, 5, dict(gamma=1.4, gamma1=0.4), euler_state
+    elif name == "shallow":
+        rp, nw, pd, make = O.RP_SHALLOW, 3, dict(grav=1.), shallow_state
+    elif name == "acoustics":
+        rp, nw, pd, make = O.RP_ACOUSTICS, 2, dict(rho=1., bulk=4., cc=2., zz=2.), acoustic_state
+    else:
+        rp, nw, pd, make = O.RP_VC_ACOUSTICS, 2, {}, acoustic_state
+    O.set_params(rp, pd)
+    q = np.asfortranarray(make(rng, mx + 4, my + 4))
+    if name == "vc_acoustics":
+        aux = np.asfortranarray(vc_aux(rng, mx + 4, my + 4))
+    q3 = np.asfortranarray(make(rng, mx + 6, my + 6))
+    aux3 = np.asfortranarray(vc_aux(rng, mx + 6, my + 6)) if aux is not None else None
+    out = {}
+    for nt in (1, threads):
+        old = O.set_threads(nt)
+        try:
+            res = []
+            for trans in (0, 1, 2):
+                res.append(O.step2(rp, q, aux, 0.1, 0.07, 0.01, O.method_array(2, trans, False),
+                                   O.mthlim_array(4, nw), nw))
+            meth = O.method_array(2, 1, True)
+            a = q.copy(order='F')
+            c1 = O.step2ds(rp, q, a, aux, 0.1, 0.07, 0.01, meth, O.mthlim_array(4, nw), nw, 1)
+            c2 = O.step2ds(rp, a, a, aux, 0.1, 0.07, 0.01, meth, O.mthlim_array(4, nw), nw, 2)
+            res.append((a, (c1, c2)))
+            res.append(O.sharp_dq(rp, q3, aux3, 0.1, 0.07, 0.01, nw))
+            out[nt] = res
+        finally:
+            O.set_threads(old)
+    for (a, ca), (b, cb) in zip(out[1], out[threads]):
+        assert np.array_equal(a, b) and ca == cb
