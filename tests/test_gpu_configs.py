@@ -122,6 +122,7 @@ def test_config2_shockbubble_4096x1024_steps_default_and_pinned_shape():
         O.fill_bc(qbc, 2, [99, 3], [1, 1], True)
         return qbc
 
+    O.set_params(O.RP_EULER, case['pd'])   # gamma1 = 0.4 as the device step takes it
     pinned = None
     for k in range(3):
         qbc = padded(q)
