@@ -123,6 +123,24 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
                            const void *qold, void *qnew, const void *aux,
                            const double *dt_dev, double *cfl_dev, int flags, void *stream);
 
+/* claw2d_classic_step without any device allocation: no counterpart in the
+   reference.  For callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Arguments and results as claw2d_classic_step.  The split step's
+   intermediate state, which claw2d_classic_step keeps in memory the context
+   allocates, lives here in a fixed device array of the library
+   (CLAW_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends): a split
+   step needs num_eqn * grid->plane * sizeof(real) bytes of it and returns
+   CLAW_ETOOLARGE beyond that; unsplit steps use none.  The current device
+   must be the context's device.  Every call on one GPU shares that GPU's
+   array: calls that may overlap on one GPU (different streams, threads)
+   race and must be serialised by the caller. */
+#define CLAW_NOALLOC_SCRATCH_BYTES (32u << 20)
+int claw2d_classic_step_noalloc(void *ctx, const ClawGrid *grid, const ClawMethod *method,
+                                const ClawRiemann *rp, const ClawBC *bc, int precision,
+                                const void *qold, void *qnew, const void *aux,
+                                const double *dt_dev, double *cfl_dev, void *stream);
+
 /* Autotuner of the unsplit step's launch shape: times the fused kernel for a
    set of row-band heights (the rows one wavefront marches through; more
    bands mean more wavefronts in flight, fewer mean fewer halo rows; the

@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libclawhip.so")
 
 CLAW_OK, CLAW_EINVAL, CLAW_EDEVICE, CLAW_ENOMEM, CLAW_ETOOLARGE = 0, 1, 2, 3, 4
+NOALLOC_SCRATCH_BYTES = 32 << 20   # CLAW_NOALLOC_SCRATCH_BYTES (claw2d_classic_step_noalloc)
 RP_ACOUSTICS, RP_VC_ACOUSTICS, RP_EULER, RP_SHALLOW = 0, 1, 2, 3
 BC_STORED, BC_EXTRAP, BC_PERIODIC, BC_WALL = 0, 1, 2, 3
 F64, F32, F64_FMA = 0, 1, 2
@@ -88,6 +89,7 @@ def lib():
                                           ctypes.POINTER(ClawRiemann), ctypes.POINTER(ClawBC), ctypes.c_int,
                                           vp, vp, vp, vp, vp, vp]
         L.claw2d_classic_step_ex.argtypes = L.claw2d_classic_step.argtypes[:11] + [ctypes.c_int, vp]
+        L.claw2d_classic_step_noalloc.argtypes = L.claw2d_classic_step.argtypes
         L.claw2d_classic_launches.argtypes = [ctypes.POINTER(ClawMethod)]
         for name, argtypes in _optional_symbols().items():
             fn = getattr(L, name, None)

@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <utility>
@@ -904,14 +905,20 @@ template <> ShallowRoe<float>::Params make_params<ShallowRoe<float>, float>(cons
 
 static int check(hipError_t e) { return e == hipSuccess ? CLAW_OK : CLAW_EDEVICE; }
 
+// fixed_scratch: when non-null, the split step's intermediate state goes there
+// (fixed_cap bytes; CLAW_ETOOLARGE beyond) and nothing is allocated; otherwise
+// the context's scratch, allocated or grown on demand.
 template <class RP, typename R>
 static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
                           const ClawBC *bc, const void *qold, void *qnew, const void *aux,
-                          const double *dt, double *cfl, int flags, hipStream_t st) {
+                          const double *dt, double *cfl, int flags, hipStream_t st,
+                          void *fixed_scratch = nullptr, size_t fixed_cap = 0) {
     int wpb = 4;
     GridDev g{grid->mx, grid->my, grid->ng, grid->ldq, grid->plane};
     BCDev bq{{bc->lower[0], bc->lower[1]}, {bc->upper[0], bc->upper[1]}};
     BCDev ba{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
+    // a split step's scratch need is checked before anything is launched
+    if (meth->split && fixed_scratch && (size_t)grid->plane * RP::NE * sizeof(R) > fixed_cap) return CLAW_ETOOLARGE;
     if (!(flags & CLAW_STEP_ACCUMULATE_CFL))
         hipLaunchKernelGGL(zero_scalar_kernel, dim3(1), dim3(1), 0, st, cfl);
     if (!meth->split) {
@@ -976,16 +983,24 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
 #undef CLAW_UNSPLIT_LAUNCH
         return check(hipGetLastError());
     }
-    // split: x pass into scratch (padded layout, ghost rows included), then y pass
+    // split: x pass into scratch (padded layout, ghost rows included), then y pass.
+    // The y pass reads scratch only where the x pass wrote it (interior columns,
+    // rows -2..my+1), so the scratch never needs clearing.
     size_t bytes = (size_t)grid->plane * RP::NE * sizeof(R);
-    if (ctx->scratch_bytes < bytes) {
-        if (ctx->scratch) (void)hipFree(ctx->scratch);
-        ctx->scratch = nullptr; ctx->scratch_bytes = 0;
-        if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return CLAW_ENOMEM;
-        ctx->scratch_bytes = bytes;
+    void *scr = fixed_scratch;
+    if (scr) {
+        if (bytes > fixed_cap) return CLAW_ETOOLARGE;
+    } else {
+        if (ctx->scratch_bytes < bytes) {
+            if (ctx->scratch) (void)hipFree(ctx->scratch);
+            ctx->scratch = nullptr; ctx->scratch_bytes = 0;
+            if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return CLAW_ENOMEM;
+            ctx->scratch_bytes = bytes;
+        }
+        scr = ctx->scratch;
     }
     SplitParams<RP, R> px;
-    px.qin = (const R *)qold; px.qout = (R *)ctx->scratch; px.aux = (const R *)aux;
+    px.qin = (const R *)qold; px.qout = (R *)scr; px.aux = (const R *)aux;
     px.qbc_src = (const R *)qold;
     px.g = g; px.bq = bq; px.ba = ba; px.P = make_params<RP, R>(rp);
     px.dt = dt; px.cfl = cfl; px.dx = grid->dx; px.dy = grid->dy; px.order = meth->order;
@@ -1002,7 +1017,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
     }
     {
         SplitParams<RP, R> q = px;
-        q.qin = (const R *)ctx->scratch; q.qout = (R *)qnew;
+        q.qin = (const R *)scr; q.qout = (R *)qnew;
         int cols = grid->mx + 2 * CNG;
         q.nsx = (cols + WAVE - 1) / WAVE;
         q.H = band_height(grid->my, q.nsx);
@@ -1017,12 +1032,12 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
 // FP64 step of this build (arguments validated by claw2d_classic_step_ex).
 int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
                      const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
-                     hipStream_t st) {
+                     hipStream_t st, void *fs, size_t fcap) {
     switch (rp->kind) {
-    case CLAW_RP_ACOUSTICS: return launch_classic<Acoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
-    case CLAW_RP_VC_ACOUSTICS: return launch_classic<VCAcoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
-    case CLAW_RP_EULER: return launch_classic<Euler5<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
-    default: return launch_classic<ShallowRoe<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    case CLAW_RP_ACOUSTICS: return launch_classic<Acoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
+    case CLAW_RP_VC_ACOUSTICS: return launch_classic<VCAcoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
+    case CLAW_RP_EULER: return launch_classic<Euler5<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
+    default: return launch_classic<ShallowRoe<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
     }
 }
 
@@ -1030,12 +1045,12 @@ int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const
 // this build's contraction and shared reciprocals.
 int classic_step_f32(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
                      const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
-                     hipStream_t st) {
+                     hipStream_t st, void *fs, size_t fcap) {
     switch (rp->kind) {
-    case CLAW_RP_ACOUSTICS: return launch_classic<Acoustics<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
-    case CLAW_RP_VC_ACOUSTICS: return launch_classic<VCAcoustics<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
-    case CLAW_RP_EULER: return launch_classic<Euler5<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
-    default: return launch_classic<ShallowRoe<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st);
+    case CLAW_RP_ACOUSTICS: return launch_classic<Acoustics<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
+    case CLAW_RP_VC_ACOUSTICS: return launch_classic<VCAcoustics<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
+    case CLAW_RP_EULER: return launch_classic<Euler5<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
+    default: return launch_classic<ShallowRoe<float>, float>(c, grid, meth, rp, bc, qold, qnew, aux, dt, cfl, flags, st, fs, fcap);
     }
 }
 #else
@@ -1185,10 +1200,10 @@ namespace clawhip {
 namespace fused {
 int classic_step_f64(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
                      const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
-                     hipStream_t st);
+                     hipStream_t st, void *fs, size_t fcap);
 int classic_step_f32(Ctx *c, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, const ClawBC *bc,
                      const void *qold, void *qnew, const void *aux, const double *dt, double *cfl, int flags,
-                     hipStream_t st);
+                     hipStream_t st, void *fs, size_t fcap);
 }  // namespace fused
 }  // namespace clawhip
 
@@ -1223,16 +1238,45 @@ int claw_ctx_destroy(void *ctx) {
 
 int claw2d_classic_launches(const ClawMethod *m) { return m && m->split ? 3 : 2; }
 
-int claw2d_classic_step(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
-                        const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
-                        const double *dt_dev, double *cfl_dev, void *stream) {
-    return claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev, 0,
-                                  stream);
+}  // extern "C"
+
+// ---- scratch of the allocation-free step -------------------------------------
+// claw2d_classic_step_noalloc keeps the split step's intermediate state (the
+// x-updated padded state) in this file-scope device array instead of memory
+// the context allocates: one copy per GPU, loaded with the library's code
+// object and held until the process ends.  In host code `scratch` names a
+// host-side stand-in; its device address on each GPU comes from
+// hipGetSymbolAddress, resolved once per device (with that device current) and
+// kept in scratch_dev[], and only that address is handed to kernels.  Every
+// call on a GPU shares that GPU's array: calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[CLAW_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static void *scratch_dev[NOALLOC_MAX_DEVICES];
+
+// The device address of `scratch` on the context's device, which must be the
+// current device (the one the caller's stream belongs to).
+static int noalloc_scratch(Ctx *c, void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return CLAW_EDEVICE;
+    if (dev != c->device || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return CLAW_EINVAL;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return CLAW_EDEVICE;
+        scratch_dev[dev] = ptr;
+    }
+    *out = scratch_dev[dev];
+    return CLAW_OK;
 }
 
-int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
-                           const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
-                           const double *dt_dev, double *cfl_dev, int flags, void *stream) {
+// Arguments validated here.  noalloc: the split step's scratch is `scratch`
+// (looked up after validation, so invalid arguments never touch the device),
+// and nothing is allocated.
+static int classic_step_impl(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                             const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
+                             const double *dt_dev, double *cfl_dev, int flags, void *stream, bool noalloc) {
     if (!ctx || !grid || !meth || !rp || !bc || !qold || !qnew || !dt_dev || !cfl_dev) return CLAW_EINVAL;
     if (qold == qnew) return CLAW_EINVAL;
     if (grid->mx < 2 || grid->my < 2 || grid->ng < 2) return CLAW_EINVAL;
@@ -1257,21 +1301,51 @@ int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *me
     default: return CLAW_EINVAL;
     }
     if (meth->num_waves != nw_expected) return CLAW_EINVAL;
+    if (precision != CLAW_F64 && precision != CLAW_F32 && precision != CLAW_F64_FMA) return CLAW_EINVAL;
+    void *fs = nullptr;
+    size_t fcap = 0;
+    if (noalloc) {
+        const int rc = noalloc_scratch(c, &fs);
+        if (rc) return rc;
+        fcap = CLAW_NOALLOC_SCRATCH_BYTES;
+    }
     if (precision == CLAW_F64_FMA)
-        return fused::classic_step_f64(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+        return fused::classic_step_f64(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st, fs, fcap);
     if (precision == CLAW_F32)
-        return fused::classic_step_f32(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+        return fused::classic_step_f32(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st, fs, fcap);
     if (precision != CLAW_F64) return CLAW_EINVAL;
     switch (rp->kind) {
     case CLAW_RP_ACOUSTICS:
-        return launch_classic<Acoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+        return launch_classic<Acoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st, fs, fcap);
     case CLAW_RP_VC_ACOUSTICS:
-        return launch_classic<VCAcoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+        return launch_classic<VCAcoustics<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st, fs, fcap);
     case CLAW_RP_EULER:
-        return launch_classic<Euler5<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+        return launch_classic<Euler5<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st, fs, fcap);
     default:
-        return launch_classic<ShallowRoe<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st);
+        return launch_classic<ShallowRoe<double>, double>(c, grid, meth, rp, bc, qold, qnew, aux, dt_dev, cfl_dev, flags, st, fs, fcap);
     }
+}
+
+extern "C" {
+
+int claw2d_classic_step(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                        const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
+                        const double *dt_dev, double *cfl_dev, void *stream) {
+    return claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev, 0,
+                                  stream);
+}
+
+int claw2d_classic_step_ex(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                           const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
+                           const double *dt_dev, double *cfl_dev, int flags, void *stream) {
+    return classic_step_impl(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev, flags, stream,
+                             false);
+}
+
+int claw2d_classic_step_noalloc(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
+                                const ClawBC *bc, int precision, const void *qold, void *qnew, const void *aux,
+                                const double *dt_dev, double *cfl_dev, void *stream) {
+    return classic_step_impl(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev, 0, stream, true);
 }
 
 int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,

@@ -43,6 +43,12 @@ def test_invalid_arguments_return_einval_without_gpu():
     rc = L.claw2d_classic_step(ctx.handle, g, m, r, bc, _lib.F64, ctypes.c_void_p(16), ctypes.c_void_p(32),
                                None, ctypes.c_void_p(48), ctypes.c_void_p(64), None)
     assert rc == _lib.CLAW_EINVAL
+    # the allocation-free step validates before it looks up its device array
+    rc = L.claw2d_classic_step_noalloc(ctx.handle, g, m, r, bc, _lib.F64, ctypes.c_void_p(16), ctypes.c_void_p(32),
+                                       None, ctypes.c_void_p(48), ctypes.c_void_p(64), None)
+    assert rc == _lib.CLAW_EINVAL
+    assert L.claw2d_classic_step_noalloc(None, g, m, r, bc, _lib.F64, ctypes.c_void_p(16), ctypes.c_void_p(32),
+                                         None, ctypes.c_void_p(48), ctypes.c_void_p(64), None) == _lib.CLAW_EINVAL
     g = _lib.ClawGrid(8, 8, 3, 0, 14, 196, 0.1, 0.1)
     bc.lower[0], bc.upper[0] = _lib.BC_PERIODIC, _lib.BC_EXTRAP   # periodic on one side only
     rc = L.claw2d_classic_step(ctx.handle, g, m, r, bc, _lib.F64, ctypes.c_void_p(16), ctypes.c_void_p(32),
