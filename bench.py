@@ -64,6 +64,9 @@ def parse():
                     help="euler: the headline line; the others measure the other configurations")
     ap.add_argument('--instances', type=int, default=512)
     ap.add_argument('--adjoint-steps', type=int, default=2000)
+    ap.add_argument('--headline-only', action='store_true',
+                    help="euler workload: skip the rest of the metric (batched speedup, adjoint/forward, "
+                         "SharpClaw, the run to t=0.2) in the line's extra")
     return ap.parse_args()
 
 
@@ -86,6 +89,94 @@ def cpu_baseline(q0, dx, dy, dt, mx, my, nsteps):
         qbc = qn
     el = time.perf_counter() - t0
     return mx * my * nsteps / el / 1e9, el
+
+
+def host_info():
+    """The host the CPU baseline ran on: core counts, CPU model, the oracle's compiler and flags."""
+    import oracle as O
+    info = {"nproc": os.cpu_count(), "cores_usable": O.host_threads(cap=1 << 16)}
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    info["cpu_model"] = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(['gcc', '--version'], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+        info["compiler"] = cc.stdout.splitlines()[0] if cc.stdout else None
+    except OSError:
+        info["compiler"] = None
+    mk = open(os.path.join(HERE, 'oracle', 'Makefile')).read()
+    info["flags"] = [ln.split('=', 1)[1].strip() for ln in mk.splitlines() if ln.startswith('CFLAGS')][0]
+    return info
+
+
+def cpu_baseline_cores(q0, dx, dy, dt, mx, my, nsteps, threads):
+    """The C restatement on `threads` host cores (OpenMP over row and column
+    bands, bit-identical to one thread): Gcell-updates/s and seconds."""
+    import oracle as O
+    old = O.set_threads(threads)
+    try:
+        return cpu_baseline(q0, dx, dy, dt, mx, my, nsteps)
+    finally:
+        O.set_threads(old)
+
+
+def measure_euler_run(args, q0, dx, dy, tfinal=0.2):
+    """The shock-bubble at the headline size as an application: Controller.run
+    to t = 0.2 through the drop-in API (ClawSolver2D, variable dt at CFL 0.8,
+    cfl_max 1.0, the inflow as a static custom boundary, the step loop on the
+    device), in the headline's arithmetic.  Wall time of the run over accepted
+    steps: Gcell-updates/s of the whole application."""
+    import torch
+    import pyclaw_amd as pc
+    from pyclaw_amd.problems import GAMMA, GAMMA1, euler_max_speed, shock_state
+    from oracle.shockbubble_ref import shockbubble_inflow_numpy
+    mx, my = q0.shape[1], q0.shape[2]
+    dt0 = 0.8 * min(dx, dy) / euler_max_speed(q0)
+
+    def run():
+        x = pc.Dimension('x', 0.0, 2.0, mx)
+        y = pc.Dimension('y', 0.0, 0.5, my)
+        dom = pc.Domain([x, y])
+        st = pc.State(dom, 5)
+        st.problem_data.update(gamma=GAMMA, gamma1=GAMMA1)
+        st.q[...] = q0
+        s = pc.ClawSolver2D(pc.riemann.rp2_euler_5wave)
+        s.dimensional_split = False
+        s.transverse_waves = 2
+        s.limiters = pc.limiters.tvd.MC
+        s.cfl_max, s.cfl_desired = 1.0, 0.8
+        s.dt_initial = dt0
+        s.bc_lower = [pc.BC.custom, pc.BC.wall]
+        s.bc_upper = [pc.BC.extrap, pc.BC.extrap]
+        s.user_bc_lower = shockbubble_inflow_numpy
+        s.custom_bc_static = True
+        s.fma = args.precision == 'f64fma'
+        claw = pc.Controller()
+        claw.solution = pc.Solution(st, dom)
+        claw.solver = s
+        claw.output_format = None
+        claw.keep_copy = True
+        claw.tfinal = tfinal
+        claw.num_output_times = 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        claw.run()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        qf = claw.frames[-1].state.q
+        return el, s.status, bool(np.isfinite(qf).all()), claw.solution.t
+    run()                                     # warm-up (kernel loads, buffers)
+    el, status, finite, tf = run()
+    n = status['numsteps']
+    return {"gcell_updates_per_s": mx * my * n / el / 1e9, "wall_s": el, "accepted_steps": n,
+            "ms_per_step": 1e3 * el / n, "t_final": tf, "cflmax": status['cflmax'], "finite": finite,
+            "arithmetic": args.precision,
+            "how": "Controller.run to t=%g, one output interval, step loop on the device (ClawSolver2D, "
+                   "device_loop default), wall clock including host waits" % tfinal}
 
 
 def main():
@@ -249,12 +340,22 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        v, el = cpu_baseline(q0, dx, dy, dt, mx, my, args.cpu_sample_steps)
-        cpu = {"value": v, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-               "sample": "%d steps of the same 4096x1024 FP64 unsplit step (%.1f s), single-thread C "
-                         "restatement of step2/flux2/limiter + Euler Roe solver (oracle/claw_oracle.c, "
-                         "gcc -O2 -ffp-contract=off); the reference Euler solver cannot run this grid "
-                         "(maxm2=1800)" % (args.cpu_sample_steps, el)}
+        import oracle as O
+        nt = O.host_threads()
+        v1, el1 = cpu_baseline(q0, dx, dy, dt, mx, my, args.cpu_sample_steps)
+        # all cores: the same restatement on OpenMP row/column bands, a sample
+        # of about 10 s of that work
+        nall = max(3, min(40, int(10.0 / max(el1 / args.cpu_sample_steps / nt, 1e-3))))
+        vn, eln = cpu_baseline_cores(q0, dx, dy, dt, mx, my, nall, nt)
+        cpu = {"value": vn, "unit": "Gcell-updates/s", "cores": nt, "kind": "port",
+               "sample": "%d steps of the same 4096x1024 FP64 unsplit step (%.1f s) on %d host threads: "
+                         "the C restatement of step2/flux2/limiter + Euler Roe solver (oracle/claw_oracle.c) "
+                         "with OpenMP over row bands then column bands, each band recomputing one slice on "
+                         "either side (bit-identical to one thread); it stands in for the reference's "
+                         "f2py Fortran + petclaw MPI path, which cannot run here, and the reference Euler "
+                         "solver cannot run this grid (maxm2=1800)" % (nall, eln, nt),
+               "single_core": {"value": v1, "steps": args.cpu_sample_steps, "seconds": el1},
+               "host": host_info()}
 
     out = {
         "metric": "Gcell-updates/s",
@@ -287,18 +388,41 @@ def main():
                   "hbm_pct": 100.0 * achieved / HBM_PEAK_GBS, "launch_shape": band,
                   "f32_rel_linf_vs_f64_per_field": f32_err, "fma_vs_reference_arithmetic": fma_err},
     }
+    if not args.headline_only and (mx, my) == (4096, 1024) and args.precision != 'f32':
+        # the rest of the metric, each part on its own configuration
+        del qa, qb, bufs
+        torch.cuda.empty_cache()
+        rest = {}
+        for key, fn in (("euler_4096x1024_run_to_t0.2", lambda: measure_euler_run(args, q0, dx, dy)),
+                        ("batched_vs_plain_speedup", lambda: measure_batched(args, nsteps=200)),
+                        ("adjoint_over_forward", lambda: measure_adjoint(args)),
+                        ("sharpclaw_shallow_1024", lambda: measure_sharpclaw(args, steps=500, warmup=5,
+                                                                             precision='f64', launch='7,3,4'))):
+            t0 = time.perf_counter()
+            try:
+                r_ = fn()
+            except Exception as e:      # recorded in the line, the headline still printed
+                r_ = {"error": "%s: %s" % (type(e).__name__, e)}
+            r_["bench_seconds"] = time.perf_counter() - t0
+            rest[key] = r_
+            torch.cuda.empty_cache()
+        out["extra"]["rest_of_metric"] = rest
     print(json.dumps(out), flush=True)
 
 
-def bench_batched(args):
+def measure_batched(args, nsteps=None):
     """512 independent layered vc-acoustics grids (sizes from {64,...,512}^2),
-    --steps steps each (300 by default) with its own CFL-0.9 dt: one hipGraph-captured sequence of
-    persistent work-queue launches vs the same grids stepped one at a time
-    through the plain single-grid entry point."""
+    nsteps steps each (--steps, 200 in the default line) with its own CFL-0.9
+    dt: one hipGraph-captured sequence of persistent work-queue launches vs the
+    same grids stepped one at a time through the plain single-grid entry point.
+    Both legs have their buffers uploaded before their timer starts; each times
+    only the stepping (the batched leg one graph replay, the other leg the
+    launches of every instance's steps)."""
     import torch
     import pyclaw_amd as pc
-    from pyclaw_amd.batch import BatchedClawSolver2D, run_one_by_one
+    from pyclaw_amd.batch import BatchedClawSolver2D, OneByOne
     from pyclaw_amd.problems import batched_sizes, layered_vc_acoustics
+    nsteps = args.steps if nsteps is None else nsteps
     rng = np.random.default_rng(2024)
     sizes = batched_sizes(rng, args.instances)
     inst = []
@@ -306,7 +430,6 @@ def bench_batched(args):
         q, aux = layered_vc_acoustics(rng, mx, my)
         dx, dy = 1.0 / mx, 1.0 / my
         inst.append(dict(q=q, aux=aux, dx=dx, dy=dy, dt_initial=0.9 * min(dx, dy) / aux[1].max()))
-    nsteps = args.steps
     cells = sum(mx * my for mx, my in sizes)
     rp = pc.riemann.rp2_vc_acoustics
 
@@ -325,28 +448,42 @@ def bench_batched(args):
     b.run(nsteps, use_graph=True)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
-    b3 = b
-    acc = b3.naccepted.cpu().numpy()
-    t0 = time.perf_counter()
-    ref = run_one_by_one(rp, {}, inst[:8], nsteps)      # equality check on a few instances
-    same = all(np.array_equal(b3.q(k), ref[k]) for k in range(8))
+    acc = b.naccepted.cpu().numpy()
+    warm = OneByOne(rp, {}, inst[:8])       # warm-up of the plain path, and the equality check
+    warm.run(nsteps)
+    ref = warm.results()
+    same = all(np.array_equal(b.q(k), ref[k]) for k in range(8))
+    del warm
+    ob = OneByOne(rp, {}, inst)             # uploads (untimed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run_one_by_one(rp, {}, inst, nsteps, keep=False)
+    ob.run(nsteps)
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t0
-    out = {"metric": "batched vs plain speedup", "value": t_loop / t_batch, "unit": "x", "n_gpus": 1,
-           "steps": nsteps, "warmup": nsteps, "ms_per_step": 1e3 * t_batch / nsteps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded layered media)",
-           "config": {"workload": "%d vc-acoustics instances, sizes {64..512}^2, %d steps each, classic unsplit "
-                                  "MC trans_inc, CFL 0.9 per instance" % (args.instances, nsteps),
-                      "global_batch": args.instances, "parallelism": "single GPU"},
-           "extra": {"batched_s": t_batch, "one_by_one_s": t_loop, "cells_per_step": cells,
-                     "batched_gcell_updates_per_s": cells * nsteps / t_batch / 1e9,
-                     "bitwise_equal_first8": same, "all_accepted": bool((acc == nsteps).all()),
-                     "work_items_per_step": b3.work_items,
-                     "tile_shape": None if shape is None else {"rows": shape[0], "waves_per_block": shape[1]}}}
-    print(json.dumps(out), flush=True)
+    ob.results()                            # no step rejected (the ping-pong assumption)
+    del ob
+    gcell = cells * nsteps / t_batch / 1e9
+    achieved = 64.0 * cells * nsteps / t_batch / 1e9
+    return {"metric": "batched vs plain speedup", "value": t_loop / t_batch, "unit": "x", "n_gpus": 1,
+            "steps": nsteps, "warmup": nsteps, "ms_per_step": 1e3 * t_batch / nsteps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded layered media)",
+            "config": {"workload": "%d vc-acoustics instances, sizes {64..512}^2, %d steps each, classic unsplit "
+                                   "MC trans_inc, CFL 0.9 per instance" % (args.instances, nsteps),
+                       "global_batch": args.instances, "parallelism": "single GPU"},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": 64,
+                         "kernel": "batch_step_kernel (whole step, update kernel included)"},
+            "extra": {"batched_s": t_batch, "one_by_one_s": t_loop, "cells_per_step": cells,
+                      "batched_gcell_updates_per_s": gcell,
+                      "one_by_one_gcell_updates_per_s": cells * nsteps / t_loop / 1e9,
+                      "timed": "stepping only in both legs (buffers uploaded beforehand)",
+                      "bitwise_equal_first8": same, "all_accepted": bool((acc == nsteps).all()),
+                      "work_items_per_step": b.work_items,
+                      "tile_shape": None if shape is None else {"rows": shape[0], "waves_per_block": shape[1]}}}
+
+
+def bench_batched(args):
+    print(json.dumps(measure_batched(args)), flush=True)
 
 
 def _grid_bufs(q0, ng, dtype, dev):
@@ -357,7 +494,7 @@ def _grid_bufs(q0, ng, dtype, dev):
     return buf.to(dev, dtype)
 
 
-def bench_sharpclaw(args):
+def measure_sharpclaw(args, steps=None, warmup=None, precision=None, launch=None):
     """SharpClaw shallow water 1024x1024 periodic, WENO5 + SSP104, FP64: one
     step = 10 fused stage launches (one C call).  The reference-arithmetic
     build by default (bit-exact); --precision f64fma times the fused build,
@@ -367,6 +504,10 @@ def bench_sharpclaw(args):
     import torch
     from pyclaw_amd import _lib
     from pyclaw_amd.problems import shallow_hump
+    steps = steps if steps is None else steps
+    warmup = warmup if warmup is None else warmup
+    precision = args.precision if precision is None else precision
+    launch = args.launch if launch is None else launch
     mx = my = 1024
     rng = np.random.default_rng(7)
     x0 = 0.25 + 0.5 * rng.random(2)
@@ -389,20 +530,19 @@ def bench_sharpclaw(args):
     cfl = torch.zeros(10, dtype=torch.float64, device=dev)
     sh = torch.cuda.current_stream().cuda_stream
     pair = [q, qn]
-    tot0 = q0.reshape(3, -1).sum(axis=1)
 
     ctx = _lib.Context(0)
-    if args.precision == 'f32':
+    if precision == 'f32':
         raise SystemExit("the sharpclaw workload runs FP64 (--precision f64fma or f64)")
-    prec = _lib.F64_FMA if args.precision == 'f64fma' else _lib.F64
+    prec = _lib.F64_FMA if precision == 'f64fma' else _lib.F64
     band = None
-    if args.launch is None and not args.autotune:
+    if launch is None and not args.autotune:
         # 7-row bands (18 strips x 147 bands, one round on 3072 slots), three waves per SIMD: the
         # fastest shape in 200-step runs (the tuner ranks shapes on the initial data, where the
         # per-step cost differs)
-        args.launch = '7,3,4'
-    if args.launch:
-        shape = [int(v) for v in args.launch.split(',')]
+        launch = '7,3,4'
+    if launch:
+        shape = [int(v) for v in launch.split(',')]
         for pr_ in {prec, _lib.F64}:
             _lib.check(L.claw2d_sharpclaw_set_launch_ex(ctx.handle, g, r, pr_, shape[0], shape[1],
                                                         shape[2] if len(shape) > 2 else 4),
@@ -424,18 +564,20 @@ def bench_sharpclaw(args):
                                             bufs[(k + 1) % 2].data_ptr(), s1a.data_ptr(), s1b.data_ptr(),
                                             s2.data_ptr(), None, dtd.data_ptr(), cfl.data_ptr(), sh)
         _lib.check(rc, 'ssp104')
-    for k in range(args.warmup):
+    for k in range(warmup):
         step(k)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    for k in range(args.steps):
-        step(args.warmup + k)
+    for k in range(steps):
+        step(warmup + k)
     e1.record()
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    fin = pair[(args.warmup + args.steps) % 2][:, ng:ng + my, ng:ng + mx].to('cpu').numpy()
-    tot1 = np.array([np.sum(fin[m], dtype=np.float64) for m in range(3)])
+    ms = e0.elapsed_time(e1) / steps
+    fin = pair[(warmup + steps) % 2][:, ng:ng + my, ng:ng + mx].to('cpu').numpy()
+    import math
+    tot0 = np.array([math.fsum(q0[m].ravel()) for m in range(3)])
+    tot1 = np.array([math.fsum(fin[m].ravel()) for m in range(3)])
     h_rel = abs(tot1[0] - tot0[0]) / tot0[0]
     mom_rel = np.abs(tot1[1:] - tot0[1:]) / tot0[0]
     cmp = None
@@ -451,27 +593,27 @@ def bench_sharpclaw(args):
         ref = [_grid_bufs(q0, ng, torch.float64, dev), None]
         ref[1] = torch.zeros_like(ref[0])
         f0 = torch.cuda.Event(enable_timing=True); f1 = torch.cuda.Event(enable_timing=True)
-        for k in range(args.warmup + args.steps):
-            if k == args.warmup:
+        for k in range(warmup + steps):
+            if k == warmup:
                 torch.cuda.synchronize()
                 f0.record()
             step(k, _lib.F64, ref)
         f1.record()
         torch.cuda.synchronize()
-        ref_ms = f0.elapsed_time(f1) / args.steps
-        qf = pair[(args.warmup + args.steps) % 2]
-        after = per_field(qf, ref[(args.warmup + args.steps) % 2])
+        ref_ms = f0.elapsed_time(f1) / steps
+        qf = pair[(warmup + steps) % 2]
+        after = per_field(qf, ref[(warmup + steps) % 2])
         te, tf = [qf.clone(), torch.zeros_like(qf)], [qf.clone(), torch.zeros_like(qf)]
         step(0, _lib.F64, te)
         step(0, _lib.F64_FMA, tf)
         cmp = {"reference_arithmetic_gcell_s": mx * my / (ref_ms * 1e-3) / 1e9,
                "reference_arithmetic_ms_per_step": ref_ms,
                "one_step_rel_linf_per_field": per_field(tf[1], te[1]),
-               "after_%d_steps_rel_linf_per_field" % (args.warmup + args.steps): after}
+               "after_%d_steps_rel_linf_per_field" % (warmup + steps): after}
     bytes_step = 552
     achieved = bytes_step * mx * my / (ms * 1e-3) / 1e9
     out = {"metric": "Gcell-updates/s", "value": mx * my / (ms * 1e-3) / 1e9, "unit": "Gcell-updates/s",
-           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded Gaussian hump)",
            "config": {"workload": "SharpClaw shallow water 1024x1024 periodic, WENO5 comp-wise + SSP104 (10 fused "
                                   "stage launches per step), CFL 0.5%s" % (
@@ -480,9 +622,14 @@ def bench_sharpclaw(args):
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": bytes_step},
            "extra": {"mass_rel_change": float(h_rel), "momentum_rel_change_over_mass": mom_rel.tolist(),
+                     "totals": "compensated sums (math.fsum); momenta relative to total h",
                      "cfl_last_stage": float(cfl[9].item()), "band_rows": band,
                      "fma_vs_reference_arithmetic": cmp}}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_sharpclaw(args):
+    print(json.dumps(measure_sharpclaw(args)), flush=True)
 
 
 def bench_acoustics(args):
@@ -690,7 +837,7 @@ def bench_sharpclaw_app(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_adjoint(args):
+def measure_adjoint(args):
     """Gradient of the receiver misfit of 2D vc acoustics at 512x512 (2000 steps
     at dt = 0.8 dx / max c, MC limiter, trans_inc, extrapolation boundaries;
     K in [1,4], rho in [1,2] smooth random fields; 64 receivers on y = 0.95;
@@ -787,7 +934,11 @@ def bench_adjoint(args):
                      "all_states_in_hbm": {"forward_s": t_fwd_all, "reverse_s": t_bwd_all,
                                            "gradient_over_forward": (t_fwd_all + t_bwd_all) / t_fwd,
                                            "bitwise_equal_to_checkpointed": same_all}}}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_adjoint(args):
+    print(json.dumps(measure_adjoint(args)), flush=True)
 
 
 def bench_copy(args):

@@ -175,74 +175,101 @@ class BatchedClawSolver2D(object):
             pass
 
 
-def run_one_by_one(riemann_solver, problem_data, instances, nsteps, order=2, transverse=1, limiters=4,
-                   bc_lower=(1, 1), bc_upper=(1, 1), aux_bc_lower=(1, 1), aux_bc_upper=(1, 1), cfl_max=1.0,
-                   cfl_desired=0.9, dt_max=1e99, dtype=torch.float64, keep=True):
+class OneByOne(object):
     """The comparison path: each instance stepped alone with the plain
     single-grid entry point (claw2d_classic_step) and the same device-side dt
-    rule (claw2d_step_update), one instance after another.  Buffers ping-pong
-    on the host side, which is valid while no step is rejected (checked)."""
-    L = _bind()
-    dev = torch.device('cuda', 0)
-    ng = NG_STORE
-    ctx = _lib.Context(0)
-    m = _lib.ClawMethod()
-    m.order, m.transverse, m.split, m.fwave, m.num_waves = order, transverse, 0, 0, riemann_solver.num_waves
-    lim = limiters if isinstance(limiters, (list, tuple)) else [limiters] * riemann_solver.num_waves
-    for k, v in enumerate(lim):
-        m.mthlim[k] = int(v)
-    r = _lib.ClawRiemann()
-    r.kind, r.efix = riemann_solver.kind, 1
-    for k, v in enumerate(riemann_solver.params(problem_data)):
-        r.p[k] = v
-    bc = _lib.ClawBC()
-    for s in range(2):
-        bc.lower[s], bc.upper[s] = bc_lower[s], bc_upper[s]
-        bc.aux_lower[s], bc.aux_upper[s] = aux_bc_lower[s], aux_bc_upper[s]
-    prec = _lib.F32 if dtype == torch.float32 else _lib.F64
-    st = torch.cuda.current_stream().cuda_stream
-    outs = []
-    bufs = []
-    for inst in instances:
-        q = np.asarray(inst['q'], dtype=np.float64)
-        ne, mx, my = q.shape
-        b0 = torch.zeros((ne, my + 2 * ng, mx + 2 * ng), dtype=dtype)
-        b0[:, ng:ng + my, ng:ng + mx] = torch.from_numpy(np.ascontiguousarray(q.transpose(0, 2, 1)))
-        b0 = b0.to(dev)
-        b1 = b0.clone()
-        aux = inst.get('aux')
-        ad = None
-        if aux is not None:
-            ad = torch.zeros((aux.shape[0], my + 2 * ng, mx + 2 * ng), dtype=dtype)
-            ad[:, ng:ng + my, ng:ng + mx] = torch.from_numpy(np.ascontiguousarray(np.asarray(aux).transpose(0, 2, 1)))
-            ad = ad.to(dev)
-        bufs.append((b0, b1, ad, mx, my, float(inst['dx']), float(inst['dy']), float(inst['dt_initial'])))
-    f64 = dict(dtype=torch.float64, device=dev)
-    i32 = dict(dtype=torch.int32, device=dev)
-    stats = []
-    for (b0, b1, ad, mx, my, dx, dy, dt0) in bufs:
-        g = _lib.ClawGrid(mx, my, ng, 0, mx + 2 * ng, (mx + 2 * ng) * (my + 2 * ng), dx, dy)
-        dt = torch.tensor([dt0], **f64)
-        cfl = torch.zeros(1, **f64)
-        t = torch.zeros(1, **f64)
-        cmax = torch.zeros(1, **f64)
-        par = torch.zeros(1, **i32)
-        nacc = torch.zeros(1, **i32)
-        nrej = torch.zeros(1, **i32)
-        pair = (b0, b1)
-        for s in range(nsteps):
-            rc = L.claw2d_classic_step(ctx.handle, g, m, r, bc, prec, pair[s % 2].data_ptr(),
-                                       pair[(s + 1) % 2].data_ptr(), ad.data_ptr() if ad is not None else None,
-                                       dt.data_ptr(), cfl.data_ptr(), st)
-            _lib.check(rc, "claw2d_classic_step")
-            rc = L.claw2d_step_update(1, cfl_max, cfl_desired, dt_max, dt.data_ptr(), cfl.data_ptr(), t.data_ptr(),
-                                      cmax.data_ptr(), par.data_ptr(), nacc.data_ptr(), nrej.data_ptr(), st)
-            _lib.check(rc, "claw2d_step_update")
-        stats.append((pair[nsteps % 2], nrej, mx, my, t))
-    torch.cuda.synchronize()
-    for (buf, nrej, mx, my, t) in stats:
-        if int(nrej.item()) != 0:
-            raise RuntimeError("a step was rejected; the one-by-one comparison path assumes none")
-        if keep:
+    rule (claw2d_step_update), one instance after another.  The constructor
+    uploads every instance (device buffers, per-instance dt/cfl/t slots);
+    run(nsteps) only queues the steps, so the two phases can be timed apart.
+    Buffers ping-pong on the host side, which is valid while no step is
+    rejected (checked by results())."""
+
+    def __init__(self, riemann_solver, problem_data, instances, order=2, transverse=1, limiters=4,
+                 bc_lower=(1, 1), bc_upper=(1, 1), aux_bc_lower=(1, 1), aux_bc_upper=(1, 1), cfl_max=1.0,
+                 cfl_desired=0.9, dt_max=1e99, dtype=torch.float64):
+        self.L = _bind()
+        dev = torch.device('cuda', 0)
+        ng = NG_STORE
+        self.ctx = _lib.Context(0)
+        m = _lib.ClawMethod()
+        m.order, m.transverse, m.split, m.fwave, m.num_waves = order, transverse, 0, 0, riemann_solver.num_waves
+        lim = limiters if isinstance(limiters, (list, tuple)) else [limiters] * riemann_solver.num_waves
+        for k, v in enumerate(lim):
+            m.mthlim[k] = int(v)
+        r = _lib.ClawRiemann()
+        r.kind, r.efix = riemann_solver.kind, 1
+        for k, v in enumerate(riemann_solver.params(problem_data)):
+            r.p[k] = v
+        bc = _lib.ClawBC()
+        for s in range(2):
+            bc.lower[s], bc.upper[s] = bc_lower[s], bc_upper[s]
+            bc.aux_lower[s], bc.aux_upper[s] = aux_bc_lower[s], aux_bc_upper[s]
+        self.m, self.r, self.bc = m, r, bc
+        self.rule = (cfl_max, cfl_desired, dt_max)
+        self.prec = _lib.F32 if dtype == torch.float32 else _lib.F64
+        f64 = dict(dtype=torch.float64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.items = []
+        for inst in instances:
+            q = np.asarray(inst['q'], dtype=np.float64)
+            ne, mx, my = q.shape
+            b0 = torch.zeros((ne, my + 2 * ng, mx + 2 * ng), dtype=dtype)
+            b0[:, ng:ng + my, ng:ng + mx] = torch.from_numpy(np.ascontiguousarray(q.transpose(0, 2, 1)))
+            b0 = b0.to(dev)
+            b1 = b0.clone()
+            aux = inst.get('aux')
+            ad = None
+            if aux is not None:
+                ad = torch.zeros((aux.shape[0], my + 2 * ng, mx + 2 * ng), dtype=dtype)
+                ad[:, ng:ng + my, ng:ng + mx] = torch.from_numpy(
+                    np.ascontiguousarray(np.asarray(aux).transpose(0, 2, 1)))
+                ad = ad.to(dev)
+            g = _lib.ClawGrid(mx, my, ng, 0, mx + 2 * ng, (mx + 2 * ng) * (my + 2 * ng), float(inst['dx']),
+                              float(inst['dy']))
+            slots = dict(dt=torch.tensor([float(inst['dt_initial'])], **f64), cfl=torch.zeros(1, **f64),
+                         t=torch.zeros(1, **f64), cmax=torch.zeros(1, **f64), par=torch.zeros(1, **i32),
+                         nacc=torch.zeros(1, **i32), nrej=torch.zeros(1, **i32))
+            self.items.append(dict(pair=(b0, b1), aux=ad, grid=g, mx=mx, my=my, **slots))
+        self.nsteps = 0
+        torch.cuda.synchronize()
+
+    def run(self, nsteps):
+        """Queue nsteps steps of every instance, one instance after another."""
+        L, m, r, bc, prec = self.L, self.m, self.r, self.bc, self.prec
+        cfl_max, cfl_desired, dt_max = self.rule
+        st = torch.cuda.current_stream().cuda_stream
+        for it in self.items:
+            pair, ad, g = it['pair'], it['aux'], it['grid']
+            aptr = ad.data_ptr() if ad is not None else None
+            dtp, cflp = it['dt'].data_ptr(), it['cfl'].data_ptr()
+            for s in range(self.nsteps, self.nsteps + nsteps):
+                rc = L.claw2d_classic_step(self.ctx.handle, g, m, r, bc, prec, pair[s % 2].data_ptr(),
+                                           pair[(s + 1) % 2].data_ptr(), aptr, dtp, cflp, st)
+                _lib.check(rc, "claw2d_classic_step")
+                rc = L.claw2d_step_update(1, cfl_max, cfl_desired, dt_max, dtp, cflp, it['t'].data_ptr(),
+                                          it['cmax'].data_ptr(), it['par'].data_ptr(), it['nacc'].data_ptr(),
+                                          it['nrej'].data_ptr(), st)
+                _lib.check(rc, "claw2d_step_update")
+        self.nsteps += nsteps
+
+    def results(self):
+        """Each instance's state (num_eqn, mx, my) on the host, after checking
+        that no step was rejected."""
+        torch.cuda.synchronize()
+        ng = NG_STORE
+        outs = []
+        for it in self.items:
+            if int(it['nrej'].item()) != 0:
+                raise RuntimeError("a step was rejected; the one-by-one comparison path assumes none")
+            buf, mx, my = it['pair'][self.nsteps % 2], it['mx'], it['my']
             outs.append(buf[:, ng:ng + my, ng:ng + mx].permute(0, 2, 1).to('cpu', torch.float64).numpy().copy())
-    return outs
+        return outs
+
+
+def run_one_by_one(riemann_solver, problem_data, instances, nsteps, keep=True, **kw):
+    """OneByOne set up and run for nsteps; the instances' final states (keep)
+    or None."""
+    ob = OneByOne(riemann_solver, problem_data, instances, **kw)
+    ob.run(nsteps)
+    out = ob.results()
+    return out if keep else None
