@@ -22,7 +22,7 @@ class Problem(object):
                  aux_bc_lower=(1, 1), aux_bc_upper=(1, 1), user_bc_lower=None, user_bc_upper=None,
                  kind='classic', dimensional_split=True, transverse=1, order=2, limiters=1,
                  cfl_max=1.0, cfl_desired=0.9, dt_initial=0.1, dt_max=1e99, dt_variable=True,
-                 max_steps=10000, step_source=None, source_split=1):
+                 max_steps=10000, step_source=None, source_split=1, time_integrator='SSP104'):
         self.__dict__.update(locals())
         del self.__dict__['self']
 
@@ -107,8 +107,10 @@ class _CFLError(Exception):
 
 
 def sharp_step(P, q, dt, t):
-    """SSP104 (sharpclaw/solver.py:221-251).  Returns (q, cfl) where cfl is the
-    last dq's Courant number, or the one that exceeded cfl_max."""
+    """SharpClawSolver.step (sharpclaw/solver.py:185-255): Euler (:199-201),
+    SSP33 (:203-218) or SSP104 (:221-251).  Returns (q, cfl) where cfl is the
+    last dq's Courant number, or the one that exceeded cfl_max (the reference's
+    dq raises CFLError there, :272-275, and q is left as it was)."""
     state = {'cfl': None}
 
     def dq(qs, ts):
@@ -118,6 +120,23 @@ def sharp_step(P, q, dt, t):
             raise _CFLError()
         return d
 
+    ti = getattr(P, 'time_integrator', 'SSP104')
+    if ti == 'Euler':
+        try:
+            q = q + dq(q, t)
+        except _CFLError:
+            pass
+        return np.asfortranarray(q), state['cfl']
+    if ti == 'SSP33':
+        try:
+            s1 = q + dq(q, t)
+            s1 = 0.75 * q + 0.25 * (s1 + dq(s1, t + dt))
+            q = 1. / 3. * q + 2. / 3. * (s1 + dq(s1, t + 0.5 * dt))
+        except _CFLError:
+            pass
+        return np.asfortranarray(q), state['cfl']
+    if ti != 'SSP104':
+        raise Exception('Unrecognized time integrator')
     try:
         s1 = q + dq(q, t) / 6.
         t1 = t + dt / 6.

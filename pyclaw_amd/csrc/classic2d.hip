@@ -1601,10 +1601,12 @@ int claw2d_loop_advance(const ClawLoopRule *rule, const ClawLoopState *cur, Claw
     if (nbytes <= 0 || nbytes % 4 || ((uintptr_t)qold | (uintptr_t)qnew) % 16) return CLAW_EINVAL;
     const long long n16 = nbytes / 16;
     const int ntail = (int)(nbytes % 16) / 4;
-    // a small grid-stride launch: accepted steps (nearly all) only read two
-    // scalars per block; the copy runs on rejected and trailing iterations
+    // a grid-stride launch sized for the copy (rejected and trailing
+    // iterations restore the step's input): one 16-byte chunk per thread up to
+    // 1024 blocks, enough to run the copy at HBM speed; accepted steps (nearly
+    // all) only read two scalars per block
     const long long want = (n16 + 255) / 256;
-    const int nb = (int)std::max<long long>(1, std::min<long long>(want, 64));
+    const int nb = (int)std::max<long long>(1, std::min<long long>(want, 1024));
     hipLaunchKernelGGL(loop_advance_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, *rule, cur, next, cfl_cur,
                        cfl_next, (const uint4 *)qold, (uint4 *)qnew, n16,
                        (const unsigned *)qold + n16 * 4, (unsigned *)qnew + n16 * 4, ntail);

@@ -355,9 +355,11 @@ class Solver(object):
 
     def _device_loop_launcher(self, solution, take_one_step):
         """The device step of _loop_launcher, or None for the host loop.  The
-        step loop runs on the device when no step needs the host: no gauges, no per-step debug log, no user hooks or ghost
-        cells filled by user functions each step, and a solver with a device
-        step (_loop_launcher)."""
+        step loop runs on the device when no step needs the host: gauges only
+        with the default compute_gauge_values on an FP64 state (recorded on the
+        device), no per-step debug log, no user hooks or ghost cells filled by
+        user functions each step, and a solver with a device step
+        (_loop_launcher)."""
         if not self.device_loop or take_one_step or self.max_steps <= 0:
             return None
         state = solution.state

@@ -28,7 +28,8 @@ def oracle_run(q0, aux, opts, step_source=None, user_bc_lower=None):
                        dimensional_split=opts.get('dimensional_split', True),
                        transverse=opts.get('transverse', 1), limiters=opts['limiters'],
                        cfl_max=opts['cfl_max'], cfl_desired=opts['cfl_desired'], dt_initial=opts['dt_initial'],
-                       step_source=step_source, source_split=opts.get('source_split', 1))
+                       step_source=step_source, source_split=opts.get('source_split', 1),
+                       time_integrator=opts.get('time_integrator', 'SSP104'))
     run = driver.Run(P, q0)
     frames = run.run(opts['tfinal'], opts['num_output_times'])
     return frames, run

@@ -49,6 +49,7 @@ def amd_controller(q0, aux, opts, step_source=None, user_bc_lower=None, autotune
             solver.step_source = step_source
     else:
         solver = pc.SharpClawSolver2D(rp)
+        solver.time_integrator = opts.get('time_integrator', 'SSP104')
         if dq_src is not None:
             solver.dq_src = dq_src
     solver.limiters = opts['limiters']
@@ -160,3 +161,29 @@ def test_fma_app_frames_close(kind, tol):
     print("fma app worst frame rel Linf", worst, flush=True)
     assert worst <= tol
     assert any(not np.array_equal(a, b) for a, b in zip(got, ref))
+
+
+@pytest.mark.parametrize("integrator", ['Euler', 'SSP33'])
+@pytest.mark.parametrize("app", ['acoustics_homogeneous', 'shallow_hump', 'shallow_hump_rejected_first_step'])
+def test_sharpclaw_euler_and_ssp33_frames_equal_oracle(integrator, app):
+    """SharpClawSolver2D.time_integrator 'Euler' and 'SSP33'
+    (sharpclaw/solver.py:199-218) through the drop-in API: every frame equal to
+    the oracle's run, fused-stage path and the stage-by-stage path (taken when
+    a hook runs before each stage)."""
+    print("app: sharpclaw %s %s" % (integrator, app), flush=True)
+    if app == 'acoustics_homogeneous':
+        q0, aux, opts = apps.acoustics_homogeneous('sharpclaw')
+        cfl = dict(cfl_max=0.5, cfl_desired=0.45) if integrator == 'Euler' else {}
+    else:
+        q0, aux, opts = apps.config3_shallow(96, nsteps=24, num_output_times=3)
+        cfl = dict(cfl_max=0.45, cfl_desired=0.4) if integrator == 'Euler' else dict(cfl_max=1.0, cfl_desired=0.9)
+    opts = dict(opts, time_integrator=integrator, **cfl)
+    if app.endswith('rejected_first_step'):
+        opts['dt_initial'] = 10 * opts['dt_initial']
+    ref, run = oracle_run(q0, aux, opts)
+    assert run.nsteps >= 3
+    got, _ = amd_run(q0, aux, opts)
+    _compare_frames(got, ref)
+    staged = amd_controller(q0, aux, opts, call_before_step_each_stage=True)
+    staged.run()
+    _compare_frames([f.state.q.copy() for f in staged.frames], ref)

@@ -286,6 +286,45 @@ def test_weno5_polynomial_exactness_and_mirror():
     assert l3[0, 2:18].max() <= 1.0 + 1e-6 and r3[0, 2:18].min() >= -1e-6
 
 
+def _stability_function(integrator, z):
+    """R(z) of the oracle's SharpClaw step on q' = z q (one step, dt = 1)."""
+    from oracle import driver
+
+    class P:
+        cfl_max = 1e9
+        time_integrator = integrator
+    orig = driver.sharp_dq
+    driver.sharp_dq = lambda P_, q, dt, t: (z * q, 0.0)
+    try:
+        q, _ = driver.sharp_step(P, np.ones((1, 1, 1)), 1.0, 0.0)
+    finally:
+        driver.sharp_dq = orig
+    return float(q.ravel()[0])
+
+
+def test_euler_and_ssp33_stability_polynomials():
+    """sharpclaw/solver.py:199-218: forward Euler is 1 + z; Shu-Osher SSP33 is
+    the cubic Taylor polynomial 1 + z + z^2/2 + z^3/6 (third order), with the
+    strong-stability interval [-1, 0] of coefficient 1."""
+    for z in [-1.0, -0.7, -0.3, 0.1, 0.4]:
+        assert _stability_function('Euler', z) == 1.0 + z
+        assert abs(_stability_function('SSP33', z) - (1 + z + z * z / 2 + z ** 3 / 6)) <= 1e-15
+    e1 = abs(_stability_function('SSP33', 0.2) - np.exp(0.2))
+    e2 = abs(_stability_function('SSP33', 0.1) - np.exp(0.1))
+    assert 12 < e1 / e2 < 20      # ratio 2^4 for a third-order method
+    assert abs(_stability_function('SSP33', -1.0)) <= 1.0
+
+
+def test_unknown_integrator_raises():
+    from oracle import driver
+
+    class P:
+        cfl_max = 1e9
+        time_integrator = 'RK45'
+    with pytest.raises(Exception, match='Unrecognized time integrator'):
+        driver.sharp_step(P, np.ones((1, 1, 1)), 1.0, 0.0)
+
+
 def test_ssp104_fourth_order():
     """The SSP104 recurrences of the reference (sharpclaw/solver.py:221-251),
     as restated in oracle.driver.sharp_step, applied to q' = lambda q, must
