@@ -397,7 +397,7 @@ def main():
                         ("batched_vs_plain_speedup", lambda: measure_batched(args, nsteps=200)),
                         ("adjoint_over_forward", lambda: measure_adjoint(args)),
                         ("sharpclaw_shallow_1024", lambda: measure_sharpclaw(args, steps=500, warmup=5,
-                                                                             precision='f64', launch='7,3,4'))):
+                                                                             precision='f64'))):
             t0 = time.perf_counter()
             try:
                 r_ = fn()
@@ -504,8 +504,8 @@ def measure_sharpclaw(args, steps=None, warmup=None, precision=None, launch=None
     import torch
     from pyclaw_amd import _lib
     from pyclaw_amd.problems import shallow_hump
-    steps = steps if steps is None else steps
-    warmup = warmup if warmup is None else warmup
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     precision = args.precision if precision is None else precision
     launch = args.launch if launch is None else launch
     mx = my = 1024
@@ -536,18 +536,15 @@ def measure_sharpclaw(args, steps=None, warmup=None, precision=None, launch=None
         raise SystemExit("the sharpclaw workload runs FP64 (--precision f64fma or f64)")
     prec = _lib.F64_FMA if precision == 'f64fma' else _lib.F64
     band = None
-    if launch is None and not args.autotune:
-        # 7-row bands (18 strips x 147 bands, one round on 3072 slots), three waves per SIMD: the
-        # fastest shape in 200-step runs (the tuner ranks shapes on the initial data, where the
-        # per-step cost differs)
-        launch = '7,3,4'
+    # without --launch or --autotune: the library's default shape for this grid (the fill
+    # decomposition, one wavefront per slot of the default build)
     if launch:
         shape = [int(v) for v in launch.split(',')]
         for pr_ in {prec, _lib.F64}:
             _lib.check(L.claw2d_sharpclaw_set_launch_ex(ctx.handle, g, r, pr_, shape[0], shape[1],
                                                         shape[2] if len(shape) > 2 else 4),
                        'claw2d_sharpclaw_set_launch_ex')
-    elif not args.no_autotune:
+    elif args.autotune:
         for pr_ in ([prec, _lib.F64] if prec != _lib.F64 else [prec]):
             hb = ctypes.c_int(0)
             _lib.check(L.claw2d_sharpclaw_autotune(ctx.handle, g, r, bc, pr_, q.data_ptr(), qn.data_ptr(), None,

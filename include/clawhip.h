@@ -221,19 +221,25 @@ int claw2d_sharpclaw_ssp104_step(void *ctx, const ClawGrid *grid, const ClawRiem
 
 /* Autotuner of the SharpClaw stage kernel's row-band height (the same idea as
    claw2d_classic_autotune): times dq evaluations of q into dq for heights
-   4..64, each with the kernel built for the default occupancy and for three
-   wavefronts per SIMD, and keeps the fastest for this context and shape;
-   every later SharpClaw call of that shape (dq, stage, SSP104 step) uses it.
-   Results do not depend on it.  Overwrites dq and *cfl_dev.  *band_out
-   (optional) receives the height, negated when the three-wave build won.
-   No counterpart in the reference. */
+   4..64 and the fill decomposition (band height 0, below), each with the
+   kernel built for the default occupancy and for three wavefronts per SIMD,
+   and keeps the fastest for this context and shape; every later SharpClaw
+   call of that shape (dq, stage, SSP104 step) uses it.  Results do not
+   depend on it.  Overwrites dq and *cfl_dev.  *band_out (optional) receives
+   the height, negated when the three-wave build won.  No counterpart in the
+   reference. */
 int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, const ClawBC *bc,
                               int precision, const void *q, void *dq, const void *aux,
                               const double *dt_dev, double *cfl_dev, void *stream, int *band_out);
 
 /* Sets the SharpClaw launch shape directly: row-band height `band` (>= 1) and
    the build for the default occupancy (waves_per_simd 1) or for three
-   wavefronts per SIMD (3).  No counterpart in the reference. */
+   wavefronts per SIMD (3).  band = 0 is the fill decomposition: one
+   wavefront per slot of the build (CUs x the wavefronts per CU that the
+   runtime's occupancy query gives for the kernel), each taking an equal
+   share of the grid's rows strip by strip (the default for large grids, with
+   the default build in one-wavefront blocks).  No counterpart in the
+   reference. */
 int claw2d_sharpclaw_set_launch(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision,
                                 int band, int waves_per_simd);
 

@@ -24,6 +24,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "common.hpp"
+
 namespace clawhip {
 
 enum RiemannKind { RP_ACOUSTICS = 0, RP_VC_ACOUSTICS = 1, RP_EULER = 2, RP_SHALLOW = 3 };
@@ -83,6 +85,55 @@ template <> __device__ __forceinline__ double rsqrt_exact<double>(double x) {
 }
 template <> __device__ __forceinline__ float rsqrt_exact<float>(float x) { return __builtin_sqrtf(x); }
 
+// Two quotients with one denominator, q1 = n1/d and q2 = n2/d.  The
+// shared-reciprocal build multiplies both numerators by rcp(d).  Otherwise
+// each quotient is the correctly rounded IEEE quotient, equal bit for bit to
+// n/d: the FP64 division sequence (scale the denominator, hardware
+// reciprocal, two Newton steps; scale the numerator, multiply, residual,
+// div_fmas, div_fixup) has a part that depends on the denominator's scaled
+// value only, and that part is done once when both numerators leave the
+// denominator unscaled alike (every pair except where the quotients come near
+// the ends of the exponent range; then n2/d is divided on its own).
+template <typename R>
+__device__ __forceinline__ void div2(R n1, R n2, R d, R &q1, R &q2) {
+    if constexpr (SHARED_RCP) {
+        const R r = rcp<R>(d);
+        q1 = n1 * r;
+        q2 = n2 * r;
+    } else {
+        q1 = n1 / d;
+        q2 = n2 / d;
+    }
+}
+template <> __device__ __forceinline__ void div2<double>(double n1, double n2, double d, double &q1, double &q2) {
+    if constexpr (SHARED_RCP) {
+        const double r = rcp<double>(d);
+        q1 = n1 * r;
+        q2 = n2 * r;
+    } else {
+        bool g1, g2, f1, f2;
+        const double ds1 = __builtin_amdgcn_div_scale(n1, d, false, &g1);
+        const double ds2 = __builtin_amdgcn_div_scale(n2, d, false, &g2);
+        const double r0 = __builtin_amdgcn_rcp(ds1);
+        const double e0 = __builtin_fma(-ds1, r0, 1.0);
+        const double r1 = __builtin_fma(r0, e0, r0);
+        const double e1 = __builtin_fma(-ds1, r1, 1.0);
+        const double r2 = __builtin_fma(r1, e1, r1);
+        const double ns1 = __builtin_amdgcn_div_scale(n1, d, true, &f1);
+        const double m1 = ns1 * r2;
+        const double rem1 = __builtin_fma(-ds1, m1, ns1);
+        q1 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem1, r2, m1, f1), d, n1);
+        if (__double_as_longlong(ds2) == __double_as_longlong(ds1)) {
+            const double ns2 = __builtin_amdgcn_div_scale(n2, d, true, &f2);
+            const double m2 = ns2 * r2;
+            const double rem2 = __builtin_fma(-ds1, m2, ns2);
+            q2 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem2, r2, m2, f2), d, n2);
+        } else {
+            q2 = n2 / d;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Constant-coefficient acoustics: q = (p, u, v), parameters rho, bulk, c, Z.
 // ---------------------------------------------------------------------------
@@ -92,6 +143,7 @@ template <typename R> struct Acoustics {
     struct Params { R rho, bulk, cc, zz; };
     struct Cache {};
     static constexpr bool SPEEDS_FROM_CACHE = false;
+    static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
     __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
@@ -133,6 +185,7 @@ template <typename R> struct VCAcoustics {
     struct Params { R unused; };
     struct Cache {};
     static constexpr bool SPEEDS_FROM_CACHE = false;
+    static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
     __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
@@ -186,6 +239,7 @@ template <typename R> struct Euler5 {
     struct Cache { R u, v, enth, a; };
     // the wave speeds of an interface follow from its cache (rpn's expressions)
     static constexpr bool SPEEDS_FROM_CACHE = true;
+    static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
     __device__ static __forceinline__ void speeds(const Cache &C, R (&s)[NW]) {
         s[0] = C.u - C.a; s[1] = C.u; s[2] = C.u; s[3] = C.u + C.a; s[4] = C.u;
     }
@@ -413,39 +467,56 @@ template <typename R> struct ShallowRoe {
     template <int DIR> __host__ __device__ static constexpr bool WNZ(int m, int mw) {
         return mw != 1 || m == (DIR == 0 ? 2 : 1);
     }
+    // The quantities of one state that each Riemann problem it is part of (as
+    // the left or the right state) computes: sqrt(h); hu/sqrt(h) and
+    // hv/sqrt(h), the terms of the Roe velocities; hu/h and sqrt(g h), the
+    // entropy fix's characteristic speed.  A caller that meets a state in two
+    // problems (SharpClaw's interface and in-cell solves) computes them once;
+    // each is the same operation on the same operands either way.
+    struct State { R hs, umh, vmh, un, sgh; };
+    static constexpr bool HAS_STATE = true;
+    // a state record moved one lane up (lane L receives lane L-1's)
+    __device__ static __forceinline__ State state_up(const State &S) {
+        return State{lane_up(S.hs), lane_up(S.umh), lane_up(S.vmh), lane_up(S.un), lane_up(S.sgh)};
+    }
+    template <int DIR>
+    __device__ static __forceinline__ State state(const Params &P, const R (&q)[NE]) {
+        constexpr int mu = MU<DIR>, mv = MV<DIR>;
+        State S;
+        S.hs = rsqrt_exact<R>(q[0]);
+        div2<R>(q[mu], q[mv], S.hs, S.umh, S.vmh);
+        S.un = SHARED_RCP ? q[mu] * rcp<R>(q[0]) : q[mu] / q[0];
+        const R gh = P.grav * q[0];
+        S.sgh = S.hs;
+        if (gh != q[0]) S.sgh = rsqrt_exact<R>(gh);   // g h equal to h (g = 1): sqrt(g h) is sqrt(h)
+        return S;
+    }
     template <int DIR>
     __device__ static __forceinline__ void rpn(const Params &P, const R (&lo)[NE], const R (&hi)[NE],
                                                const R *, const R *, R (&wave)[NW][NE], R (&s)[NW],
                                                R (&amdq)[NE], R (&apdq)[NE], Cache &C) {
+        rpn_s<DIR>(P, lo, hi, state<DIR>(P, lo), state<DIR>(P, hi), wave, s, amdq, apdq, C);
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpn_s(const Params &P, const R (&lo)[NE], const R (&hi)[NE],
+                                                 const State &Sl, const State &Sr, R (&wave)[NW][NE], R (&s)[NW],
+                                                 R (&amdq)[NE], R (&apdq)[NE], Cache &C) {
         constexpr int mu = MU<DIR>, mv = MV<DIR>;
         const R grav = P.grav;
         R hl = lo[0], hr = hi[0];
         R hbar = (hl + hr) * R(0.5);
-        R hsqrtl = rsqrt_exact<R>(hl), hsqrtr = rsqrt_exact<R>(hr);
+        R hsqrtl = Sl.hs, hsqrtr = Sr.hs;
         R hsq2 = hsqrtl + hsqrtr;
+        // u = (hu_l/sqrt(h_l) + hu_r/sqrt(h_r)) / (sqrt(h_l) + sqrt(h_r)), v alike
         R u, v;
-        if constexpr (SHARED_RCP) {
-            const R il = rcp<R>(hsqrtl), ir = rcp<R>(hsqrtr), is = rcp<R>(hsq2);
-            u = (lo[mu] * il + hi[mu] * ir) * is;
-            v = (lo[mv] * il + hi[mv] * ir) * is;
-        } else {
-            u = (lo[mu] / hsqrtl + hi[mu] / hsqrtr) / hsq2;
-            v = (lo[mv] / hsqrtl + hi[mv] / hsqrtr) / hsq2;
-        }
+        div2<R>(Sl.umh + Sr.umh, Sl.vmh + Sr.vmh, hsq2, u, v);
         R a = rsqrt_exact<R>(grav * hbar);
         C.u = u; C.v = v; C.a = a;
         R d0 = hr - hl;
         R d1 = hi[mu] - lo[mu];
         R d2 = hi[mv] - lo[mv];
         R a1, a3;
-        if constexpr (SHARED_RCP) {
-            const R i2a = rcp<R>(R(2) * a);
-            a1 = ((u + a) * d0 - d1) * i2a;
-            a3 = (-(u - a) * d0 + d1) * i2a;
-        } else {
-            a1 = ((u + a) * d0 - d1) / (R(2) * a);
-            a3 = (-(u - a) * d0 + d1) / (R(2) * a);
-        }
+        div2<R>((u + a) * d0 - d1, -(u - a) * d0 + d1, R(2) * a, a1, a3);
         R a2 = -v * d0 + d2;
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
         wave[1][0] = R(0); wave[1][mu] = R(0); wave[1][mv] = a2; s[1] = u;
@@ -453,7 +524,7 @@ template <typename R> struct ShallowRoe {
         // depths are positive: the shared-reciprocal build multiplies by 1/h
         do {
             R sfract;
-            R s0 = (SHARED_RCP ? lo[mu] * rcp<R>(hl) : lo[mu] / hl) - rsqrt_exact<R>(grav * hl);
+            R s0 = Sl.un - Sl.sgh;
             if (s0 >= R(0) && s[0] > R(0)) {
 #pragma unroll
                 for (int m = 0; m < NE; ++m) amdq[m] = R(0);
@@ -472,7 +543,7 @@ template <typename R> struct ShallowRoe {
 #pragma unroll
             for (int m = 0; m < NE; ++m)
                 if (WNZ<DIR>(m, 1)) amdq[m] = amdq[m] + s[1] * wave[1][m];
-            R s03 = (SHARED_RCP ? hi[mu] * rcp<R>(hr) : hi[mu] / hr) + rsqrt_exact<R>(grav * hr);
+            R s03 = Sr.un + Sr.sgh;
             R h3 = hr - wave[2][0];
             R hu3 = hi[mu] - wave[2][mu];
             R s3 = (SHARED_RCP ? hu3 * rcp<R>(h3) : hu3 / h3) + rsqrt_exact<R>(grav * h3);

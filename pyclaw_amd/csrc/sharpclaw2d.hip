@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <new>
 #include <type_traits>
 
@@ -53,6 +54,10 @@ enum StageMode {
     SM_SSP33_2 = 5,     // out = 0.75*q0 + 0.25*(in + dq)
     SM_SSP33_3 = 6      // out = 1./3.*q0 + 2./3.*(in + dq)
 };
+
+// The solver's per-state quantities (a placeholder type for solvers without them).
+template <class RP, bool HAS> struct StateOf { using type = char; };
+template <class RP> struct StateOf<RP, true> { using type = typename RP::State; };
 
 // weno.f90:32-79 for one component: left (ql) and right (qr) edge values of
 // the cell whose average is q0, from the averages qm2..qp2.
@@ -94,8 +99,12 @@ __device__ __forceinline__ void weno5(R qm2, R qm1, R q0, R qp1, R qp2, R &ql, R
         omega0 = R(0.1) * i0; omega1 = R(0.6) * i1; omega2 = R(0.3) * i2;
         omega3 = R(0.3) * i0; omega4 = R(0.6) * i1; omega5 = R(0.1) * i2;
     } else {
-        omega0 = R(0.1) / t00; omega1 = R(0.6) / t11; omega2 = R(0.3) / t22;
-        omega3 = R(0.3) / t00; omega4 = R(0.6) / t11; omega5 = R(0.1) / t22;
+        // omega0 = 0.1/t00, omega3 = 0.3/t00 and omega2 = 0.3/t22, omega5 =
+        // 0.1/t22 share their denominators' part of the division sequence;
+        // omega1 and omega4 are the same quotient 0.6/t11
+        div2<R>(R(0.1), R(0.3), t00, omega0, omega3);
+        omega1 = R(0.6) / t11; omega4 = omega1;
+        div2<R>(R(0.3), R(0.1), t22, omega2, omega5);
     }
     R fr0 = R(1.8333333333333333333333333333333333) * q0 - R(1.1666666666666666666666666666666667) * qp1
         + R(0.33333333333333333333333333333333333) * qp2;
@@ -133,35 +142,33 @@ struct SharpParams {
     double dx, dy;
     int mode;
     int H, nsx, nsy, nsyb, wpb;
+    int fill;         // > 0: that many wavefronts, each taking an equal share of the (strip, row) space
 };
 
+// Rows [R0, R1) of column strip sx; cfl is max-combined with the Courant
+// numbers of the interfaces this band owns.
 template <class RP, typename R>
-__device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
+__device__ __forceinline__ void sharp_band(const SharpParams<RP, R> &p, const int sx, const int R0, const int R1,
+                                           R &cfl) {
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
     constexpr int NA = HAS_AUX ? RP::NA : 1;
     using Cache = typename RP::Cache;
+    // per-state quantities shared by the two Riemann problems a reconstructed
+    // state takes part in (riemann.hpp); solvers without them recompute
+    constexpr bool ST = RP::HAS_STATE;
+    using State = typename std::conditional<ST, typename StateOf<RP, ST>::type, char>::type;
 
     const int lane = threadIdx.x & (WAVE - 1);
-    const int vb = xcd_block();
-    const int sx = vb / p.nsyb;
-    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-    if (sx >= p.nsx || sy >= p.nsy) return;
-
     const GridDev g = p.g;
     const int c0 = sx * SX_USEFUL;
     const int i = c0 - 3 + lane;
-    const int R0 = sy * p.H;
-    const int R1 = min(R0 + p.H, g.my);
     const bool produce = lane >= 3 && lane < 3 + SX_USEFUL && i < g.mx;
 
     const double dtd = *p.dt;   // uniform constants in scalar registers (whole wavefront active)
     const R dtdx = uniform(R(dtd / p.dx)), dtdy = uniform(R(dtd / p.dy));
     const auto &P = p.P;
     const int mode = p.mode;
-    // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
-    // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
-    R cfl = R(0);
 
     // five-row window of q (rows w-2..w+2) and aux (rows w-1, w)
     R win[5][NE];
@@ -175,6 +182,7 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     R qrY_prev[NE], apY_prev[NE], am2Y_prev[NE], ap2Y_prev[NE];
 #pragma unroll
     for (int m = 0; m < NE; ++m) { qrY_prev[m] = apY_prev[m] = am2Y_prev[m] = ap2Y_prev[m] = R(0); }
+    State srY_prev{};
 #pragma unroll
     for (int k = 0; k < 5; ++k)
 #pragma unroll
@@ -198,15 +206,26 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
 #pragma unroll
         for (int m = 0; m < NE; ++m) weno5<R>(win[0][m], win[1][m], win[2][m], win[3][m], win[4][m], qlY[m], qrY[m]);
         R amY[NE], apY[NE], am2Y[NE], ap2Y[NE];
+        State slY{}, srY{};
+        if constexpr (ST) { srY = RP::template state<1>(P, qrY); }
         if (w >= R0) {
             R wave[NW][NE], s[NW];
             Cache c;
-            RP::template rpn<1>(P, qrY_prev, qlY, aw[1], aw[2], wave, s, amY, apY, c);
+            if constexpr (ST) {
+                slY = RP::template state<1>(P, qlY);
+                RP::template rpn_s<1>(P, qrY_prev, qlY, srY_prev, slY, wave, s, amY, apY, c);
+            } else {
+                RP::template rpn<1>(P, qrY_prev, qlY, aw[1], aw[2], wave, s, amY, apY, c);
+            }
             if (w <= g.my && i >= -1 && i <= g.mx) {
 #pragma unroll
                 for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdy * rabs(s[mw]));
             }
-            RP::template rpn<1>(P, qlY, qrY, aw[2], aw[2], wave, s, am2Y, ap2Y, c);
+            if constexpr (ST) {
+                RP::template rpn_s<1>(P, qlY, qrY, slY, srY, wave, s, am2Y, ap2Y, c);
+            } else {
+                RP::template rpn<1>(P, qlY, qrY, aw[2], aw[2], wave, s, am2Y, ap2Y, c);
+            }
         }
 
         // ---- output row o = w-1: x part and combination --------------------------
@@ -233,12 +252,25 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
             {
                 R wave[NW][NE], s[NW];
                 Cache c;
-                RP::template rpn<0>(P, qrXl, qlX, alx, aw[1], wave, s, amX, apX, c);
-                if (i >= 0 && i <= g.mx && lane >= 3 && lane <= WAVE - 3) {
+                if constexpr (ST) {
+                    // the left neighbour's right-edge state and its quantities come
+                    // from the lane to the left
+                    const State srX = RP::template state<0>(P, qrX), slX = RP::template state<0>(P, qlX);
+                    const State srXl = RP::state_up(srX);
+                    RP::template rpn_s<0>(P, qrXl, qlX, srXl, slX, wave, s, amX, apX, c);
+                    if (i >= 0 && i <= g.mx && lane >= 3 && lane <= WAVE - 3) {
 #pragma unroll
-                    for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
+                        for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
+                    }
+                    RP::template rpn_s<0>(P, qlX, qrX, slX, srX, wave, s, am2X, ap2X, c);
+                } else {
+                    RP::template rpn<0>(P, qrXl, qlX, alx, aw[1], wave, s, amX, apX, c);
+                    if (i >= 0 && i <= g.mx && lane >= 3 && lane <= WAVE - 3) {
+#pragma unroll
+                        for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
+                    }
+                    RP::template rpn<0>(P, qlX, qrX, aw[1], aw[1], wave, s, am2X, ap2X, c);
                 }
-                RP::template rpn<0>(P, qlX, qrX, aw[1], aw[1], wave, s, am2X, ap2X, c);
             }
             if (out_row) {
                 const long long off = (long long)(o + g.ng) * g.ldq + (i + g.ng);
@@ -277,6 +309,40 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
             qrY_prev[m] = qrY[m];
             if (w >= R0) { apY_prev[m] = apY[m]; am2Y_prev[m] = am2Y[m]; ap2Y_prev[m] = ap2Y[m]; }
         }
+        srY_prev = srY;
+    }
+}
+
+// One wavefront's work: a band of p.H rows of one strip, or (p.fill > 0) an
+// equal share of the strips' rows taken in strip-major order, at most a few
+// bands (a share can run over the end of a strip into the next).
+template <class RP, typename R>
+__device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
+    // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
+    R cfl = R(0);
+    // this wavefront's rows as a range [u, u1) of u = strip * my + row
+    long long u, u1;
+    if (p.fill > 0) {
+        const int k = xcd_block() * p.wpb + (threadIdx.x >> 6);
+        if (k >= p.fill) return;
+        const long long total = (long long)p.nsx * p.g.my;
+        u = total * k / p.fill;
+        u1 = total * (k + 1) / p.fill;
+    } else {
+        const int vb = xcd_block();
+        const int sx = vb / p.nsyb;
+        const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+        if (sx >= p.nsx || sy >= p.nsy) return;
+        u = (long long)sx * p.g.my + (long long)sy * p.H;
+        u1 = u + min(p.H, p.g.my - sy * p.H);
+    }
+    while (u < u1) {
+        const int sx = (int)(u / p.g.my), R0 = (int)(u % p.g.my);
+        const int R1 = (int)min((long long)p.g.my, R0 + (u1 - u));
+        sharp_band<RP, R>(p, sx, R0, R1, cfl);
+        u += R1 - R0;
     }
     double c = wave_max((double)cfl);
     if (lane == 0) atomic_max_nonneg(p.cfl, c);
@@ -287,9 +353,10 @@ __global__ __launch_bounds__(256) void sharp_stage_kernel(const SharpParams<RP, 
     sharp_stage_body<RP, R>(p);
 }
 
-// The same kernel compiled for three wavefronts per SIMD (the FP64
-// shallow-water and acoustics instantiations fit without spilling); the
-// SharpClaw autotuner times both builds.
+// The same kernel compiled for three wavefronts per SIMD (168 VGPRs: the FP64
+// acoustics instantiations fit, shallow water and Euler spill); the SharpClaw
+// autotuner times both builds.  The default build's occupancy follows its
+// register use (FP64 shallow water: 222 VGPRs, two wavefronts per SIMD).
 template <class RP, typename R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void sharp_stage_kernel_w3(const SharpParams<RP, R> p) {
@@ -321,15 +388,20 @@ template <> Euler5<float>::Params make_sharp_params<Euler5<float>, float>(const 
 template <> ShallowRoe<double>::Params make_sharp_params<ShallowRoe<double>, double>(const ClawRiemann *rp) { return {rp->p[0]}; }
 template <> ShallowRoe<float>::Params make_sharp_params<ShallowRoe<float>, float>(const ClawRiemann *rp) { return {(float)rp->p[0]}; }
 
-static int sharp_band_height(int rows, int strips_x, int &w) {
+// Default launch shape.  Band height 0 is the fill decomposition: one
+// wavefront per slot of the build (CUs x the wavefronts per CU that the
+// kernel's register and LDS use allow), each taking an equal share of the
+// strips' rows, so that every SIMD holds the same number of wavefronts with
+// the same work.
+static int sharp_band_height(int rows, int strips_x, int &w, int &wpb) {
     const long long work = (long long)rows * strips_x;
     if (work >= 7LL * 2048) {
-        // enough 7-row bands for two or more wavefronts per SIMD: the build for
-        // three per SIMD with bands of about 3072 wavefronts (1024x1024: 7 rows,
-        // what the tuner picks there)
-        w = 3;
-        const int h = (int)std::max<long long>(7, (work + 3071) / 3072);
-        return std::min(h, std::max(rows, 1));
+        // shares of at least ~7 rows at two wavefronts per SIMD (1024x1024
+        // shallow water: 9 rows each, 0.87 Gcell/s against 0.79 for 7-row
+        // bands of the three-per-SIMD build in four-wavefront blocks)
+        w = 1;
+        wpb = 1;
+        return 0;
     }
     long long target = 2048;
     int h = (int)std::max<long long>(16, (work + target - 1) / target);
@@ -346,10 +418,10 @@ static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int
                       int &wpb) {
     w = 1;
     wpb = 4;
-    if (ctx && ctx->force_band > 0) {
+    if (ctx && ctx->force_band != 0) {   // a tuner candidate; -1 is the fill decomposition
         w = ctx->force_w;
         if (ctx->force_wpb > 0) wpb = ctx->force_wpb;
-        return std::min(ctx->force_band, grid->my);
+        return ctx->force_band > 0 ? std::min(ctx->force_band, grid->my) : 0;
     }
     if (ctx) {
         const TuneKey key = sharp_key(grid, rp, precision);
@@ -358,7 +430,33 @@ static int sharp_band(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, int
         auto it = ctx->band.find(key);
         if (it != ctx->band.end()) { w = it->second.second; return std::min(it->second.first, grid->my); }
     }
-    return sharp_band_height(grid->my, strips, w);
+    int dw = wpb;
+    const int h = sharp_band_height(grid->my, strips, w, dw);
+    if (!(ctx && ctx->wpb.count(sharp_key(grid, rp, precision)))) wpb = dw;
+    return h;
+}
+
+// Wavefront slots of a build on the current (or the context's) device: CUs x
+// the blocks per CU that the runtime's occupancy query gives for the kernel
+// (its registers and LDS) x wavefronts per block.  The query's answer is kept
+// per instantiation, build and block size.
+template <class RP, typename R>
+static int sharp_slots(Ctx *ctx, int w, int wpb) {
+    static std::atomic<int> per_cu[2][5];
+    const int b = w == 3 ? 1 : 0;
+    int nb = per_cu[b][wpb].load(std::memory_order_relaxed);
+    if (nb <= 0) {
+        const void *fn = w == 3 ? reinterpret_cast<const void *>(&sharp_stage_kernel_w3<RP, R>)
+                                : reinterpret_cast<const void *>(&sharp_stage_kernel<RP, R>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, WAVE * wpb, 0) != hipSuccess || nb <= 0)
+            nb = std::max(1, 4 * (w == 3 ? 3 : 1) / wpb);
+        per_cu[b][wpb].store(nb, std::memory_order_relaxed);
+    }
+    int dev = 0, cus = 0;
+    if (ctx) dev = ctx->device;
+    else (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    return cus * nb * wpb;
 }
 
 template <class RP, typename R>
@@ -376,14 +474,20 @@ static int launch_sharp(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, c
     p.nsx = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
     int w = 1;
     p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_SHARP_F64_KEY, p.nsx, w, wpb);
+    p.fill = 0;
+    if (p.H == 0) {             // fill decomposition: one wavefront per slot
+        p.fill = sharp_slots<RP, R>(ctx, w, wpb);
+        p.H = grid->my;
+    }
     p.nsy = (grid->my + p.H - 1) / p.H;
     p.wpb = wpb;
     p.nsyb = (p.nsy + wpb - 1) / wpb;
+    const int nblocks = p.fill > 0 ? (p.fill + wpb - 1) / wpb : p.nsx * p.nsyb;
     if (zero_cfl) hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, st, cfl);
     if (w == 3)
-        hipLaunchKernelGGL((sharp_stage_kernel_w3<RP, R>), dim3(p.nsx * p.nsyb), dim3(WAVE * wpb), 0, st, p);
+        hipLaunchKernelGGL((sharp_stage_kernel_w3<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
     else
-        hipLaunchKernelGGL((sharp_stage_kernel<RP, R>), dim3(p.nsx * p.nsyb), dim3(WAVE * wpb), 0, st, p);
+        hipLaunchKernelGGL((sharp_stage_kernel<RP, R>), dim3(nblocks), dim3(WAVE * wpb), 0, st, p);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 
@@ -514,7 +618,7 @@ int claw2d_sharpclaw_set_launch(void *ctx, const ClawGrid *grid, const ClawRiema
 
 int claw2d_sharpclaw_set_launch_ex(void *ctx, const ClawGrid *grid, const ClawRiemann *rp, int precision, int band,
                                    int waves_per_simd, int waves_per_block) {
-    if (!ctx || !grid || !rp || band < 1 || (waves_per_simd != 1 && waves_per_simd != 3)) return CLAW_EINVAL;
+    if (!ctx || !grid || !rp || band < 0 || (waves_per_simd != 1 && waves_per_simd != 3)) return CLAW_EINVAL;
     if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) return CLAW_EINVAL;
     Ctx *c = (Ctx *)ctx;
     const TuneKey key = sharp_key(grid, rp, precision);
@@ -545,18 +649,23 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
         if (e0) (void)hipEventDestroy(e0);
         return CLAW_EDEVICE;
     }
-    // row-band heights (ending with the balanced height of each build) x
-    // (1 or 3 wavefronts per SIMD) x (1 or 4 wavefronts per block)
+    // row-band heights (ending with the balanced heights for one and two
+    // wavefronts per SIMD, for the default build, and for three, for the
+    // three-per-SIMD build, then the fill decomposition, -1) x (default or
+    // three-per-SIMD build) x (1 or 4 wavefronts per block)
     const int strips = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
-    const int cand[11] = {4, 6, 8, 12, 16, 24, 32, 48, 64, balanced_band(c->device, grid->my, strips, 1),
-                          balanced_band(c->device, grid->my, strips, 3)};
+    constexpr int NC = 13;
+    const int cand[NC] = {4, 6, 8, 12, 16, 24, 32, 48, 64, balanced_band(c->device, grid->my, strips, 1),
+                          balanced_band(c->device, grid->my, strips, 2),
+                          balanced_band(c->device, grid->my, strips, 3), -1};
     int best = 0, best_w = 1, best_wpb = 4;
+    bool have = false;
     float best_ms = 0.f;
-    for (int k = 0; k < 44; ++k) {
-        const int h = cand[k % 11], w = (k / 11) % 2 == 0 ? 1 : 3, wpb = k < 22 ? 4 : 1;
+    for (int k = 0; k < 4 * NC; ++k) {
+        const int h = cand[k % NC], w = (k / NC) % 2 == 0 ? 1 : 3, wpb = k < 2 * NC ? 4 : 1;
         if (h > grid->my && h != cand[0]) continue;
-        if ((k % 11) == 9 && w != 1) continue;
-        if ((k % 11) == 10 && w != 3) continue;
+        if ((k % NC) >= 9 && (k % NC) <= 10 && w != 1) continue;
+        if ((k % NC) == 11 && w != 3) continue;
         c->force_band = h;
         c->force_w = w;
         c->force_wpb = wpb;
@@ -571,7 +680,7 @@ int claw2d_sharpclaw_autotune(void *ctx, const ClawGrid *grid, const ClawRiemann
         if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : CLAW_EDEVICE; break; }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (best == 0 || ms < best_ms) { best = h; best_w = w; best_wpb = wpb; best_ms = ms; }
+        if (!have || ms < best_ms) { best = h > 0 ? h : 0; best_w = w; best_wpb = wpb; best_ms = ms; have = true; }
     }
     c->force_band = 0;
     c->force_w = 0;

@@ -214,7 +214,7 @@ def test_config3_sharpclaw_large_grid_default_shape():
     qd = to_device_padded(pad)
     dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
     out = []
-    for shape in (None, (16, 1, 4)):
+    for shape in (None, (16, 1, 4), (7, 3, 4), (0, 3, 1)):
         ctx = _lib.Context(0)
         band, w, wpb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         if shape is not None:
@@ -223,7 +223,7 @@ def test_config3_sharpclaw_large_grid_default_shape():
         _lib.check(_lib.lib().claw2d_sharpclaw_get_launch(ctx.handle, grid, rp, _lib.F64, ctypes.byref(band),
                                                           ctypes.byref(w), ctypes.byref(wpb)),
                    'claw2d_sharpclaw_get_launch')
-        assert (band.value, w.value, wpb.value) == (shape if shape is not None else (7, 3, 4))
+        assert (band.value, w.value, wpb.value) == (shape if shape is not None else (0, 1, 1))
         dq = torch.zeros_like(qd)
         cfl = torch.zeros((), dtype=torch.float64, device='cuda')
         _lib.check(_lib.lib().claw2d_sharpclaw_dq(ctx.handle, grid, rp, bc, _lib.F64, qd.data_ptr(), dq.data_ptr(),
