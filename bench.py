@@ -396,8 +396,9 @@ def main():
         for key, fn in (("euler_4096x1024_run_to_t0.2", lambda: measure_euler_run(args, q0, dx, dy)),
                         ("batched_vs_plain_speedup", lambda: measure_batched(args, nsteps=200)),
                         ("adjoint_over_forward", lambda: measure_adjoint(args)),
+                        # launch='': the library's default shape for this grid, not the Euler line's
                         ("sharpclaw_shallow_1024", lambda: measure_sharpclaw(args, steps=500, warmup=5,
-                                                                             precision='f64'))):
+                                                                             precision='f64', launch=''))):
             t0 = time.perf_counter()
             try:
                 r_ = fn()
