@@ -171,6 +171,11 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     if constexpr (HAS_AUX) {
         load_aux<NA, R, O32>(p.aux, g, p.ba, i, R0 - 2, a_m1);
     }
+    // per-cell quantities shared by the four interfaces around a cell (the
+    // Euler solver's): stP for the row behind the lead row
+    constexpr bool CELL = RP::CELL_STATE;
+    typename RP::CellState stP{};
+    if constexpr (CELL) stP = RP::cell(P, qprev);
 
     for (int L = R0 - 1; L <= R1 + 1; ++L) {
         // ---- 1. load the lead row L ----------------------------------------
@@ -179,6 +184,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         if constexpr (HAS_AUX) {
             load_aux<NA, R, O32>(p.aux, g, p.ba, i, L, a_0);
         }
+        typename RP::CellState stC{};
+        if constexpr (CELL) stC = RP::cell(P, qc);
 
         // ---- 2. x-interface (i-1/2, L-1): solve, limit, transverse, x-part of cell (i, L-1)
         if (L >= R0) {
@@ -194,7 +201,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                     alo[a] = lane_up(a_m1[a]); alm2[a] = lane_up(a_m2[a]); al0[a] = lane_up(a_0[a]);
                 }
             }
-            RP::template rpn<0>(P, lo, qprev, alo, a_m1, wave, s, amdq, apdq, cx);
+            if constexpr (CELL) RP::template rpn_c<0>(P, lo, qprev, RP::cell_up(stP), stP, wave, s, amdq, apdq, cx);
+            else RP::template rpn<0>(P, lo, qprev, alo, a_m1, wave, s, amdq, apdq, cx);
             if (lane >= 1 && row >= -1 && row <= g.my && i >= 0 && i <= g.mx) {
 #pragma unroll
                 for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
@@ -266,7 +274,8 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         // ---- 3. y-interface (i, L-1/2): normal solve (unlimited) --------------
         R wN[NW][NE], sN[NW], amN[NE], apN[NE], dotN[NW];
         Cache cN;
-        RP::template rpn<1>(P, qprev, qc, a_m1, a_0, wN, sN, amN, apN, cN);
+        if constexpr (CELL) RP::template rpn_c<1>(P, qprev, qc, stP, stC, wN, sN, amN, apN, cN);
+        else RP::template rpn<1>(P, qprev, qc, a_m1, a_0, wN, sN, amN, apN, cN);
         if (L >= 0 && L <= g.my && i >= -1 && i <= g.mx) {
 #pragma unroll
             for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdy * rabs(sN[mw]));
@@ -385,6 +394,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             qprev[m] = qc[m];
         }
         cY = cN;
+        if constexpr (CELL) stP = stC;
         if constexpr (HAS_AUX) {
 #pragma unroll
             for (int a = 0; a < NA; ++a) {

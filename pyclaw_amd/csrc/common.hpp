@@ -188,9 +188,31 @@ template <typename R> __device__ __forceinline__ R lmax(R a, R b) { return rmax2
 template <typename R> __device__ __forceinline__ R lmin(R a, R b) { return rmin2<R>(a, b); }
 #endif
 
+// The limiter's ratio r = b/a (a = |W|^2 > 0).  The fused-multiply-add build
+// multiplies b by the hardware reciprocal of a refined by two Newton steps
+// (within about one ulp of b/a) instead of running the IEEE division
+// sequence; a below 2^-1000 is raised to 2^-1000 first, where the
+// reciprocal would overflow.  That changes r only for a wave under 2^-500 in
+// size, and only towards zero with its sign kept, so phi changes by at most
+// 2 and the limited wave by at most 2^-499.
+#ifdef CLAW_SHARED_RECIPROCALS
+template <typename R> __device__ __forceinline__ R lim_ratio(R b, R a) { return b / a; }
+template <> __device__ __forceinline__ double lim_ratio<double>(double b, double a) {
+    const double x = __builtin_fmax(a, 0x1p-1000);
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return b * r;
+}
+#else
+template <typename R> __device__ __forceinline__ R lim_ratio(R b, R a) { return b / a; }
+#endif
+
 template <typename R>
 __device__ __forceinline__ R philim(R a, R b, int meth) {
-    R r = b / a;
+    R r = lim_ratio<R>(b, a);
     switch (meth) {
     case 2: return lmax<R>(lmax<R>(R(0), lmin<R>(R(1), R(2) * r)), lmin<R>(R(2), r));
     case 3: { R ar = rabs(r); return (r + ar) / (R(1) + ar); }

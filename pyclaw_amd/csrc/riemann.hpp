@@ -134,6 +134,63 @@ template <> __device__ __forceinline__ void div2<double>(double n1, double n2, d
     }
 }
 
+// Three quotients with one denominator, as div2: q_k = n_k/d.  The exact
+// build shares the denominator's part of the division sequence among the
+// numerators that leave it scaled alike (checked per numerator; a numerator
+// that does not is divided on its own), so each q_k equals n_k/d bit for bit.
+template <typename R>
+__device__ __forceinline__ void div3(R n1, R n2, R n3, R d, R &q1, R &q2, R &q3) {
+    if constexpr (SHARED_RCP) {
+        const R r = rcp<R>(d);
+        q1 = n1 * r;
+        q2 = n2 * r;
+        q3 = n3 * r;
+    } else {
+        q1 = n1 / d;
+        q2 = n2 / d;
+        q3 = n3 / d;
+    }
+}
+template <> __device__ __forceinline__ void div3<double>(double n1, double n2, double n3, double d, double &q1,
+                                                         double &q2, double &q3) {
+    if constexpr (SHARED_RCP) {
+        const double r = rcp<double>(d);
+        q1 = n1 * r;
+        q2 = n2 * r;
+        q3 = n3 * r;
+    } else {
+        bool g, f1, f2, f3;
+        const double ds1 = __builtin_amdgcn_div_scale(n1, d, false, &g);
+        const double r0 = __builtin_amdgcn_rcp(ds1);
+        const double e0 = __builtin_fma(-ds1, r0, 1.0);
+        const double r1 = __builtin_fma(r0, e0, r0);
+        const double e1 = __builtin_fma(-ds1, r1, 1.0);
+        const double r2 = __builtin_fma(r1, e1, r1);
+        const double ns1 = __builtin_amdgcn_div_scale(n1, d, true, &f1);
+        const double m1 = ns1 * r2;
+        const double rem1 = __builtin_fma(-ds1, m1, ns1);
+        q1 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem1, r2, m1, f1), d, n1);
+        const double ds2 = __builtin_amdgcn_div_scale(n2, d, false, &g);
+        if (__double_as_longlong(ds2) == __double_as_longlong(ds1)) {
+            const double ns2 = __builtin_amdgcn_div_scale(n2, d, true, &f2);
+            const double m2 = ns2 * r2;
+            const double rem2 = __builtin_fma(-ds1, m2, ns2);
+            q2 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem2, r2, m2, f2), d, n2);
+        } else {
+            q2 = n2 / d;
+        }
+        const double ds3 = __builtin_amdgcn_div_scale(n3, d, false, &g);
+        if (__double_as_longlong(ds3) == __double_as_longlong(ds1)) {
+            const double ns3 = __builtin_amdgcn_div_scale(n3, d, true, &f3);
+            const double m3 = ns3 * r2;
+            const double rem3 = __builtin_fma(-ds1, m3, ns3);
+            q3 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem3, r2, m3, f3), d, n3);
+        } else {
+            q3 = n3 / d;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Constant-coefficient acoustics: q = (p, u, v), parameters rho, bulk, c, Z.
 // ---------------------------------------------------------------------------
@@ -144,6 +201,8 @@ template <typename R> struct Acoustics {
     struct Cache {};
     static constexpr bool SPEEDS_FROM_CACHE = false;
     static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
+    static constexpr bool CELL_STATE = false;  // no per-cell quantities shared between solves
+    struct CellState {};
     __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
@@ -186,6 +245,8 @@ template <typename R> struct VCAcoustics {
     struct Cache {};
     static constexpr bool SPEEDS_FROM_CACHE = false;
     static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
+    static constexpr bool CELL_STATE = false;  // no per-cell quantities shared between solves
+    struct CellState {};
     __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
     template <int DIR> static constexpr int MV = DIR == 0 ? 2 : 1;
@@ -251,34 +312,59 @@ template <typename R> struct Euler5 {
              : (mw == 1) ? (m == (DIR == 0 ? 2 : 1) || m == 3)
              : (m == 4);
     }
+    // The quantities of one cell that every Riemann problem it is part of
+    // computes, as the left or the right state, in x and in y alike (the
+    // reference's expressions, rpn2_euler_5wave.f:87-104 and the entropy
+    // fix's :205-286): rs = sqrt(rho); ms = (rho u, rho v)/sqrt(rho) and
+    // hs = (E + p)/sqrt(rho), the terms of the Roe averages; un = (rho u,
+    // rho v)/rho and c = sqrt(gamma p/rho), the entropy fix's characteristic
+    // speeds.  A kernel that meets a cell in four problems computes them once;
+    // each is the same operation on the same operands either way (the
+    // pressure's sum of squared momenta is commutative in the two directions).
+    struct CellState { R rs, ms[2], hs, un[2], c; };
+    static constexpr bool CELL_STATE = true;
+    __device__ static __forceinline__ CellState cell(const Params &P, const R (&q)[NE]) {
+        CellState S;
+        const R gamma1 = P.gamma1;
+        S.rs = rsqrt_exact<R>(q[0]);
+        R p;
+        if constexpr (SHARED_RCP) {
+            const R ir = rcp<R>(q[0]), irs = rcp<R>(S.rs);
+            p = gamma1 * (q[3] - R(0.5) * (q[1] * q[1] + q[2] * q[2]) * ir);
+            S.ms[0] = q[1] * irs; S.ms[1] = q[2] * irs;
+            S.hs = (q[3] + p) * irs;
+            S.un[0] = q[1] * ir; S.un[1] = q[2] * ir;
+            S.c = P.efix ? rsqrt_exact<R>(P.gamma * p * ir) : R(0);
+        } else {
+            R ke;
+            div3<R>(R(0.5) * (q[1] * q[1] + q[2] * q[2]), q[1], q[2], q[0], ke, S.un[0], S.un[1]);
+            p = gamma1 * (q[3] - ke);
+            div3<R>(q[1], q[2], q[3] + p, S.rs, S.ms[0], S.ms[1], S.hs);
+            S.c = P.efix ? rsqrt_exact<R>(P.gamma * p / q[0]) : R(0);
+        }
+        return S;
+    }
+    // a cell record moved one lane up (lane L receives lane L-1's)
+    __device__ static __forceinline__ CellState cell_up(const CellState &S) {
+        return CellState{lane_up(S.rs), {lane_up(S.ms[0]), lane_up(S.ms[1])}, lane_up(S.hs),
+                         {lane_up(S.un[0]), lane_up(S.un[1])}, lane_up(S.c)};
+    }
     template <int DIR>
     __device__ static __forceinline__ void rpn(const Params &P, const R (&lo)[NE], const R (&hi)[NE],
-                                               const R *, const R *, R (&wave)[NW][NE], R (&s)[NW],
+                                               const R *al, const R *ar, R (&wave)[NW][NE], R (&s)[NW],
                                                R (&amdq)[NE], R (&apdq)[NE], Cache &C) {
+        rpn_c<DIR>(P, lo, hi, cell(P, lo), cell(P, hi), wave, s, amdq, apdq, C);
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void rpn_c(const Params &P, const R (&lo)[NE], const R (&hi)[NE],
+                                                 const CellState &Sl, const CellState &Sr, R (&wave)[NW][NE],
+                                                 R (&s)[NW], R (&amdq)[NE], R (&apdq)[NE], Cache &C) {
         constexpr int mu = MU<DIR>, mv = MV<DIR>;
         const R gamma1 = P.gamma1;
-        R rhsqrtl = rsqrt_exact<R>(lo[0]);
-        R rhsqrtr = rsqrt_exact<R>(hi[0]);
-        R pl, pr, u, v, enth;
-        R il0 = R(0), ih0 = R(0);   // 1/rho of each side (shared-reciprocal build)
-        if constexpr (SHARED_RCP) {
-            il0 = rcp<R>(lo[0]);
-            ih0 = rcp<R>(hi[0]);
-            const R irl = rcp<R>(rhsqrtl), irr = rcp<R>(rhsqrtr);
-            pl = gamma1 * (lo[3] - R(0.5) * (lo[1] * lo[1] + lo[2] * lo[2]) * il0);
-            pr = gamma1 * (hi[3] - R(0.5) * (hi[1] * hi[1] + hi[2] * hi[2]) * ih0);
-            const R irs = rcp<R>(rhsqrtl + rhsqrtr);
-            u = (lo[mu] * irl + hi[mu] * irr) * irs;
-            v = (lo[mv] * irl + hi[mv] * irr) * irs;
-            enth = ((lo[3] + pl) * irl + (hi[3] + pr) * irr) * irs;
-        } else {
-            pl = gamma1 * (lo[3] - R(0.5) * (lo[1] * lo[1] + lo[2] * lo[2]) / lo[0]);
-            pr = gamma1 * (hi[3] - R(0.5) * (hi[1] * hi[1] + hi[2] * hi[2]) / hi[0]);
-            R rhsq2 = rhsqrtl + rhsqrtr;
-            u = (lo[mu] / rhsqrtl + hi[mu] / rhsqrtr) / rhsq2;
-            v = (lo[mv] / rhsqrtl + hi[mv] / rhsqrtr) / rhsq2;
-            enth = ((lo[3] + pl) / rhsqrtl + (hi[3] + pr) / rhsqrtr) / rhsq2;
-        }
+        // u = (lo[mu]/sqrt(rho_l) + hi[mu]/sqrt(rho_r)) / (sqrt(rho_l) + sqrt(rho_r)), v and H alike
+        R u, v, enth;
+        div3<R>(Sl.ms[mu - 1] + Sr.ms[mu - 1], Sl.ms[mv - 1] + Sr.ms[mv - 1], Sl.hs + Sr.hs, Sl.rs + Sr.rs,
+                u, v, enth);
         R u2v2 = u * u + v * v;
         R a2 = gamma1 * (enth - R(0.5) * u2v2);
         R a = rsqrt_exact<R>(a2);
@@ -331,16 +417,8 @@ template <typename R> struct Euler5 {
         // rpn2_euler_5wave.f:205-286: amdq with the transonic-rarefaction fix
         do {
             R sfract;
-            R s0;
-            if constexpr (SHARED_RCP) {
-                // the fix's pim1 is the pressure pl above
-                s0 = lo[mu] * il0 - rsqrt_exact<R>(gamma * pl * il0);
-            } else {
-                R rhoim1 = lo[0];
-                R pim1 = gamma1 * (lo[3] - R(0.5) * (lo[mu] * lo[mu] + lo[mv] * lo[mv]) / rhoim1);
-                R cim1 = rsqrt_exact<R>(gamma * pim1 / rhoim1);
-                s0 = lo[mu] / rhoim1 - cim1;
-            }
+            // the left state's u - c (its pressure is the pl of the Roe averages)
+            const R s0 = Sl.un[mu - 1] - Sl.c;
             if (s0 >= R(0) && s[0] > R(0)) {
 #pragma unroll
                 for (int m = 0; m < NE; ++m) amdq[m] = R(0);
@@ -379,16 +457,12 @@ template <typename R> struct Euler5 {
             R rhou2 = hi[mu] - wave[3][mu];
             R rhov2 = hi[mv] - wave[3][mv];
             R en2 = hi[3] - wave[3][3];
+            s3 = Sr.un[mu - 1] + Sr.c;   // the right state's u + c
             if constexpr (SHARED_RCP) {
-                s3 = hi[mu] * ih0 + rsqrt_exact<R>(gamma * pr * ih0);
                 const R i2 = rcp<R>(rho2);
                 R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) * i2);
                 s2 = rhou2 * i2 + rsqrt_exact<R>(gamma * p2 * i2);
             } else {
-                R rhoi = hi[0];
-                R pi = gamma1 * (hi[3] - R(0.5) * (hi[mu] * hi[mu] + hi[mv] * hi[mv]) / rhoi);
-                R ci = rsqrt_exact<R>(gamma * pi / rhoi);
-                s3 = hi[mu] / rhoi + ci;
                 R p2 = gamma1 * (en2 - R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
                 R c2 = rsqrt_exact<R>(gamma * p2 / rho2);
                 s2 = rhou2 / rho2 + c2;
@@ -458,6 +532,8 @@ template <typename R> struct Euler5 {
 template <typename R> struct ShallowRoe {
     static constexpr int NE = 3, NW = 3, NA = 0;
     static constexpr bool HAS_EFIX = false;   // the entropy fix is always on
+    static constexpr bool CELL_STATE = false;  // the per-state quantities below depend on the direction
+    struct CellState {};
     struct Params { R grav; };
     struct Cache { R u, v, a; };
     static constexpr bool SPEEDS_FROM_CACHE = false;
