@@ -121,6 +121,11 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 
     const int lane = threadIdx.x & (WAVE - 1);
     const GridDev g = p.g;
+    // strip and band are the same in every lane (the whole wavefront is
+    // active here): say so, so that row indices, row addresses and the
+    // ghost-row rules are scalar work with scalar branches
+    sx = __builtin_amdgcn_readfirstlane(sx);
+    sy = __builtin_amdgcn_readfirstlane(sy);
     const int c0 = sx * XU_USEFUL;
     const int i = c0 - 2 + lane;
     const int R0 = sy * p.H;
@@ -167,9 +172,12 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 #pragma unroll
     for (int a = 0; a < NA; ++a) { a_m2[a] = a_m1[a] = a_0[a] = R(1); }
 
-    load_q<NE, R, O32>(p.qold, g, p.bq, i, R0 - 2, qprev);
+    // ghost-cell sources: the lane's column part once, the row part per row
+    const ColSrc colq = col_source(i, g, p.bq);
+    const ColSrc cola = col_source(i, g, p.ba);
+    load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(R0 - 2, g, p.bq), qprev);
     if constexpr (HAS_AUX) {
-        load_aux<NA, R, O32>(p.aux, g, p.ba, i, R0 - 2, a_m1);
+        load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(R0 - 2, g, p.ba), a_m1);
     }
     // per-cell quantities shared by the four interfaces around a cell (the
     // Euler solver's): stP for the row behind the lead row
@@ -180,9 +188,9 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     for (int L = R0 - 1; L <= R1 + 1; ++L) {
         // ---- 1. load the lead row L ----------------------------------------
         R qc[NE];
-        load_q<NE, R, O32>(p.qold, g, p.bq, i, L, qc);
+        load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L, g, p.bq), qc);
         if constexpr (HAS_AUX) {
-            load_aux<NA, R, O32>(p.aux, g, p.ba, i, L, a_0);
+            load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(L, g, p.ba), a_0);
         }
         typename RP::CellState stC{};
         if constexpr (CELL) stC = RP::cell(P, qc);

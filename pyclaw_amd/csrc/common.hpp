@@ -121,6 +121,44 @@ __host__ __device__ __forceinline__ SrcCell bc_source(int i, int j, const GridDe
     return r;
 }
 
+// The same source cell as bc_source, split into a column part (per lane; a
+// lane that keeps its column computes it once) and a row part (the same for
+// every lane of a row: scalar work).  A y ghost row with stored values takes
+// the clamped column (x rule skipped); any other row takes the column's x
+// rule.
+struct ColSrc { int ii, ii_st; bool fx; };
+struct RowSrc { int jj; bool fy, st; };
+
+__host__ __device__ __forceinline__ ColSrc col_source(int i, const GridDev &g, const BCDev &bc) {
+    ColSrc c;
+    c.fx = false;
+    c.ii = i;
+    c.ii_st = iclamp(i, -g.ng, g.mx + g.ng - 1);
+    if (i < 0 || i >= g.mx) {
+        const int code = i < 0 ? bc.lo[0] : bc.hi[0];
+        if (code == CLAW_BC_STORED) c.ii = c.ii_st;
+        else c.ii = bc_map1(i, g.mx, code, c.fx);
+    }
+    return c;
+}
+
+__host__ __device__ __forceinline__ RowSrc row_source(int j, const GridDev &g, const BCDev &bc) {
+    RowSrc r;
+    r.fy = false;
+    r.st = false;
+    r.jj = j;
+    if (j < 0 || j >= g.my) {
+        const int code = j < 0 ? bc.lo[1] : bc.hi[1];
+        if (code == CLAW_BC_STORED) {
+            r.st = true;
+            r.jj = iclamp(j, -g.ng, g.my + g.ng - 1);
+        } else {
+            r.jj = bc_map1(j, g.my, code, r.fy);
+        }
+    }
+    return r;
+}
+
 // Element e of the plane that starts at base.  O32 (every plane smaller than
 // 4 GiB): the byte offset is formed in 32 bits, so the access is a uniform
 // 64-bit base (the plane) plus one 32-bit per-lane offset, with no per-lane
@@ -151,6 +189,28 @@ __device__ __forceinline__ void load_q(const R *__restrict__ q, const GridDev &g
         if ((m == 1 && s.fx) || (m == 2 && s.fy)) v = -v;
         out[m] = v;
     }
+}
+
+// load_q / load_aux for a cell given as column and row parts (see col_source)
+template <int NE, typename R, bool O32 = false>
+__device__ __forceinline__ void load_q_cr(const R *__restrict__ q, const GridDev &g, const ColSrc &c,
+                                          const RowSrc &r, R (&out)[NE]) {
+    const long long off = (long long)(r.jj + g.ng) * g.ldq + ((r.st ? c.ii_st : c.ii) + g.ng);
+    const bool fx = !r.st && c.fx;
+#pragma unroll
+    for (int m = 0; m < NE; ++m) {
+        R v = plane_at<O32>(q + m * g.plane, off);
+        if ((m == 1 && fx) || (m == 2 && r.fy)) v = -v;
+        out[m] = v;
+    }
+}
+
+template <int NA, typename R, bool O32 = false>
+__device__ __forceinline__ void load_aux_cr(const R *__restrict__ a, const GridDev &g, const ColSrc &c,
+                                            const RowSrc &r, R (&out)[NA]) {
+    const long long off = (long long)(r.jj + g.ng) * g.ldq + ((r.st ? c.ii_st : c.ii) + g.ng);
+#pragma unroll
+    for (int m = 0; m < NA; ++m) out[m] = plane_at<O32>(a + m * g.plane, off);
 }
 
 template <int NA, typename R, bool O32 = false>
