@@ -57,6 +57,14 @@ namespace CLAW_CLASSIC_VARIANT {
 constexpr int XS_USEFUL = WAVE - 3;   // produced columns per wavefront, split x pass
 constexpr int XU_USEFUL = WAVE - 4;   // produced columns per wavefront, unsplit step
 constexpr int CNG = 2;               // ghost width of the classic method (classic/solver.py:82)
+// The unsplit march runs two rows per loop trip in the fused build (its
+// row-to-row state then needs no register copies); the exact build's longer
+// division sequences would spill with two rows in flight, so it runs one.
+#ifdef CLAW_SHARED_RECIPROCALS
+constexpr bool ROW_PAIRS = true;
+#else
+constexpr bool ROW_PAIRS = false;
+#endif
 
 template <class RP, typename R>
 struct ClassicParams {
@@ -185,7 +193,10 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     typename RP::CellState stP{};
     if constexpr (CELL) stP = RP::cell(P, qprev);
 
-    for (int L = R0 - 1; L <= R1 + 1; ++L) {
+    // One row of the march (lead row L), as a lambda so that the loop below
+    // can run it twice per trip: the state carried from row to row then
+    // alternates between two register sets instead of being copied.
+    auto row_step = [&](const int L) __attribute__((always_inline)) {
         // ---- 1. load the lead row L ----------------------------------------
         R qc[NE];
         load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L, g, p.bq), qc);
@@ -409,6 +420,14 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
                 a_m2[a] = a_m1[a]; a_m1[a] = a_0[a];
             }
         }
+    };
+    if constexpr (ROW_PAIRS) {
+        for (int L = R0 - 1; L <= R1 + 1; L += 2) {
+            row_step(L);
+            if (L + 1 <= R1 + 1) row_step(L + 1);
+        }
+    } else {
+        for (int L = R0 - 1; L <= R1 + 1; ++L) row_step(L);
     }
 
     // max with 0 leaves the slot as it is: the Courant slots hold
