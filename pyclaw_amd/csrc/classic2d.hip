@@ -118,10 +118,13 @@ struct CarrySlots {
 // SPEC: 0, p.order, p.trans and p.mthlim select at run time; 1, the method
 // is known to be order 2 with transverse correction waves (transverse = 2),
 // so its branches fold at compile time; 2, in addition every wave family is
-// limited with the MC limiter (id 4) and a solver with an entropy fix has it on.
+// limited with the MC limiter (id 4) and a solver with an entropy fix has it
+// on; 3, as 2 with transverse increment waves (transverse = 1).
 template <class RP, typename R, bool O32 = false, int SPEC = 0>
 __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy, R *__restrict__ cs) {
     constexpr bool M22 = SPEC >= 1;
+    constexpr int SPEC_TRANS = SPEC == 3 ? 1 : 2;
+    constexpr bool ALL_MC = SPEC >= 2;
     constexpr int NE = RP::NE, NW = RP::NW;
     constexpr bool HAS_AUX = RP::NA > 0;
     constexpr int NA = HAS_AUX ? RP::NA : 1;
@@ -148,9 +151,9 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     const R dtdx = uniform(R(dtd / p.dx)), dtdy = uniform(R(dtd / p.dy));
     const R hx = uniform(R(0.5) * dtdx), hy = uniform(R(0.5) * dtdy);
     const R avx = uniform(R(0.5) * (dtdx + dtdx)), avy = uniform(R(0.5) * (dtdy + dtdy));
-    const int order = M22 ? 2 : p.order, trans = M22 ? 2 : p.trans;
+    const int order = M22 ? 2 : p.order, trans = M22 ? SPEC_TRANS : p.trans;
     typename RP::Params P = p.P;
-    if constexpr (SPEC == 2 && RP::HAS_EFIX) P.efix = 1;
+    if constexpr (ALL_MC && RP::HAS_EFIX) P.efix = 1;
 
     // Courant number: the reference's max(cfl, dtdx * s, -dtdx * s) is
     // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
@@ -187,6 +190,17 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     if constexpr (HAS_AUX) {
         load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(R0 - 2, g, p.ba), a_m1);
     }
+    // Solvers with registers to spare load the lead row one row ahead (qnext,
+    // anext), so its load overlaps the previous row's work instead of
+    // stalling the wavefront; the last row's prefetch reads the row after
+    // the band (a valid source row, not used).  The Euler solver has no
+    // registers left for it.
+    constexpr bool PREFETCH = !RP::CELL_STATE;
+    R qnext[NE], anext[NA];
+    if constexpr (PREFETCH) {
+        load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(R0 - 1, g, p.bq), qnext);
+        if constexpr (HAS_AUX) load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(R0 - 1, g, p.ba), anext);
+    }
     // per-cell quantities shared by the four interfaces around a cell (the
     // Euler solver's): stP for the row behind the lead row
     constexpr bool CELL = RP::CELL_STATE;
@@ -196,18 +210,37 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     // One row of the march (lead row L), as a lambda so that the loop below
     // can run it twice per trip: the state carried from row to row then
     // alternates between two register sets instead of being copied.
-    auto row_step = [&](const int L) __attribute__((always_inline)) {
+    // full: the x-interface and the finish of the y-interface behind run
+    // too (every row after the first); the first row (R0 - 1) only loads
+    // and solves the y-interface below the band's first row.
+    // The Euler solver keeps a test of the row's position before each phase:
+    // the branches keep the phases apart in the schedule, and its phases
+    // scheduled together would not fit the register file.
+    constexpr bool ROW_GUARDS = RP::CELL_STATE;
+    auto row_step = [&](auto full, const int L) __attribute__((always_inline)) {
+        constexpr bool FULL = decltype(full)::value;
         // ---- 1. load the lead row L ----------------------------------------
         R qc[NE];
-        load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L, g, p.bq), qc);
-        if constexpr (HAS_AUX) {
-            load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(L, g, p.ba), a_0);
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int m = 0; m < NE; ++m) qc[m] = qnext[m];
+            load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L + 1, g, p.bq), qnext);
+            if constexpr (HAS_AUX) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a) a_0[a] = anext[a];
+                load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(L + 1, g, p.ba), anext);
+            }
+        } else {
+            load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L, g, p.bq), qc);
+            if constexpr (HAS_AUX) {
+                load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(L, g, p.ba), a_0);
+            }
         }
         typename RP::CellState stC{};
         if constexpr (CELL) stC = RP::cell(P, qc);
 
         // ---- 2. x-interface (i-1/2, L-1): solve, limit, transverse, x-part of cell (i, L-1)
-        if (L >= R0) {
+        if (FULL && (!ROW_GUARDS || L >= R0)) {
             const int row = L - 1;
             R lo[NE], wave[NW][NE], s[NW], amdq[NE], apdq[NE], fadd[NE], cq[NE];
             R alo[NA], alm2[NA], al0[NA];
@@ -235,15 +268,24 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 #pragma unroll
                     for (int m = 0; m < NE; ++m)
                         wn[m] = RP::template WNZ<0>(m, mw) ? lane_dn(wave[mw][m]) : R(0);
-                    const int mth = SPEC == 2 ? 4 : p.mthlim[mw];
+                    const int mth = ALL_MC ? 4 : p.mthlim[mw];
                     if (mth == 0) continue;
                     R wnorm2 = R(0), dotr = R(0);
 #pragma unroll
                     for (int m = 0; m < NE; ++m)
                         if (RP::template WNZ<0>(m, mw)) { wnorm2 = wnorm2 + wave[mw][m] * wave[mw][m]; dotr = dotr + wave[mw][m] * wn[m]; }
                     R dotl = lane_up(dotr);
-                    if (wnorm2 != R(0)) {
+                    if constexpr (RP::SKIP_ZERO_WAVES) {
+                        if (wnorm2 != R(0)) {
+                            R phi = philim<R>(wnorm2, s[mw] > R(0) ? dotl : dotr, mth);
+#pragma unroll
+                            for (int m = 0; m < NE; ++m)
+                                if (RP::template WNZ<0>(m, mw)) wave[mw][m] = phi * wave[mw][m];
+                        }
+                    } else {
+                        // a zero wave is left as it is (phi = 1 multiplies exactly)
                         R phi = philim<R>(wnorm2, s[mw] > R(0) ? dotl : dotr, mth);
+                        phi = wnorm2 != R(0) ? phi : R(1);
 #pragma unroll
                         for (int m = 0; m < NE; ++m)
                             if (RP::template WNZ<0>(m, mw)) wave[mw][m] = phi * wave[mw][m];
@@ -315,7 +357,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         }
 
         // ---- 4. finish y-interface (i, L-3/2) and cell row L-2 ---------------
-        if (L >= R0 + 1) {
+        if (FULL && (!ROW_GUARDS || L >= R0 + 1)) {
             R sYv[NW];
             if constexpr (RP::SPEEDS_FROM_CACHE) {
                 RP::speeds(cY, sYv);
@@ -329,14 +371,22 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             if (order == 2) {
 #pragma unroll
                 for (int mw = 0; mw < NW; ++mw) {
-                    const int mth = SPEC == 2 ? 4 : p.mthlim[mw];
+                    const int mth = ALL_MC ? 4 : p.mthlim[mw];
                     if (mth == 0) continue;
                     R wnorm2 = R(0);
 #pragma unroll
                     for (int m = 0; m < NE; ++m)
                         if (RP::template WNZ<1>(m, mw)) wnorm2 = wnorm2 + wY[mw][m] * wY[mw][m];
-                    if (wnorm2 != R(0)) {
+                    if constexpr (RP::SKIP_ZERO_WAVES) {
+                        if (wnorm2 != R(0)) {
+                            R phi = philim<R>(wnorm2, sYv[mw] > R(0) ? slot[(CS::DOTL + mw) * WAVE] : dotN[mw], mth);
+#pragma unroll
+                            for (int m = 0; m < NE; ++m)
+                                if (RP::template WNZ<1>(m, mw)) wY[mw][m] = phi * wY[mw][m];
+                        }
+                    } else {
                         R phi = philim<R>(wnorm2, sYv[mw] > R(0) ? slot[(CS::DOTL + mw) * WAVE] : dotN[mw], mth);
+                        phi = wnorm2 != R(0) ? phi : R(1);
 #pragma unroll
                         for (int m = 0; m < NE; ++m)
                             if (RP::template WNZ<1>(m, mw)) wY[mw][m] = phi * wY[mw][m];
@@ -421,13 +471,20 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             }
         }
     };
+    // The rows after the first run the whole row step with no test of the
+    // row's position except at the store: the one finish whose results are
+    // never stored (the y-interface below the band's first row, at L = R0)
+    // costs one extra finish per band and keeps the row step one block.
+    // (With row guards the first row needs no copy of its own.)
+    if constexpr (!ROW_GUARDS) row_step(std::false_type{}, R0 - 1);
+    constexpr int OFF = ROW_GUARDS ? 1 : 0;
     if constexpr (ROW_PAIRS) {
-        for (int L = R0 - 1; L <= R1 + 1; L += 2) {
-            row_step(L);
-            if (L + 1 <= R1 + 1) row_step(L + 1);
+        for (int L = R0 - OFF; L <= R1 + 1; L += 2) {
+            row_step(std::true_type{}, L);
+            if (L + 1 <= R1 + 1) row_step(std::true_type{}, L + 1);
         }
     } else {
-        for (int L = R0 - 1; L <= R1 + 1; ++L) row_step(L);
+        for (int L = R0 - OFF; L <= R1 + 1; ++L) row_step(std::true_type{}, L);
     }
 
     // max with 0 leaves the slot as it is: the Courant slots hold
@@ -764,7 +821,7 @@ struct BatchStepParams {
 };
 
 // O32: every instance's component planes are under 4 GiB (32-bit in-plane offsets).
-template <class RP, typename R, bool O32>
+template <class RP, typename R, bool O32, int SPEC>
 __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<RP, R> b) {
     const int lane = threadIdx.x & (WAVE - 1);
     // blockDim.x / WAVE wavefronts, each with its carry slots in dynamic LDS
@@ -792,7 +849,7 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<R
 #pragma unroll
         for (int q = 0; q < 8; ++q) p.mthlim[q] = b.mthlim[q];
         p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0;
-        unsplit_band<RP, R, O32>(p, local / I.nsy, local % I.nsy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+        unsplit_band<RP, R, O32, SPEC>(p, local / I.nsy, local % I.nsy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
     }
 }
 
@@ -942,6 +999,18 @@ template <> ShallowRoe<float>::Params make_params<ShallowRoe<float>, float>(cons
 
 static int check(hipError_t e) { return e == hipSuccess ? CLAW_OK : CLAW_EDEVICE; }
 
+// The compile-time method specialisation of the unsplit kernels (SPEC of
+// unsplit_band) that a method can use.
+template <class RP>
+static int unsplit_spec(const ClawMethod *meth, const ClawRiemann *rp) {
+    if (meth->order != 2 || (meth->transverse != 1 && meth->transverse != 2)) return 0;
+    bool all_mc = !(RP::HAS_EFIX && !rp->efix);
+    for (int k = 0; k < RP::NW; ++k)
+        if (meth->mthlim[k] != 4) all_mc = false;
+    if (meth->transverse == 2) return all_mc ? 2 : 1;
+    return all_mc ? 3 : 0;
+}
+
 // fixed_scratch: when non-null, the split step's intermediate state goes there
 // (fixed_cap bytes; CLAW_ETOOLARGE beyond) and nothing is allocated; otherwise
 // the context's scratch, allocated or grown on demand.
@@ -1002,15 +1071,11 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         const bool o32 = !ctx->wide_offsets && (unsigned long long)grid->plane * sizeof(R) <= 0xffffffffull;
         // compile-time specialisations: order 2 with transverse correction
         // waves, and that with the MC limiter on every wave family
-        int spec = 0;
-        if (meth->order == 2 && meth->transverse == 2) {
-            spec = (RP::HAS_EFIX && !rp->efix) ? 1 : 2;
-            for (int k = 0; k < RP::NW; ++k)
-                if (meth->mthlim[k] != 4) spec = 1;
-        }
+        const int spec = unsplit_spec<RP>(meth, rp);
         const dim3 grd(nblocks), blk(WAVE * wpb);
 #define CLAW_UNSPLIT_LAUNCH(KERNEL, O)                                                            \
-        if (spec == 2) hipLaunchKernelGGL((KERNEL<RP, R, O, 2>), grd, blk, lds, st, p);        \
+        if (spec == 3) hipLaunchKernelGGL((KERNEL<RP, R, O, 3>), grd, blk, lds, st, p);        \
+        else if (spec == 2) hipLaunchKernelGGL((KERNEL<RP, R, O, 2>), grd, blk, lds, st, p);   \
         else if (spec == 1) hipLaunchKernelGGL((KERNEL<RP, R, O, 1>), grd, blk, lds, st, p);   \
         else hipLaunchKernelGGL((KERNEL<RP, R, O, 0>), grd, blk, lds, st, p);
         if (w2 && o32) { CLAW_UNSPLIT_LAUNCH(classic_unsplit_kernel_w2, true) }
@@ -1132,8 +1197,12 @@ static int batch_step_launch(BatchPlan *pl, void *qpool, const void *auxpool, co
         if ((unsigned long long)I.plane * sizeof(R) > 0xffffffffull) o32 = false;
     if (pl->blocks == 0) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, o32 ? batch_step_kernel<RP, R, true>
-                                                                       : batch_step_kernel<RP, R, false>,
+        const bool q3 = unsplit_spec<RP>(&pl->meth, &pl->rp) == 3;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                         o32 ? (q3 ? batch_step_kernel<RP, R, true, 3>
+                                                                   : batch_step_kernel<RP, R, true, 0>)
+                                                             : (q3 ? batch_step_kernel<RP, R, false, 3>
+                                                                   : batch_step_kernel<RP, R, false, 0>),
                                                          WAVE * pl->wpb, lds) !=
             hipSuccess)
             per_cu = 1;
@@ -1144,10 +1213,17 @@ static int batch_step_launch(BatchPlan *pl, void *qpool, const void *auxpool, co
         // no more waves than work items
         pl->blocks = std::min(pl->blocks, (pl->nitems + pl->wpb - 1) / pl->wpb);
     }
-    if (o32)
-        hipLaunchKernelGGL((batch_step_kernel<RP, R, true>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
+    // the transverse-increment MC method has its own compile-time build; the
+    // others take the runtime-general one
+    const bool s3 = unsplit_spec<RP>(&pl->meth, &pl->rp) == 3;
+    if (o32 && s3)
+        hipLaunchKernelGGL((batch_step_kernel<RP, R, true, 3>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
+    else if (o32)
+        hipLaunchKernelGGL((batch_step_kernel<RP, R, true, 0>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
+    else if (s3)
+        hipLaunchKernelGGL((batch_step_kernel<RP, R, false, 3>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
     else
-        hipLaunchKernelGGL((batch_step_kernel<RP, R, false>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
+        hipLaunchKernelGGL((batch_step_kernel<RP, R, false, 0>), dim3(pl->blocks), dim3(WAVE * pl->wpb), lds, st, b);
     return hipGetLastError() == hipSuccess ? CLAW_OK : CLAW_EDEVICE;
 }
 

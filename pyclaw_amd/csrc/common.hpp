@@ -80,13 +80,13 @@ __host__ __device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v
 
 // Map a 1-D ghost index to its source (solver.py:386-434 for one side).
 // Returns the source index in [0, n-1]; sets flip for wall.
+// (Written as selects: with uniform arguments it is straight-line scalar code.)
 __host__ __device__ __forceinline__ int bc_map1(int k, int n, int code, bool &flip) {
-    int s;
-    if (k < 0) {
-        s = code == CLAW_BC_PERIODIC ? k + n : (code == CLAW_BC_WALL ? -1 - k : 0);
-    } else {
-        s = code == CLAW_BC_PERIODIC ? k - n : (code == CLAW_BC_WALL ? 2 * n - 1 - k : n - 1);
-    }
+    const bool lo = k < 0;
+    const int per = lo ? k + n : k - n;
+    const int wal = lo ? -1 - k : 2 * n - 1 - k;
+    const int ext = lo ? 0 : n - 1;
+    const int s = code == CLAW_BC_PERIODIC ? per : (code == CLAW_BC_WALL ? wal : ext);
     flip = (code == CLAW_BC_WALL);
     return iclamp(s, 0, n - 1);
 }
@@ -125,37 +125,37 @@ __host__ __device__ __forceinline__ SrcCell bc_source(int i, int j, const GridDe
 // lane that keeps its column computes it once) and a row part (the same for
 // every lane of a row: scalar work).  A y ghost row with stored values takes
 // the clamped column (x rule skipped); any other row takes the column's x
-// rule.
-struct ColSrc { int ii, ii_st; bool fx; };
+// rule.  The clamped column is kept as an offset from the x rule's column
+// (dst), so choosing between the two is an add of a selected value, which
+// compiles to registers (a choice between two fields of one record can be
+// compiled as a load from a chosen address, which puts the record in
+// scratch memory).
+struct ColSrc { int ii, dst; bool fx; };
 struct RowSrc { int jj; bool fy, st; };
 
 __host__ __device__ __forceinline__ ColSrc col_source(int i, const GridDev &g, const BCDev &bc) {
     ColSrc c;
     c.fx = false;
     c.ii = i;
-    c.ii_st = iclamp(i, -g.ng, g.mx + g.ng - 1);
+    const int ii_st = iclamp(i, -g.ng, g.mx + g.ng - 1);
     if (i < 0 || i >= g.mx) {
         const int code = i < 0 ? bc.lo[0] : bc.hi[0];
-        if (code == CLAW_BC_STORED) c.ii = c.ii_st;
+        if (code == CLAW_BC_STORED) c.ii = ii_st;
         else c.ii = bc_map1(i, g.mx, code, c.fx);
     }
+    c.dst = ii_st - c.ii;
     return c;
 }
 
 __host__ __device__ __forceinline__ RowSrc row_source(int j, const GridDev &g, const BCDev &bc) {
     RowSrc r;
-    r.fy = false;
-    r.st = false;
-    r.jj = j;
-    if (j < 0 || j >= g.my) {
-        const int code = j < 0 ? bc.lo[1] : bc.hi[1];
-        if (code == CLAW_BC_STORED) {
-            r.st = true;
-            r.jj = iclamp(j, -g.ng, g.my + g.ng - 1);
-        } else {
-            r.jj = bc_map1(j, g.my, code, r.fy);
-        }
-    }
+    const bool out = j < 0 || j >= g.my;
+    const int code = j < 0 ? bc.lo[1] : bc.hi[1];
+    bool wall;
+    const int jm = bc_map1(j, g.my, code, wall);
+    r.st = out && code == CLAW_BC_STORED;
+    r.jj = !out ? j : (r.st ? iclamp(j, -g.ng, g.my + g.ng - 1) : jm);
+    r.fy = out && !r.st && wall;
     return r;
 }
 
@@ -195,7 +195,7 @@ __device__ __forceinline__ void load_q(const R *__restrict__ q, const GridDev &g
 template <int NE, typename R, bool O32 = false>
 __device__ __forceinline__ void load_q_cr(const R *__restrict__ q, const GridDev &g, const ColSrc &c,
                                           const RowSrc &r, R (&out)[NE]) {
-    const long long off = (long long)(r.jj + g.ng) * g.ldq + ((r.st ? c.ii_st : c.ii) + g.ng);
+    const long long off = (long long)(r.jj + g.ng) * g.ldq + (c.ii + (r.st ? c.dst : 0) + g.ng);
     const bool fx = !r.st && c.fx;
 #pragma unroll
     for (int m = 0; m < NE; ++m) {
@@ -208,7 +208,7 @@ __device__ __forceinline__ void load_q_cr(const R *__restrict__ q, const GridDev
 template <int NA, typename R, bool O32 = false>
 __device__ __forceinline__ void load_aux_cr(const R *__restrict__ a, const GridDev &g, const ColSrc &c,
                                             const RowSrc &r, R (&out)[NA]) {
-    const long long off = (long long)(r.jj + g.ng) * g.ldq + ((r.st ? c.ii_st : c.ii) + g.ng);
+    const long long off = (long long)(r.jj + g.ng) * g.ldq + (c.ii + (r.st ? c.dst : 0) + g.ng);
 #pragma unroll
     for (int m = 0; m < NA; ++m) out[m] = plane_at<O32>(a + m * g.plane, off);
 }

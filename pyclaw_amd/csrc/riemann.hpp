@@ -202,6 +202,9 @@ template <typename R> struct Acoustics {
     static constexpr bool SPEEDS_FROM_CACHE = false;
     static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
     static constexpr bool CELL_STATE = false;  // no per-cell quantities shared between solves
+    // limiters run in every lane (zero waves keep phi = 1): the waves of
+    // these solvers are rarely zero across a whole wavefront
+    static constexpr bool SKIP_ZERO_WAVES = false;
     struct CellState {};
     __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
@@ -246,6 +249,9 @@ template <typename R> struct VCAcoustics {
     static constexpr bool SPEEDS_FROM_CACHE = false;
     static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
     static constexpr bool CELL_STATE = false;  // no per-cell quantities shared between solves
+    // limiters run in every lane (zero waves keep phi = 1): the waves of
+    // these solvers are rarely zero across a whole wavefront
+    static constexpr bool SKIP_ZERO_WAVES = false;
     struct CellState {};
     __device__ static __forceinline__ void speeds(const Cache &, R (&)[NW]) {}
     template <int DIR> static constexpr int MU = DIR == 0 ? 1 : 2;
@@ -323,6 +329,10 @@ template <typename R> struct Euler5 {
     // pressure's sum of squared momenta is commutative in the two directions).
     struct CellState { R rs, ms[2], hs, un[2], c; };
     static constexpr bool CELL_STATE = true;
+    // a wave family whose waves are zero in every lane of a wavefront skips
+    // its limiter (flows with quiescent regions, such as a shock entering
+    // gas at rest, have many)
+    static constexpr bool SKIP_ZERO_WAVES = true;
     __device__ static __forceinline__ CellState cell(const Params &P, const R (&q)[NE]) {
         CellState S;
         const R gamma1 = P.gamma1;
@@ -533,6 +543,7 @@ template <typename R> struct ShallowRoe {
     static constexpr int NE = 3, NW = 3, NA = 0;
     static constexpr bool HAS_EFIX = false;   // the entropy fix is always on
     static constexpr bool CELL_STATE = false;  // the per-state quantities below depend on the direction
+    static constexpr bool SKIP_ZERO_WAVES = false;
     struct CellState {};
     struct Params { R grav; };
     struct Cache { R u, v, a; };
