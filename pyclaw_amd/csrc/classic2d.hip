@@ -966,10 +966,14 @@ __global__ void batch_arm_kernel(int n, double *cfl, int *counter) {
 // Host side
 // ---------------------------------------------------------------------------
 static int band_height(int rows, int strips_x) {
-    // about 2048 wavefronts (two per SIMD) with bands of at least 4 rows; small
-    // grids trade the 3 extra rows each band marches through for more waves
-    long long target = 2048;
-    int h = (int)std::max<long long>(4, ((long long)rows * strips_x + target - 1) / target);
+    // about 2048 wavefronts (two per SIMD) with bands of at least 4 rows; where
+    // that gives bands under 8 rows (mid-size grids), about 1024 (one per
+    // SIMD): each band marches 3 rows more than it produces, which costs more
+    // there than a second wavefront per SIMD gains (vc acoustics 512x512:
+    // 35.0 us per step at 4-row bands, 27.8 at 5)
+    const long long work = (long long)rows * strips_x;
+    int h = (int)std::max<long long>(4, (work + 2047) / 2048);
+    if (h < 8) h = (int)std::max<long long>(4, (work + 1023) / 1024);
     return std::min(h, std::max(rows, 1));
 }
 
