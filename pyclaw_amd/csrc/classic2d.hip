@@ -190,16 +190,21 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     if constexpr (HAS_AUX) {
         load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(R0 - 2, g, p.ba), a_m1);
     }
-    // Solvers with registers to spare load the lead row one row ahead (qnext,
-    // anext), so its load overlaps the previous row's work instead of
-    // stalling the wavefront; the last row's prefetch reads the row after
-    // the band (a valid source row, not used).  The Euler solver has no
-    // registers left for it.
+    // Solvers with registers to spare load the lead row two rows ahead: row
+    // L + 1 is in qnext/anext and row L + 2 in qpre/apre when row L starts,
+    // and the rows move down a stage before the next load is issued, so no
+    // register copy reads a load issued in the same row (such a copy waits
+    // for the load).  The last rows' loads read rows after the band (valid
+    // source rows, not used).  The Euler solver has no registers left for it.
     constexpr bool PREFETCH = !RP::CELL_STATE;
-    R qnext[NE], anext[NA];
+    R qnext[NE], anext[NA], qpre[NE], apre[NA];
     if constexpr (PREFETCH) {
         load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(R0 - 1, g, p.bq), qnext);
-        if constexpr (HAS_AUX) load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(R0 - 1, g, p.ba), anext);
+        load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(R0, g, p.bq), qpre);
+        if constexpr (HAS_AUX) {
+            load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(R0 - 1, g, p.ba), anext);
+            load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(R0, g, p.ba), apre);
+        }
     }
     // per-cell quantities shared by the four interfaces around a cell (the
     // Euler solver's): stP for the row behind the lead row
@@ -223,12 +228,12 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         R qc[NE];
         if constexpr (PREFETCH) {
 #pragma unroll
-            for (int m = 0; m < NE; ++m) qc[m] = qnext[m];
-            load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L + 1, g, p.bq), qnext);
+            for (int m = 0; m < NE; ++m) { qc[m] = qnext[m]; qnext[m] = qpre[m]; }
+            load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L + 2, g, p.bq), qpre);
             if constexpr (HAS_AUX) {
 #pragma unroll
-                for (int a = 0; a < NA; ++a) a_0[a] = anext[a];
-                load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(L + 1, g, p.ba), anext);
+                for (int a = 0; a < NA; ++a) { a_0[a] = anext[a]; anext[a] = apre[a]; }
+                load_aux_cr<NA, R, O32>(p.aux, g, cola, row_source(L + 2, g, p.ba), apre);
             }
         } else {
             load_q_cr<NE, R, O32>(p.qold, g, colq, row_source(L, g, p.bq), qc);

@@ -87,13 +87,11 @@ template <> __device__ __forceinline__ float rsqrt_exact<float>(float x) { retur
 
 // Two quotients with one denominator, q1 = n1/d and q2 = n2/d.  The
 // shared-reciprocal build multiplies both numerators by rcp(d).  Otherwise
-// each quotient is the correctly rounded IEEE quotient, equal bit for bit to
-// n/d: the FP64 division sequence (scale the denominator, hardware
-// reciprocal, two Newton steps; scale the numerator, multiply, residual,
-// div_fmas, div_fixup) has a part that depends on the denominator's scaled
-// value only, and that part is done once when both numerators leave the
-// denominator unscaled alike (every pair except where the quotients come near
-// the ends of the exponent range; then n2/d is divided on its own).
+// each is the IEEE quotient n/d on its own.  (Sharing the denominator's part
+// of the division sequence between the two is exact only when both numerators
+// leave the denominator scaled alike, and the per-lane test for the other
+// case splits the code into small blocks: measured slower than two whole
+// divisions.)
 template <typename R>
 __device__ __forceinline__ void div2(R n1, R n2, R d, R &q1, R &q2) {
     if constexpr (SHARED_RCP) {
@@ -105,39 +103,7 @@ __device__ __forceinline__ void div2(R n1, R n2, R d, R &q1, R &q2) {
         q2 = n2 / d;
     }
 }
-template <> __device__ __forceinline__ void div2<double>(double n1, double n2, double d, double &q1, double &q2) {
-    if constexpr (SHARED_RCP) {
-        const double r = rcp<double>(d);
-        q1 = n1 * r;
-        q2 = n2 * r;
-    } else {
-        bool g1, g2, f1, f2;
-        const double ds1 = __builtin_amdgcn_div_scale(n1, d, false, &g1);
-        const double ds2 = __builtin_amdgcn_div_scale(n2, d, false, &g2);
-        const double r0 = __builtin_amdgcn_rcp(ds1);
-        const double e0 = __builtin_fma(-ds1, r0, 1.0);
-        const double r1 = __builtin_fma(r0, e0, r0);
-        const double e1 = __builtin_fma(-ds1, r1, 1.0);
-        const double r2 = __builtin_fma(r1, e1, r1);
-        const double ns1 = __builtin_amdgcn_div_scale(n1, d, true, &f1);
-        const double m1 = ns1 * r2;
-        const double rem1 = __builtin_fma(-ds1, m1, ns1);
-        q1 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem1, r2, m1, f1), d, n1);
-        if (__double_as_longlong(ds2) == __double_as_longlong(ds1)) {
-            const double ns2 = __builtin_amdgcn_div_scale(n2, d, true, &f2);
-            const double m2 = ns2 * r2;
-            const double rem2 = __builtin_fma(-ds1, m2, ns2);
-            q2 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem2, r2, m2, f2), d, n2);
-        } else {
-            q2 = n2 / d;
-        }
-    }
-}
-
-// Three quotients with one denominator, as div2: q_k = n_k/d.  The exact
-// build shares the denominator's part of the division sequence among the
-// numerators that leave it scaled alike (checked per numerator; a numerator
-// that does not is divided on its own), so each q_k equals n_k/d bit for bit.
+// Three quotients with one denominator, as div2: q_k = n_k/d.
 template <typename R>
 __device__ __forceinline__ void div3(R n1, R n2, R n3, R d, R &q1, R &q2, R &q3) {
     if constexpr (SHARED_RCP) {
@@ -151,46 +117,6 @@ __device__ __forceinline__ void div3(R n1, R n2, R n3, R d, R &q1, R &q2, R &q3)
         q3 = n3 / d;
     }
 }
-template <> __device__ __forceinline__ void div3<double>(double n1, double n2, double n3, double d, double &q1,
-                                                         double &q2, double &q3) {
-    if constexpr (SHARED_RCP) {
-        const double r = rcp<double>(d);
-        q1 = n1 * r;
-        q2 = n2 * r;
-        q3 = n3 * r;
-    } else {
-        bool g, f1, f2, f3;
-        const double ds1 = __builtin_amdgcn_div_scale(n1, d, false, &g);
-        const double r0 = __builtin_amdgcn_rcp(ds1);
-        const double e0 = __builtin_fma(-ds1, r0, 1.0);
-        const double r1 = __builtin_fma(r0, e0, r0);
-        const double e1 = __builtin_fma(-ds1, r1, 1.0);
-        const double r2 = __builtin_fma(r1, e1, r1);
-        const double ns1 = __builtin_amdgcn_div_scale(n1, d, true, &f1);
-        const double m1 = ns1 * r2;
-        const double rem1 = __builtin_fma(-ds1, m1, ns1);
-        q1 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem1, r2, m1, f1), d, n1);
-        const double ds2 = __builtin_amdgcn_div_scale(n2, d, false, &g);
-        if (__double_as_longlong(ds2) == __double_as_longlong(ds1)) {
-            const double ns2 = __builtin_amdgcn_div_scale(n2, d, true, &f2);
-            const double m2 = ns2 * r2;
-            const double rem2 = __builtin_fma(-ds1, m2, ns2);
-            q2 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem2, r2, m2, f2), d, n2);
-        } else {
-            q2 = n2 / d;
-        }
-        const double ds3 = __builtin_amdgcn_div_scale(n3, d, false, &g);
-        if (__double_as_longlong(ds3) == __double_as_longlong(ds1)) {
-            const double ns3 = __builtin_amdgcn_div_scale(n3, d, true, &f3);
-            const double m3 = ns3 * r2;
-            const double rem3 = __builtin_fma(-ds1, m3, ns3);
-            q3 = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem3, r2, m3, f3), d, n3);
-        } else {
-            q3 = n3 / d;
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Constant-coefficient acoustics: q = (p, u, v), parameters rho, bulk, c, Z.
 // ---------------------------------------------------------------------------
