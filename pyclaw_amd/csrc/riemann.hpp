@@ -499,9 +499,10 @@ template <typename R> struct ShallowRoe {
         S.hs = rsqrt_exact<R>(q[0]);
         div2<R>(q[mu], q[mv], S.hs, S.umh, S.vmh);
         S.un = SHARED_RCP ? q[mu] * rcp<R>(q[0]) : q[mu] / q[0];
-        const R gh = P.grav * q[0];
-        S.sgh = S.hs;
-        if (gh != q[0]) S.sgh = rsqrt_exact<R>(gh);   // g h equal to h (g = 1): sqrt(g h) is sqrt(h)
+        // with g = 1, g h is h and sqrt(g h) is sqrt(h) (also for g > 0 where
+        // g h equals h: h = 0 or inf); the test is on the uniform g, so it
+        // does not split the code per lane
+        S.sgh = P.grav == R(1) ? S.hs : rsqrt_exact<R>(P.grav * q[0]);
         return S;
     }
     template <int DIR>
