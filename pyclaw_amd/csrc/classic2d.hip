@@ -103,6 +103,18 @@ __host__ __device__ constexpr int wnz_slot(int mw, int m) {
     return k;
 }
 
+// The largest |s| over an interface's wave speeds, skipping speeds the
+// solver repeats: dt/dx * max |s| is the reference's max over the waves of
+// dt/dx * |s| exactly (rounding is monotone, and IEEE max skips NaN either way).
+template <class RP, typename R>
+__device__ __forceinline__ R max_abs_speed(const R (&s)[RP::NW]) {
+    R m = rabs(s[0]);
+#pragma unroll
+    for (int mw = 1; mw < RP::NW; ++mw)
+        if (RP::SDIST(mw)) m = fmax_ieee(m, rabs(s[mw]));
+    return m;
+}
+
 template <class RP>
 struct CarrySlots {
     static constexpr int UAP = 0, UF = RP::NE, UBM = 2 * RP::NE, UBP = 3 * RP::NE, DOTL = 4 * RP::NE;
@@ -261,8 +273,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             if constexpr (CELL) RP::template rpn_c<0>(P, lo, qprev, RP::cell_up(stP), stP, wave, s, amdq, apdq, cx);
             else RP::template rpn<0>(P, lo, qprev, alo, a_m1, wave, s, amdq, apdq, cx);
             if (lane >= 1 && row >= -1 && row <= g.my && i >= 0 && i <= g.mx) {
-#pragma unroll
-                for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
+                cfl = fmax_ieee(cfl, dtdx * max_abs_speed<RP, R>(s));
             }
 #pragma unroll
             for (int m = 0; m < NE; ++m) { fadd[m] = R(0); cq[m] = R(0); }
@@ -324,13 +335,20 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
             }
 #pragma unroll
             for (int m = 0; m < NE; ++m) {
-                R dAm = lane_dn(amdq[m]), dF = lane_dn(fadd[m]);
                 R dBm = lane_dn(bmA[m]), dBp = lane_dn(bpA[m]);
-                R qadd = (R(0) - dtdx * apdq[m]) - dtdx * dAm;
                 R g1 = (R(0) - hx * dBm) - hx * bmP[m];
                 R g2 = (R(0) - hx * dBp) - hx * bpP[m];
                 R v = qprev[m] + dtdy * g2Xs[m];
-                v = ((v + qadd) - dtdx * (dF - fadd[m])) - dtdy * (g2 - g1);
+                if constexpr (SHARED_RCP) {
+                    // the right neighbour's A-dq and correction flux in one
+                    // lane move (the same sum regrouped: no bitwise promise here)
+                    R dAF = lane_dn(amdq[m] + fadd[m]);
+                    v = ((v - dtdx * apdq[m]) - dtdx * (dAF - fadd[m])) - dtdy * (g2 - g1);
+                } else {
+                    R dAm = lane_dn(amdq[m]), dF = lane_dn(fadd[m]);
+                    R qadd = (R(0) - dtdx * apdq[m]) - dtdx * dAm;
+                    v = ((v + qadd) - dtdx * (dF - fadd[m])) - dtdy * (g2 - g1);
+                }
                 accOld[m] = accOld[m] - dtdy * g1;
                 accNew[m] = v;
                 g2Xs[m] = g2;
@@ -343,8 +361,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         if constexpr (CELL) RP::template rpn_c<1>(P, qprev, qc, stP, stC, wN, sN, amN, apN, cN);
         else RP::template rpn<1>(P, qprev, qc, a_m1, a_0, wN, sN, amN, apN, cN);
         if (L >= 0 && L <= g.my && i >= -1 && i <= g.mx) {
-#pragma unroll
-            for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdy * rabs(sN[mw]));
+            cfl = fmax_ieee(cfl, dtdy * max_abs_speed<RP, R>(sN));
         }
         R wY[NW][NE];
 #pragma unroll
@@ -597,8 +614,7 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
         Cache cx;
         RP::template rpn<0>(P, lo, hi, alo, ahi, wave, s, amdq, apdq, cx);
         if (i >= 0 && i <= g.mx) {
-#pragma unroll
-            for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdx * rabs(s[mw]));
+            cfl = fmax_ieee(cfl, dtdx * max_abs_speed<RP, R>(s));
         }
 #pragma unroll
         for (int m = 0; m < NE; ++m) fadd[m] = R(0);
@@ -712,8 +728,7 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
         Cache cN;
         RP::template rpn<1>(P, qm1, qc, aprev, a0, wN, sN, amN, apN, cN);
         if (L >= 0 && L <= g.my && valid_col) {
-#pragma unroll
-            for (int mw = 0; mw < NW; ++mw) cfl = fmax_ieee(cfl, dtdy * rabs(sN[mw]));
+            cfl = fmax_ieee(cfl, dtdy * max_abs_speed<RP, R>(sN));
         }
 #pragma unroll
         for (int mw = 0; mw < NW; ++mw) {

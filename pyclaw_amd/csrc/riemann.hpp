@@ -128,6 +128,7 @@ template <typename R> struct Acoustics {
     static constexpr bool SPEEDS_FROM_CACHE = false;
     static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
     static constexpr bool CELL_STATE = false;  // no per-cell quantities shared between solves
+    __host__ __device__ static constexpr bool SDIST(int) { return true; }   // every wave speed may differ
     // limiters run in every lane (zero waves keep phi = 1): the waves of
     // these solvers are rarely zero across a whole wavefront
     static constexpr bool SKIP_ZERO_WAVES = false;
@@ -175,6 +176,7 @@ template <typename R> struct VCAcoustics {
     static constexpr bool SPEEDS_FROM_CACHE = false;
     static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
     static constexpr bool CELL_STATE = false;  // no per-cell quantities shared between solves
+    __host__ __device__ static constexpr bool SDIST(int) { return true; }   // every wave speed may differ
     // limiters run in every lane (zero waves keep phi = 1): the waves of
     // these solvers are rarely zero across a whole wavefront
     static constexpr bool SKIP_ZERO_WAVES = false;
@@ -229,7 +231,10 @@ template <typename R> struct Euler5 {
     struct Params { R gamma, gamma1; int efix; };
     // Roe averages kept for the transverse solves of the same interface; the
     // other quantities rpt needs are recomputed with rpn's expressions
-    struct Cache { R u, v, enth, a; };
+    // (g1a2 = gamma1 / a^2 and, in the shared-reciprocal build, i2a = 1/(2a):
+    // rpn's values, so a transverse solve of a carried interface does not
+    // divide again)
+    struct Cache { R u, v, enth, a, g1a2, i2a; };
     // the wave speeds of an interface follow from its cache (rpn's expressions)
     static constexpr bool SPEEDS_FROM_CACHE = true;
     static constexpr bool HAS_STATE = false;   // no per-state quantities shared between solves
@@ -255,6 +260,8 @@ template <typename R> struct Euler5 {
     // pressure's sum of squared momenta is commutative in the two directions).
     struct CellState { R rs, ms[2], hs, un[2], c; };
     static constexpr bool CELL_STATE = true;
+    // speeds u - a, u, u, u + a, u: waves 2 and 4 repeat wave 1's speed
+    __host__ __device__ static constexpr bool SDIST(int mw) { return mw != 2 && mw != 4; }
     // a wave family whose waves are zero in every lane of a wavefront skips
     // its limiter (flows with quiescent regions, such as a shock entering
     // gas at rest, have many)
@@ -306,7 +313,8 @@ template <typename R> struct Euler5 {
         R a = rsqrt_exact<R>(a2);
         R g1a2 = SHARED_RCP ? gamma1 * rcp<R>(a2) : gamma1 / a2;
         R euv = enth - u2v2;
-        C.u = u; C.v = v; C.enth = enth; C.a = a;
+        const R i2a = SHARED_RCP ? rcp<R>(R(2) * a) : R(0);
+        C.u = u; C.v = v; C.enth = enth; C.a = a; C.g1a2 = g1a2; C.i2a = i2a;
 
         R d0 = hi[0] - lo[0];
         R d1 = hi[mu] - lo[mu];
@@ -314,7 +322,7 @@ template <typename R> struct Euler5 {
         R d3 = hi[3] - lo[3];
         R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
         R a2w = d2 - v * d0;
-        R a4 = SHARED_RCP ? (d1 + (a - u) * d0 - a * a3) * rcp<R>(R(2) * a) : (d1 + (a - u) * d0 - a * a3) / (R(2) * a);
+        R a4 = SHARED_RCP ? (d1 + (a - u) * d0 - a * a3) * i2a : (d1 + (a - u) * d0 - a * a3) / (R(2) * a);
         R a1 = d0 - a3 - a4;
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v;
         wave[0][3] = a1 * (enth - u * a); wave[0][4] = R(0); s[0] = u - a;
@@ -423,16 +431,16 @@ template <typename R> struct Euler5 {
                                                const R *, R (&bm)[NE], R (&bp)[NE], const Cache &C) {
         constexpr int mu = MU<DIR>, mv = MV<DIR>;
         const R u = C.u, v = C.v, a = C.a, enth = C.enth;
-        // the same expressions as in rpn (rpn2_euler_5wave.f:100-104), so the values are identical
+        // the same expressions as in rpn (rpn2_euler_5wave.f:100-104), so the
+        // values are identical; gamma1 / a^2 is rpn's own
         const R u2v2 = u * u + v * v;
-        const R g1a2 = SHARED_RCP ? P.gamma1 * rcp<R>(P.gamma1 * (enth - R(0.5) * u2v2))
-                                  : P.gamma1 / (P.gamma1 * (enth - R(0.5) * u2v2));
+        const R g1a2 = C.g1a2;
         const R euv = enth - u2v2;
         R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
         R a2 = asdq[mu] - u * asdq[0];
         R a4;
-        if constexpr (SHARED_RCP)   // 1/(2a) is shared by the interface's two transverse solves
-            a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) * rcp<R>(R(2) * a);
+        if constexpr (SHARED_RCP)   // 1/(2a) is rpn's, shared by the interface's two transverse solves
+            a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) * C.i2a;
         else
             a4 = (asdq[mv] + (a - v) * asdq[0] - a * a3) / (R(2) * a);
         R a1 = asdq[0] - a3 - a4;
@@ -469,6 +477,7 @@ template <typename R> struct ShallowRoe {
     static constexpr int NE = 3, NW = 3, NA = 0;
     static constexpr bool HAS_EFIX = false;   // the entropy fix is always on
     static constexpr bool CELL_STATE = false;  // the per-state quantities below depend on the direction
+    __host__ __device__ static constexpr bool SDIST(int) { return true; }
     static constexpr bool SKIP_ZERO_WAVES = false;
     struct CellState {};
     struct Params { R grav; };
