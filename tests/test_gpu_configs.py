@@ -25,7 +25,7 @@ import pytest
 import apps
 import oracle as O
 from helpers import NG, _bc_struct, _dev_grid, _rp_struct, from_device_interior, gpu_classic_step, \
-    RP_CASES, to_device_padded
+    oracle_classic_step, padded_with_bc, RP_CASES, to_device_padded
 from test_fixtures_oracle import oracle_run
 from test_gpu_apps import amd_controller
 
@@ -233,3 +233,43 @@ def test_config3_sharpclaw_large_grid_default_shape():
         out.append((from_device_interior(dq, mx, my), float(cfl.item())))
     for got, cgot in out:
         assert np.array_equal(got, ref) and cgot == cref
+
+
+@pytest.mark.parametrize("name,trans", [("vc_acoustics", 1), ("acoustics", 2), ("shallow", 1)])
+def test_midsize_default_band_rule_equals_oracle(name, trans):
+    """Mid-size grids (512x512, 9 strips): the default launch shape sizes its
+    row bands for about one wavefront per SIMD (5 rows here, where two per
+    SIMD would give 4-row bands); two steps of the light solvers' one-block
+    row march at that shape (its odd and even row counts, walls low,
+    extrapolation high, MC, transverse increment or correction waves) equal
+    the oracle bit for bit."""
+    from pyclaw_amd import _lib
+    print("mid-size 512x512 default band rule:", name, "transverse", trans, flush=True)
+    mx = my = 512
+    case = RP_CASES[name]
+    rng = np.random.default_rng(11)
+    ctx = _lib.Context(0)
+    band, w, wpb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    m = _lib.ClawMethod()
+    m.order, m.transverse, m.split, m.num_waves = 2, trans, 0, case['nw']
+    for k in range(case['nw']):
+        m.mthlim[k] = 4
+    _lib.check(_lib.lib().claw2d_classic_get_launch(ctx.handle, _dev_grid(mx, my, 1.0, 1.0), m, _rp_struct(case),
+                                                   _lib.F64, ctypes.byref(band), ctypes.byref(w), ctypes.byref(wpb)),
+               'claw2d_classic_get_launch')
+    assert 5 <= band.value <= 7, band.value
+    q = case['make'](rng, mx, my)
+    aux = None
+    if case['naux']:
+        from helpers import vc_aux
+        aux = vc_aux(rng, mx, my)
+    dx, dy = 1.0 / mx, 1.0 / my
+    dt = 0.3 * dx
+    for k in range(2):
+        qbc = padded_with_bc(q, 2, (3, 3), (1, 1), True)
+        auxbc = padded_with_bc(aux, 2, (3, 3), (1, 1), False) if aux is not None else None
+        ref, cref = oracle_classic_step(qbc, auxbc, case, dt, 2, trans, dx=dx, dy=dy)
+        got, cgot = gpu_classic_step(qbc, auxbc, case, dt, 2, trans, bc_lower=(3, 3), bc_upper=(1, 1),
+                                     abc_lower=(3, 3), abc_upper=(1, 1), dx=dx, dy=dy)
+        assert np.array_equal(got, ref) and cgot == cref, "step %d max %g" % (k, np.abs(got - ref).max())
+        q = ref
