@@ -238,10 +238,13 @@ def main():
 
     band = None
     if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
-        # 18-row bands (69 strips x 57 bands: just under two rounds of the 2048 wavefront slots),
-        # one-wavefront blocks: the fastest shape in repeated 300-step runs (the tuner ranks shapes
-        # on the initial data, where the per-step cost differs)
-        args.launch = '18,2,1'
+        # the paired fill decomposition (one-wavefront blocks of the two-per-SIMD build, one per
+        # wavefront slot, each SIMD's older wavefront taking two units of rows (three in the
+        # reference-arithmetic build) for the younger one's one): the library's default for this
+        # grid, pinned so the line does not depend on
+        # the tuner, which ranks shapes on the initial data, where the per-step cost differs
+        # (--launch 18,2,1: the bands in about two rounds of wavefronts before it)
+        args.launch = '0,2,1'
     if args.launch:
         shape = [int(v) for v in args.launch.split(',')]
         rows_, waves_, wpb_ = shape[0], shape[1], (shape[2] if len(shape) > 2 else 4)
@@ -249,7 +252,8 @@ def main():
             _lib.check(L.claw2d_classic_set_launch_ex(ctx.handle, grid, m, r, pr_, rows_, waves_, wpb_),
                        'claw2d_classic_set_launch_ex')
     elif not args.no_autotune:
-        # pick the kernel's launch shape for this grid (answers do not depend on it)
+        # pick the kernel's launch shape for this grid (answers do not depend on it beyond the last
+        # bits of the fused build)
         hb = ctypes.c_int(0)
         _lib.check(L.claw2d_classic_autotune(ctx.handle, grid, m, r, bc, prec, qa.data_ptr(), qb.data_ptr(), None,
                                              dtd.data_ptr(), cfl.data_ptr(), sh, ctypes.byref(hb)),

@@ -146,15 +146,23 @@ int claw2d_classic_step_noalloc(void *ctx, const ClawGrid *grid, const ClawMetho
    bands mean more wavefronts in flight, fewer mean fewer halo rows; the
    candidates include the height that makes one band per wavefront slot of
    the device), each with the kernel compiled for one and for two
-   wavefronts per SIMD and with blocks of one or four wavefronts, and
+   wavefronts per SIMD and with blocks of one or four wavefronts, and the
+   paired fill decomposition where it tiles the grid (band 0 below), and
    keeps the fastest for this context and problem shape (Riemann solver,
    order, transverse mode, precision, mx, my); later classic steps of that
-   shape use it.  Results do not depend on the launch shape (every cell is
-   computed the same way and the Courant maximum is order-free), so tuning
-   never changes answers.  Runs real steps from qold into qnew and
+   shape use it.  With CLAW_F64 and CLAW_F32 results do not depend on the
+   launch shape (every cell is computed the same way and the Courant maximum
+   is order-free), so tuning never changes answers; with CLAW_F64_FMA the
+   march runs two rows per loop trip and, for the light solvers, a band's
+   first row in a copy of its own, and the compiler's contractions can round
+   these copies differently, so answers can move in the last bits with the
+   band starts (Euler and acoustics: equal for every shape whose bands start
+   on even rows, the paired fill decomposition included).  Runs real steps
+   from qold into qnew and
    max-combines into *cfl_dev.  Blocks until the timing launches finish.
    *band_out (optional) receives the chosen height, negated when the
-   two-waves-per-SIMD build won; 0 for the split step (not tuned).
+   two-waves-per-SIMD build won; 0 for the paired fill decomposition and
+   for the split step (not tuned).
    No counterpart in the reference (its step has no launch shape). */
 int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *method,
                             const ClawRiemann *rp, const ClawBC *bc, int precision,
@@ -169,13 +177,23 @@ int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod 
                               const ClawRiemann *rp, int precision, int band, int waves_per_simd);
 
 /* Same, also setting the wavefronts per block (1, 2 or 4; the shape set by
-   claw2d_classic_set_launch uses 4).  No counterpart in the reference. */
+   claw2d_classic_set_launch uses 4).  Band 0 with waves_per_simd 2 and
+   waves_per_block 1 selects the paired fill decomposition: one-wavefront
+   blocks, one per wavefront slot of the device; the two wavefronts that
+   share a SIMD split their rows 2 : 1 (CLAW_F64_FMA) or 3 : 1, the larger
+   part to the one dispatched first, which the SIMD's issue arbitration
+   favours; every
+   piece lies within one column strip.  Where that cannot tile the grid
+   (fewer pairs than strips, or units under a row) the step runs the
+   default bands.  The default shape for FP64 grids of at least 16384
+   strip-rows (strips of 60 columns).  No counterpart in the reference. */
 int claw2d_classic_set_launch_ex(void *ctx, const ClawGrid *grid, const ClawMethod *method,
                                  const ClawRiemann *rp, int precision, int band, int waves_per_simd,
                                  int waves_per_block);
 
 /* The launch shape classic steps of this problem shape use on this context
-   (tuned, set, or the default).  No counterpart in the reference. */
+   (tuned, set, or the default; band 0 is the paired fill decomposition).
+   No counterpart in the reference. */
 int claw2d_classic_get_launch(void *ctx, const ClawGrid *grid, const ClawMethod *method,
                               const ClawRiemann *rp, int precision, int *band, int *waves_per_simd,
                               int *waves_per_block);

@@ -52,6 +52,15 @@
 #endif
 
 namespace clawhip {
+// The older wavefront's share of a SIMD in the paired fill decomposition
+// (unsplit_block), in units of the younger one's, per build.  Euler
+// shock-bubble 4096x1024, Gcell-updates/s by share, fused build: 2: 22.8,
+// 3: 21.8; reference arithmetic: 2: 14.3, 3: 14.5 (18-row bands in about two
+// rounds of wavefronts: 20.1 and 12.9): the fused build's older wavefront
+// gets further ahead of the younger one on the same SIMD.
+constexpr int FILL_SHARE_FUSED = 2, FILL_SHARE_EXACT = 3;
+inline int fill_share(int precision) { return precision == CLAW_F64_FMA ? FILL_SHARE_FUSED : FILL_SHARE_EXACT; }
+
 namespace CLAW_CLASSIC_VARIANT {
 
 constexpr int XS_USEFUL = WAVE - 3;   // produced columns per wavefront, split x pass
@@ -83,6 +92,8 @@ struct ClassicParams {
     int nsx, nsy;   // column strips, row bands
     int nsyb;       // row-band groups (one group per block)
     int wpb;        // wavefronts per block
+    int fill;       // > 0: the paired fill decomposition (see unsplit_block), the
+                    // older wavefront's share of a SIMD in units of the younger's
 };
 
 
@@ -122,7 +133,7 @@ struct CarrySlots {
     static constexpr int COUNT = WY + wnz_slot<RP, 1>(RP::NW, 0);
 };
 
-// One work item of the unsplit step: column strip sx, row band sy (whole
+// One work item of the unsplit step: rows [R0, R1) of column strip sx (whole
 // wavefront, uniform arguments).  Used by the single-grid kernel below and by
 // the batched work-queue kernel.  cs: this wavefront's carry slots
 // (CarrySlots<RP>::COUNT x WAVE values in LDS).
@@ -133,7 +144,7 @@ struct CarrySlots {
 // limited with the MC limiter (id 4) and a solver with an entropy fix has it
 // on; 3, as 2 with transverse increment waves (transverse = 1).
 template <class RP, typename R, bool O32 = false, int SPEC = 0>
-__device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int sy, R *__restrict__ cs) {
+__device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int R0, int R1, R *__restrict__ cs) {
     constexpr bool M22 = SPEC >= 1;
     constexpr int SPEC_TRANS = SPEC == 3 ? 1 : 2;
     constexpr bool ALL_MC = SPEC >= 2;
@@ -148,11 +159,10 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
     // active here): say so, so that row indices, row addresses and the
     // ghost-row rules are scalar work with scalar branches
     sx = __builtin_amdgcn_readfirstlane(sx);
-    sy = __builtin_amdgcn_readfirstlane(sy);
+    R0 = __builtin_amdgcn_readfirstlane(R0);
+    R1 = __builtin_amdgcn_readfirstlane(R1);
     const int c0 = sx * XU_USEFUL;
     const int i = c0 - 2 + lane;
-    const int R0 = sy * p.H;
-    const int R1 = min(R0 + p.H, g.my);
     // lanes 0, 1 and 62, 63 are halo: lane 0's own x-interface would need the
     // column left of the strip and is never used (it is not a limiter neighbour
     // of a produced cell's interface), lane 1's is
@@ -518,14 +528,70 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 // dynamic LDS (p.wpb x CarrySlots::COUNT x WAVE values).
 extern __shared__ __attribute__((aligned(16))) unsigned char classic_carry_lds[];
 
+// The paired fill decomposition: one wavefront per slot of the two-per-SIMD
+// build, in blocks of one wavefront.  Blocks are dispatched to the XCDs in
+// turn (block b to XCD b % 8) and each XCD fills one slot of every SIMD
+// before the second, so blocks loc and loc + half of an XCD (loc = b / 8,
+// half = gridDim.x / 16) share a SIMD, and the first of them is the older
+// wavefront there, which the SIMD's issue arbitration favours (oldest
+// first): run an equal share each and the older one finishes after about two
+// thirds of the younger one's time, which then runs alone.  So the pair's
+// work is split unequally: with P = gridDim.x / 2 pairs and S strips, strip
+// s holds U_s = floor(P (A+1) (s+1) / S) - floor(P (A+1) s / S) units of
+// rows (A = p.fill), n_s = floor(P (s+1) / S) - floor(P s / S) older pieces
+// of A units each first, then U_s - A n_s younger pieces of one unit (kept
+// together: neighbouring pieces of one kind go to blocks dispatched one
+// after the other, which share an XCD's L2 for the rows between them;
+// alternating the kinds ran 7% slower); a
+// unit is my / U_s rows, the row boundaries inside a strip rounded down to
+// an even row (band starts of one parity, as with bands of an even height:
+// the fused build's two-row march then runs each row through the same copy
+// of the row step as there, and its results equal theirs).  Every piece lies
+// in one strip (one band per wavefront), the pieces tile every strip exactly
+// whatever the placement, and only the speed depends on the dispatch
+// order.  The host enables it where every strip gets at least one older
+// piece, no strip has fewer units than A n_s, and no unit is under two rows
+// (so no piece is empty).
+template <class RP, typename R>
+__device__ __forceinline__ void fill_piece(const ClassicParams<RP, R> &p, int &sx, int &R0, int &R1) {
+    const int b = blockIdx.x, half = gridDim.x >> 4, loc = b >> 3;
+    const bool young = loc >= half;
+    const int idx = (b & 7) * half + (young ? loc - half : loc);   // pair index
+    const int P = gridDim.x >> 1, S = p.nsx, A = p.fill, UT = P * (A + 1);
+    int s;
+    if (!young) {
+        s = ((idx + 1) * S - 1) / P;           // floor(P s / S) <= idx < floor(P (s+1) / S)
+    } else {
+        // younger pieces before strip s: floor(UT s / S) - A floor(P s / S),
+        // non-decreasing in s, within a strip of s * P / S
+        s = min(idx * S / P, S - 1);
+        while (s > 0 && UT * s / S - A * (P * s / S) > idx) --s;
+        while (s < S - 1 && UT * (s + 1) / S - A * (P * (s + 1) / S) <= idx) ++s;
+    }
+    const int O0 = P * s / S, U0 = UT * s / S, Us = UT * (s + 1) / S - U0;
+    const int u0 = young ? A * (P * (s + 1) / S - O0) + idx - (U0 - A * O0) : A * (idx - O0);
+    const int u1 = u0 + (young ? 1 : A);
+    sx = s;
+    R0 = (p.g.my * u0 / Us) & ~1;
+    R1 = u1 == Us ? p.g.my : (p.g.my * u1 / Us) & ~1;
+}
+
 template <class RP, typename R, bool O32, int SPEC>
 __device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
-    const int vb = xcd_block();
-    const int sx = vb / p.nsyb;
-    const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-    if (sx >= p.nsx || sy >= p.nsy) return;
+    int sx, R0, R1;
     R *carry = reinterpret_cast<R *>(classic_carry_lds);
-    unsplit_band<RP, R, O32, SPEC>(p, sx, sy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+    if (p.fill > 0) {
+        fill_piece(p, sx, R0, R1);
+    } else {
+        const int vb = xcd_block();
+        sx = vb / p.nsyb;
+        const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
+        if (sx >= p.nsx || sy >= p.nsy) return;
+        R0 = sy * p.H;
+        R1 = min(R0 + p.H, p.g.my);
+        carry += (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE;
+    }
+    unsplit_band<RP, R, O32, SPEC>(p, sx, R0, R1, carry);
 }
 
 template <class RP, typename R, bool O32, int SPEC>
@@ -868,8 +934,10 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<R
         p.order = b.order; p.trans = b.trans;
 #pragma unroll
         for (int q = 0; q < 8; ++q) p.mthlim[q] = b.mthlim[q];
-        p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0;
-        unsplit_band<RP, R, O32, SPEC>(p, local / I.nsy, local % I.nsy, carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+        p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0; p.fill = 0;
+        const int sy = local % I.nsy;
+        unsplit_band<RP, R, O32, SPEC>(p, local / I.nsy, sy * I.H, min(sy * I.H + I.H, I.my),
+                                       carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
     }
 }
 
@@ -997,6 +1065,35 @@ static int band_height(int rows, int strips_x) {
     return std::min(h, std::max(rows, 1));
 }
 
+// this build's share in the paired fill decomposition
+#ifdef CLAW_SHARED_RECIPROCALS
+constexpr int FILL_SHARE = FILL_SHARE_FUSED;
+#else
+constexpr int FILL_SHARE = FILL_SHARE_EXACT;
+#endif
+
+// Whether the paired fill decomposition (unsplit_block) can tile `rows` rows
+// of `strips` strips with `blocks` one-wavefront blocks and share A: every
+// strip gets at least one older piece, no strip has fewer units than its
+// older pieces take, no unit is under two rows, and the kernel's integer
+// arithmetic stays within 32 bits.
+static bool fill_usable(int blocks, int strips, int rows, int A) {
+    if (blocks < 16 || blocks % 16 != 0 || strips < 1 || rows < 1 || A < 1) return false;
+    const long long P = blocks / 2, S = strips, UT = P * (A + 1);
+    if (UT * S >= (1LL << 31) || P * (S + 1) >= (1LL << 31)) return false;
+    for (long long s = 0; s < S; ++s) {
+        const long long no = P * (s + 1) / S - P * s / S, us = UT * (s + 1) / S - UT * s / S;
+        if (no < 1 || us < A * no || 2 * us > rows || (long long)rows * us >= (1LL << 31)) return false;
+    }
+    return true;
+}
+
+static int device_cus(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    return cus;
+}
+
 static TuneKey tune_key(const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, int precision) {
     return TuneKey{rp->kind, meth->order, meth->transverse, precision, grid->mx, grid->my};
 }
@@ -1062,34 +1159,48 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         p.H = band_height(grid->my, p.nsx);
         int w2 = 0;
         const long long work = (long long)grid->my * p.nsx;
-        const bool chosen = ctx->force_band > 0 ||
+        const bool chosen = ctx->force_band != 0 ||
                             ctx->band.count(tune_key(grid, meth, rp, std::is_same<R, float>::value
                                                                          ? CLAW_F32 : CLAW_CLASSIC_F64_KEY));
+        int fill = 0;
         if (!chosen && std::is_same<R, double>::value && work >= 8LL * 2048) {
             // FP64 grids with work for more than two wavefronts per SIMD at
-            // 4-row bands: the two-waves-per-SIMD build in one-wavefront blocks
-            // with bands for ~3926 wavefronts (4096x1024: 18 rows, the tuned
-            // shape; 14.4 -> 18.3 Gcell/s for Euler against 35-row bands of
-            // the one-wave build in 4-wavefront blocks)
+            // 4-row bands: the two-waves-per-SIMD build in one-wavefront
+            // blocks, in the paired fill decomposition where it tiles the grid,
+            // else with bands for ~3926 wavefronts (4096x1024: 18 rows; 14.4
+            // -> 18.3 Gcell/s for Euler against 35-row bands of the one-wave
+            // build in 4-wavefront blocks, 20.2 -> 22.0 for the paired fill)
             w2 = 1;
             wpb = 1;
+            fill = FILL_SHARE;
             p.H = (int)std::min<long long>(grid->my, std::max<long long>(4, (work + 3925) / 3926));
         }
-        if (ctx->force_band > 0) {
-            p.H = std::min(ctx->force_band, grid->my);
+        if (ctx->force_band != 0) {   // a tuner candidate; -1 is the paired fill decomposition
+            if (ctx->force_band > 0) p.H = std::min(ctx->force_band, grid->my);
+            fill = ctx->force_band < 0 ? FILL_SHARE : 0;
             w2 = ctx->force_w == 2;
             wpb = ctx->force_wpb > 0 ? ctx->force_wpb : wpb;
         } else {
             const TuneKey key = tune_key(grid, meth, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_CLASSIC_F64_KEY);
             auto it = ctx->band.find(key);
-            if (it != ctx->band.end()) { p.H = std::min(it->second.first, grid->my); w2 = it->second.second == 2; }
+            if (it != ctx->band.end()) {
+                if (it->second.first > 0) p.H = std::min(it->second.first, grid->my);
+                fill = it->second.first > 0 ? 0 : FILL_SHARE;
+                w2 = it->second.second == 2;
+            }
             auto iw = ctx->wpb.find(key);
             if (iw != ctx->wpb.end()) wpb = iw->second;
         }
+        // the paired fill decomposition: one-wavefront blocks of the
+        // two-per-SIMD build, one per slot (where it does not tile the grid,
+        // the bands above)
+        const int fill_blocks = device_cus(ctx->device) * 8;
+        p.fill = fill && w2 && fill_usable(fill_blocks, p.nsx, grid->my, fill) ? fill : 0;
+        if (p.fill) wpb = 1;
         p.nsy = (grid->my + p.H - 1) / p.H;
         p.wpb = wpb;
         p.nsyb = (p.nsy + wpb - 1) / wpb;
-        int nblocks = p.nsx * p.nsyb;
+        const int nblocks = p.fill ? fill_blocks : p.nsx * p.nsyb;
         const size_t lds = (size_t)wpb * CarrySlots<RP>::COUNT * WAVE * sizeof(R);
         // 32-bit offsets when every byte offset within a component plane fits
         const bool o32 = !ctx->wide_offsets && (unsigned long long)grid->plane * sizeof(R) <= 0xffffffffull;
@@ -1493,8 +1604,11 @@ int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod 
 int claw2d_classic_set_launch_ex(void *ctx, const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp,
                                  int precision, int band, int waves_per_simd, int waves_per_block) {
     if (!ctx || !grid || !meth || !rp) return CLAW_EINVAL;
-    if (band < 1 || (waves_per_simd != 1 && waves_per_simd != 2)) return CLAW_EINVAL;
+    if (band < 0 || (waves_per_simd != 1 && waves_per_simd != 2)) return CLAW_EINVAL;
     if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) return CLAW_EINVAL;
+    // band 0: the paired fill decomposition, which runs the two-per-SIMD build
+    // in one-wavefront blocks
+    if (band == 0 && (waves_per_simd != 2 || waves_per_block != 1)) return CLAW_EINVAL;
     Ctx *c = (Ctx *)ctx;
     const TuneKey key = tune_key(grid, meth, rp, precision);
     c->band[key] = std::make_pair(band, waves_per_simd);
@@ -1516,7 +1630,8 @@ int claw2d_classic_get_launch(void *ctx, const ClawGrid *grid, const ClawMethod 
         *waves_per_simd = 1;
         *waves_per_block = 4;
         if (precision != CLAW_F32 && !meth->split && work >= 8LL * 2048) {
-            *band = (int)std::min<long long>(grid->my, std::max<long long>(4, (work + 3925) / 3926));
+            *band = fill_usable(device_cus(c->device) * 8, strips, grid->my, fill_share(precision))
+                        ? 0 : (int)std::min<long long>(grid->my, std::max<long long>(4, (work + 3925) / 3926));
             *waves_per_simd = 2;
             *waves_per_block = 1;
         }
@@ -1576,6 +1691,28 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (best == 0 || ms < best_ms) { best = h; best_w = w; best_wpb = wpb; best_ms = ms; }
+    }
+    // the paired fill decomposition (two-per-SIMD build, one-wavefront blocks)
+    if (!rc && fill_usable(device_cus(c->device) * 8, strips, grid->my, fill_share(precision))) {
+        c->force_band = -1;
+        c->force_w = 2;
+        c->force_wpb = 1;
+        rc = claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev,
+                                    CLAW_STEP_ACCUMULATE_CFL, stream);
+        const int reps = 5;
+        if (!rc) (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = claw2d_classic_step_ex(ctx, grid, meth, rp, bc, precision, qold, qnew, aux, dt_dev, cfl_dev,
+                                        CLAW_STEP_ACCUMULATE_CFL, stream);
+        if (!rc) {
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = CLAW_EDEVICE;
+        }
+        if (!rc) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) { best = 0; best_w = 2; best_wpb = 1; best_ms = ms; }
+        }
     }
     c->force_band = 0;
     c->force_w = 0;

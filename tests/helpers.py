@@ -64,12 +64,14 @@ def padded_with_bc(interior, ng, bc_lower, bc_upper, negate, ghost_rng=None):
 
 def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False, mthlim=None,
                      bc_lower=(1, 1), bc_upper=(1, 1), abc_lower=(1, 1), abc_upper=(1, 1),
-                     dx=0.1, dy=0.07, precision='f64', wide_offsets=False, efix=1):
+                     dx=0.1, dy=0.07, precision='f64', wide_offsets=False, efix=1, shape=None):
     """Run claw2d_classic_step on the GPU.  qbc_ref/auxbc_ref are reference-layout
     padded arrays (ghost width 2 for the classic step); they are copied into the
     device layout with NG ghost layers (the outermost layer is padding).
     wide_offsets: the unsplit kernels with 64-bit per-lane offsets
-    (claw_ctx_set_wide_offsets)."""
+    (claw_ctx_set_wide_offsets).  shape: (rows, waves per SIMD, waves per
+    block) set on the context first (claw2d_classic_set_launch_ex; rows 0 is
+    the paired fill decomposition)."""
     import torch
     from pyclaw_amd import _lib
     ne, nxp, nyp = qbc_ref.shape
@@ -110,6 +112,9 @@ def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False
     if wide_offsets:
         _lib.check(_lib.lib().claw_ctx_set_wide_offsets(ctx.handle, 1), 'claw_ctx_set_wide_offsets')
     prec = {'f64': _lib.F64, 'f32': _lib.F32, 'f64fma': _lib.F64_FMA}[precision]
+    if shape is not None:
+        _lib.check(_lib.lib().claw2d_classic_set_launch_ex(ctx.handle, grid, m, r, prec, *shape),
+                   'claw2d_classic_set_launch_ex')
     rc = _lib.lib().claw2d_classic_step(ctx.handle, grid, m, r, bc, prec,
                                         qd.data_ptr(), qn.data_ptr(), ad.data_ptr() if ad is not None else None,
                                         dtd.data_ptr(), cfl.data_ptr(),

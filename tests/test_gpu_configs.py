@@ -101,8 +101,11 @@ def _launch(ctx, mx, my, case, prec):
 
 def test_config2_shockbubble_4096x1024_steps_default_and_pinned_shape():
     """Three steps at the headline size with the default launch shape (the
-    large-grid default: two-waves build, one-wavefront blocks, ~3926-wavefront
-    bands) and one step with a pinned shape, each equal to the oracle."""
+    large-grid default: the paired fill decomposition, one-wavefront blocks
+    of the two-waves build) and one step with each of two pinned shapes (18-row
+    bands in about two rounds of wavefronts, the previous default; 36-row
+    bands of the one-wave build in 4-wavefront blocks), each equal to the
+    oracle."""
     from pyclaw_amd import _lib
     from pyclaw_amd.problems import euler_max_speed, shock_state
     print("config 2: shock-bubble 4096x1024, 3 exact steps", flush=True)
@@ -111,7 +114,7 @@ def test_config2_shockbubble_4096x1024_steps_default_and_pinned_shape():
     dt = 0.8 * min(dx, dy) / euler_max_speed(q)
     case = RP_CASES['euler']
     ctx = _lib.Context(0)
-    assert _launch(ctx, mx, my, case, _lib.F64) == (18, 2, 1)
+    assert _launch(ctx, mx, my, case, _lib.F64) == (0, 2, 1)
     inflow = np.array(shock_state())
     inflow = np.array([inflow[0], inflow[0] * inflow[1], 0.0, inflow[2], 0.0])
 
@@ -131,8 +134,9 @@ def test_config2_shockbubble_4096x1024_steps_default_and_pinned_shape():
         got, cgot = gpu_classic_step(qbc, None, case, dt, 2, 2, bc_lower=(0, 3), bc_upper=(1, 1), dx=dx, dy=dy)
         assert np.array_equal(got, ref) and cgot == cref, "step %d max %g" % (k, np.abs(got - ref).max())
         if k == 0:
-            pinned = _pinned_step(qbc, case, dt, dx, dy, (36, 1, 4))
-            assert np.array_equal(pinned[0], ref) and pinned[1] == cref
+            for shape in ((18, 2, 1), (36, 1, 4)):
+                pinned = _pinned_step(qbc, case, dt, dx, dy, shape)
+                assert np.array_equal(pinned[0], ref) and pinned[1] == cref, shape
         print("step", k, "equal, cfl", cgot, flush=True)
         q = ref
 
@@ -273,3 +277,62 @@ def test_midsize_default_band_rule_equals_oracle(name, trans):
                                      abc_lower=(3, 3), abc_upper=(1, 1), dx=dx, dy=dy)
         assert np.array_equal(got, ref) and cgot == cref, "step %d max %g" % (k, np.abs(got - ref).max())
         q = ref
+
+
+@pytest.mark.parametrize("name,trans", [("euler", 2), ("euler", 1), ("acoustics", 2), ("shallow", 1),
+                                        ("vc_acoustics", 1)])
+def test_paired_fill_decomposition_equals_oracle(name, trans):
+    """Launch shape (0, 2, 1), the paired fill decomposition: one-wavefront
+    blocks of the two-waves build, one per wavefront slot, the older wavefront
+    of each SIMD taking three units of rows (two in the fused build) for the
+    younger one's one, every piece within one strip, its boundaries on even
+    rows.  On 1000x512 (17 strips of 240-241 units, or 180-181 in the fused
+    build; pieces of 2-8 rows) and 2000x300 (34 strips) it
+    tiles the grid; on 3000x64 it cannot (a strip would need units under two
+    rows) and the step runs the default bands.  Each step equals the oracle
+    bit for bit; the fused build through the fill agrees with the oracle to
+    the fused build's rounding, and for Euler and acoustics equals its own
+    step with 18-row bands bit for bit (the fused build's light-solver march
+    solves a band's first y-interface in a separately compiled copy of the
+    row step, whose contractions can round differently, so for shallow water
+    and vc acoustics the fused answers move in the last bits with the band
+    starts) (walls low, extrapolation high, MC, transverse correction or
+    increment waves)."""
+    from pyclaw_amd import _lib
+    case = RP_CASES[name]
+    for mx, my in ((1000, 512), (2000, 300), (3000, 64)):
+        print("paired fill:", name, "transverse", trans, "grid", mx, my, flush=True)
+        rng = np.random.default_rng(mx + my)
+        q = case['make'](rng, mx, my)
+        aux = None
+        if case['naux']:
+            from helpers import vc_aux
+            aux = vc_aux(rng, mx, my)
+        dx, dy = 1.0 / mx, 1.0 / my
+        dt = 0.3 * min(dx, dy) if name != 'euler' else 0.1 * min(dx, dy)
+        ctx = _lib.Context(0)
+        m = _lib.ClawMethod()
+        m.order, m.transverse, m.split, m.num_waves = 2, trans, 0, case['nw']
+        for k in range(case['nw']):
+            m.mthlim[k] = 4
+        grid = _dev_grid(mx, my, dx, dy)
+        _lib.check(_lib.lib().claw2d_classic_set_launch_ex(ctx.handle, grid, m, _rp_struct(case), _lib.F64, 0, 2, 1),
+                   'claw2d_classic_set_launch_ex')
+        band, w, wpb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().claw2d_classic_get_launch(ctx.handle, grid, m, _rp_struct(case), _lib.F64,
+                                                       ctypes.byref(band), ctypes.byref(w), ctypes.byref(wpb)),
+                   'claw2d_classic_get_launch')
+        assert (band.value, w.value, wpb.value) == (0, 2, 1)
+        qbc = padded_with_bc(q, 2, (3, 3), (1, 1), True)
+        auxbc = padded_with_bc(aux, 2, (3, 3), (1, 1), False) if aux is not None else None
+        ref, cref = oracle_classic_step(qbc, auxbc, case, dt, 2, trans, dx=dx, dy=dy)
+        kw = dict(bc_lower=(3, 3), bc_upper=(1, 1), abc_lower=(3, 3), abc_upper=(1, 1), dx=dx, dy=dy)
+        got, cgot = gpu_classic_step(qbc, auxbc, case, dt, 2, trans, shape=(0, 2, 1), **kw)
+        assert np.array_equal(got, ref) and cgot == cref, "max %g" % np.abs(got - ref).max()
+        fma, cfma = gpu_classic_step(qbc, auxbc, case, dt, 2, trans, shape=(0, 2, 1), precision='f64fma', **kw)
+        scale = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)[:, None, None] + 1e-300
+        assert (np.abs(fma - ref) / scale).max() <= 1e-12 and abs(cfma - cref) <= 1e-12 * cref
+        if name in ('euler', 'acoustics'):
+            fma18, cfma18 = gpu_classic_step(qbc, auxbc, case, dt, 2, trans, shape=(18, 2, 1), precision='f64fma',
+                                             **kw)
+            assert np.array_equal(fma, fma18) and cfma == cfma18
