@@ -239,9 +239,8 @@ def main():
     band = None
     if args.launch is None and not args.autotune and args.precision != 'f32' and (mx, my) == (4096, 1024):
         # the paired fill decomposition (one-wavefront blocks of the two-per-SIMD build, one per
-        # wavefront slot, each SIMD's older wavefront taking two units of rows (three in the
-        # reference-arithmetic build) for the younger one's one): the library's default for this
-        # grid, pinned so the line does not depend on
+        # wavefront slot, each SIMD's older wavefront taking two units of rows for the younger
+        # one's one): the library's default for this grid, pinned so the line does not depend on
         # the tuner, which ranks shapes on the initial data, where the per-step cost differs
         # (--launch 18,2,1: the bands in about two rounds of wavefronts before it)
         args.launch = '0,2,1'

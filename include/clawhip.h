@@ -180,9 +180,8 @@ int claw2d_classic_set_launch(void *ctx, const ClawGrid *grid, const ClawMethod 
    claw2d_classic_set_launch uses 4).  Band 0 with waves_per_simd 2 and
    waves_per_block 1 selects the paired fill decomposition: one-wavefront
    blocks, one per wavefront slot of the device; the two wavefronts that
-   share a SIMD split their rows 2 : 1 (CLAW_F64_FMA) or 3 : 1, the larger
-   part to the one dispatched first, which the SIMD's issue arbitration
-   favours; every
+   share a SIMD split their rows about 2 : 1, the larger part to the one
+   dispatched first, which the SIMD's issue arbitration favours; every
    piece lies within one column strip.  Where that cannot tile the grid
    (fewer pairs than strips, or units under a row) the step runs the
    default bands.  The default shape for FP64 grids of at least 16384

@@ -52,14 +52,14 @@
 #endif
 
 namespace clawhip {
-// The older wavefront's share of a SIMD in the paired fill decomposition
-// (unsplit_block), in units of the younger one's, per build.  Euler
-// shock-bubble 4096x1024, Gcell-updates/s by share, fused build: 2: 22.8,
-// 3: 21.8; reference arithmetic: 2: 14.3, 3: 14.5 (18-row bands in about two
-// rounds of wavefronts: 20.1 and 12.9): the fused build's older wavefront
-// gets further ahead of the younger one on the same SIMD.
-constexpr int FILL_SHARE_FUSED = 2, FILL_SHARE_EXACT = 3;
-inline int fill_share(int precision) { return precision == CLAW_F64_FMA ? FILL_SHARE_FUSED : FILL_SHARE_EXACT; }
+// The paired fill decomposition's shape (fill_piece, common.hpp): the older
+// wavefront's share of a SIMD, 2 units of rows for the younger one's 1, the
+// younger pieces exactly a unit (16 sixteenths).  Euler shock-bubble
+// 4096x1024, Gcell-updates/s, fused build (reference arithmetic): share 2,
+// younger pieces 16/16: 22.8 (14.6); 18/16: 22.8 (-); 14/16: 21.8 (14.4);
+// 12/16: 20.7; share 3, 16/16: 21.8 (13.9); 20/16: - (14.6); 18-row bands in
+// about two rounds of wavefronts: 20.1 (12.9).
+constexpr int FILL_A = 2, FILL_C = 16;
 
 namespace CLAW_CLASSIC_VARIANT {
 
@@ -92,8 +92,8 @@ struct ClassicParams {
     int nsx, nsy;   // column strips, row bands
     int nsyb;       // row-band groups (one group per block)
     int wpb;        // wavefronts per block
-    int fill;       // > 0: the paired fill decomposition (see unsplit_block), the
-                    // older wavefront's share of a SIMD in units of the younger's
+    int fill;       // > 0: the paired fill decomposition (fill_piece, common.hpp), its
+                    // shape fill_param(A, C)
 };
 
 
@@ -528,60 +528,14 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
 // dynamic LDS (p.wpb x CarrySlots::COUNT x WAVE values).
 extern __shared__ __attribute__((aligned(16))) unsigned char classic_carry_lds[];
 
-// The paired fill decomposition: one wavefront per slot of the two-per-SIMD
-// build, in blocks of one wavefront.  Blocks are dispatched to the XCDs in
-// turn (block b to XCD b % 8) and each XCD fills one slot of every SIMD
-// before the second, so blocks loc and loc + half of an XCD (loc = b / 8,
-// half = gridDim.x / 16) share a SIMD, and the first of them is the older
-// wavefront there, which the SIMD's issue arbitration favours (oldest
-// first): run an equal share each and the older one finishes after about two
-// thirds of the younger one's time, which then runs alone.  So the pair's
-// work is split unequally: with P = gridDim.x / 2 pairs and S strips, strip
-// s holds U_s = floor(P (A+1) (s+1) / S) - floor(P (A+1) s / S) units of
-// rows (A = p.fill), n_s = floor(P (s+1) / S) - floor(P s / S) older pieces
-// of A units each first, then U_s - A n_s younger pieces of one unit (kept
-// together: neighbouring pieces of one kind go to blocks dispatched one
-// after the other, which share an XCD's L2 for the rows between them;
-// alternating the kinds ran 7% slower); a
-// unit is my / U_s rows, the row boundaries inside a strip rounded down to
-// an even row (band starts of one parity, as with bands of an even height:
-// the fused build's two-row march then runs each row through the same copy
-// of the row step as there, and its results equal theirs).  Every piece lies
-// in one strip (one band per wavefront), the pieces tile every strip exactly
-// whatever the placement, and only the speed depends on the dispatch
-// order.  The host enables it where every strip gets at least one older
-// piece, no strip has fewer units than A n_s, and no unit is under two rows
-// (so no piece is empty).
-template <class RP, typename R>
-__device__ __forceinline__ void fill_piece(const ClassicParams<RP, R> &p, int &sx, int &R0, int &R1) {
-    const int b = blockIdx.x, half = gridDim.x >> 4, loc = b >> 3;
-    const bool young = loc >= half;
-    const int idx = (b & 7) * half + (young ? loc - half : loc);   // pair index
-    const int P = gridDim.x >> 1, S = p.nsx, A = p.fill, UT = P * (A + 1);
-    int s;
-    if (!young) {
-        s = ((idx + 1) * S - 1) / P;           // floor(P s / S) <= idx < floor(P (s+1) / S)
-    } else {
-        // younger pieces before strip s: floor(UT s / S) - A floor(P s / S),
-        // non-decreasing in s, within a strip of s * P / S
-        s = min(idx * S / P, S - 1);
-        while (s > 0 && UT * s / S - A * (P * s / S) > idx) --s;
-        while (s < S - 1 && UT * (s + 1) / S - A * (P * (s + 1) / S) <= idx) ++s;
-    }
-    const int O0 = P * s / S, U0 = UT * s / S, Us = UT * (s + 1) / S - U0;
-    const int u0 = young ? A * (P * (s + 1) / S - O0) + idx - (U0 - A * O0) : A * (idx - O0);
-    const int u1 = u0 + (young ? 1 : A);
-    sx = s;
-    R0 = (p.g.my * u0 / Us) & ~1;
-    R1 = u1 == Us ? p.g.my : (p.g.my * u1 / Us) & ~1;
-}
-
 template <class RP, typename R, bool O32, int SPEC>
 __device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
     int sx, R0, R1;
     R *carry = reinterpret_cast<R *>(classic_carry_lds);
     if (p.fill > 0) {
-        fill_piece(p, sx, R0, R1);
+        // even band starts; the older pieces first in each strip (Euler
+        // 4096x1024, fused build: 23.1 Gcell/s against 22.1 younger first)
+        fill_piece<true, true>(p.nsx, p.g.my, p.fill & 255, p.fill >> 8, sx, R0, R1);
     } else {
         const int vb = xcd_block();
         sx = vb / p.nsyb;
@@ -1065,34 +1019,7 @@ static int band_height(int rows, int strips_x) {
     return std::min(h, std::max(rows, 1));
 }
 
-// this build's share in the paired fill decomposition
-#ifdef CLAW_SHARED_RECIPROCALS
-constexpr int FILL_SHARE = FILL_SHARE_FUSED;
-#else
-constexpr int FILL_SHARE = FILL_SHARE_EXACT;
-#endif
 
-// Whether the paired fill decomposition (unsplit_block) can tile `rows` rows
-// of `strips` strips with `blocks` one-wavefront blocks and share A: every
-// strip gets at least one older piece, no strip has fewer units than its
-// older pieces take, no unit is under two rows, and the kernel's integer
-// arithmetic stays within 32 bits.
-static bool fill_usable(int blocks, int strips, int rows, int A) {
-    if (blocks < 16 || blocks % 16 != 0 || strips < 1 || rows < 1 || A < 1) return false;
-    const long long P = blocks / 2, S = strips, UT = P * (A + 1);
-    if (UT * S >= (1LL << 31) || P * (S + 1) >= (1LL << 31)) return false;
-    for (long long s = 0; s < S; ++s) {
-        const long long no = P * (s + 1) / S - P * s / S, us = UT * (s + 1) / S - UT * s / S;
-        if (no < 1 || us < A * no || 2 * us > rows || (long long)rows * us >= (1LL << 31)) return false;
-    }
-    return true;
-}
-
-static int device_cus(int device) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
-    return cus;
-}
 
 static TuneKey tune_key(const ClawGrid *grid, const ClawMethod *meth, const ClawRiemann *rp, int precision) {
     return TuneKey{rp->kind, meth->order, meth->transverse, precision, grid->mx, grid->my};
@@ -1162,7 +1089,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         const bool chosen = ctx->force_band != 0 ||
                             ctx->band.count(tune_key(grid, meth, rp, std::is_same<R, float>::value
                                                                          ? CLAW_F32 : CLAW_CLASSIC_F64_KEY));
-        int fill = 0;
+        bool fill = false;
         if (!chosen && std::is_same<R, double>::value && work >= 8LL * 2048) {
             // FP64 grids with work for more than two wavefronts per SIMD at
             // 4-row bands: the two-waves-per-SIMD build in one-wavefront
@@ -1172,12 +1099,12 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
             // build in 4-wavefront blocks, 20.2 -> 22.0 for the paired fill)
             w2 = 1;
             wpb = 1;
-            fill = FILL_SHARE;
+            fill = true;
             p.H = (int)std::min<long long>(grid->my, std::max<long long>(4, (work + 3925) / 3926));
         }
         if (ctx->force_band != 0) {   // a tuner candidate; -1 is the paired fill decomposition
             if (ctx->force_band > 0) p.H = std::min(ctx->force_band, grid->my);
-            fill = ctx->force_band < 0 ? FILL_SHARE : 0;
+            fill = ctx->force_band < 0;
             w2 = ctx->force_w == 2;
             wpb = ctx->force_wpb > 0 ? ctx->force_wpb : wpb;
         } else {
@@ -1185,7 +1112,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
             auto it = ctx->band.find(key);
             if (it != ctx->band.end()) {
                 if (it->second.first > 0) p.H = std::min(it->second.first, grid->my);
-                fill = it->second.first > 0 ? 0 : FILL_SHARE;
+                fill = it->second.first <= 0;
                 w2 = it->second.second == 2;
             }
             auto iw = ctx->wpb.find(key);
@@ -1195,7 +1122,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         // two-per-SIMD build, one per slot (where it does not tile the grid,
         // the bands above)
         const int fill_blocks = device_cus(ctx->device) * 8;
-        p.fill = fill && w2 && fill_usable(fill_blocks, p.nsx, grid->my, fill) ? fill : 0;
+        p.fill = fill && w2 && fill_usable(fill_blocks, p.nsx, grid->my, FILL_A, FILL_C) ? fill_param(FILL_A, FILL_C) : 0;
         if (p.fill) wpb = 1;
         p.nsy = (grid->my + p.H - 1) / p.H;
         p.wpb = wpb;
@@ -1630,7 +1557,7 @@ int claw2d_classic_get_launch(void *ctx, const ClawGrid *grid, const ClawMethod 
         *waves_per_simd = 1;
         *waves_per_block = 4;
         if (precision != CLAW_F32 && !meth->split && work >= 8LL * 2048) {
-            *band = fill_usable(device_cus(c->device) * 8, strips, grid->my, fill_share(precision))
+            *band = fill_usable(device_cus(c->device) * 8, strips, grid->my, FILL_A, FILL_C)
                         ? 0 : (int)std::min<long long>(grid->my, std::max<long long>(4, (work + 3925) / 3926));
             *waves_per_simd = 2;
             *waves_per_block = 1;
@@ -1693,7 +1620,7 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         if (best == 0 || ms < best_ms) { best = h; best_w = w; best_wpb = wpb; best_ms = ms; }
     }
     // the paired fill decomposition (two-per-SIMD build, one-wavefront blocks)
-    if (!rc && fill_usable(device_cus(c->device) * 8, strips, grid->my, fill_share(precision))) {
+    if (!rc && fill_usable(device_cus(c->device) * 8, strips, grid->my, FILL_A, FILL_C)) {
         c->force_band = -1;
         c->force_w = 2;
         c->force_wpb = 1;

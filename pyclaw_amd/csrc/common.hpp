@@ -66,6 +66,109 @@ __device__ __forceinline__ int xcd_block() {
     return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
 }
 
+// The paired fill decomposition of a marching kernel's (strip, row) space:
+// one-wavefront blocks, one per wavefront slot of a build that holds two
+// wavefronts per SIMD.  Blocks are dispatched to the XCDs in turn (block b
+// to XCD b % 8) and each XCD fills one slot of every SIMD before the
+// second, so blocks loc and loc + half of an XCD (loc = b / 8, half =
+// gridDim.x / 16) share a SIMD, and the first of them is the older
+// wavefront there, which the SIMD's issue arbitration favours (oldest
+// first): with equal shares the older one finishes after about two thirds
+// of the younger one's time, and the younger one then runs alone.  So a
+// SIMD's work is split about A : 1.  With P = gridDim.x / 2 pairs and S
+// strips of my rows, strip s holds U_s = floor(P (A+1) (s+1) / S) -
+// floor(P (A+1) s / S) units, n_s = floor(P (s+1) / S) - floor(P s / S)
+// older pieces (A units each) and m_s = U_s - A n_s younger ones (one unit
+// each).  The younger pieces are y = floor(C my / (16 U_s)) rows each (C:
+// sixteenths of a unit; rounded down to even with EVEN), all alike; the
+// older ones share the rest of the strip, their boundaries rounded down (to
+// even with EVEN), and take the remainder.  OLDER_FIRST puts the older
+// pieces at the start of the strip, else the younger ones.  Each kind is
+// kept together: neighbouring pieces of one kind go to blocks dispatched one
+// after the other, which share an XCD's L2 for the rows between them
+// (alternating the kinds ran 7% slower).
+// EVEN puts every band start on an even row, as with bands of an even
+// height.  Every piece lies in one strip, so a wavefront marches one band;
+// the pieces tile every strip exactly whatever the placement, and only the
+// speed depends on the dispatch order.  Usable where fill_usable holds (so
+// no piece is empty).
+template <bool EVEN, bool OLDER_FIRST>
+__device__ __forceinline__ void fill_piece(const int S, const int my, const int A, const int C, int &sx, int &R0,
+                                           int &R1) {
+    const int b = blockIdx.x, half = gridDim.x >> 4, loc = b >> 3;
+    const bool young = loc >= half;
+    const int idx = (b & 7) * half + (young ? loc - half : loc);   // pair index
+    const int P = gridDim.x >> 1, UT = P * (A + 1);
+    int s;
+    if (!young) {
+        s = ((idx + 1) * S - 1) / P;           // floor(P s / S) <= idx < floor(P (s+1) / S)
+    } else {
+        // younger pieces before strip s: floor(UT s / S) - A floor(P s / S),
+        // non-decreasing in s, within a strip of s * P / S
+        s = min(idx * S / P, S - 1);
+        while (s > 0 && UT * s / S - A * (P * s / S) > idx) --s;
+        while (s < S - 1 && UT * (s + 1) / S - A * (P * (s + 1) / S) <= idx) ++s;
+    }
+    constexpr int MASK = EVEN ? ~1 : ~0;
+    const int O0 = P * s / S, U0 = UT * s / S, Us = UT * (s + 1) / S - U0;
+    const int no = P * (s + 1) / S - O0, ny = Us - A * no;
+    const int y = (C * my / (16 * Us)) & MASK;
+    sx = s;
+    if (OLDER_FIRST) {
+        // older pieces in [0, Lo), younger ones after (the last piece takes an
+        // odd last row)
+        const int Lo = ny > 0 ? (my - ny * y) & MASK : my;
+        if (young) {
+            const int l = idx - (U0 - A * O0);
+            R0 = Lo + l * y;
+            R1 = l + 1 == ny ? my : R0 + y;
+        } else {
+            const int l = idx - O0;
+            R0 = (l * Lo / no) & MASK;
+            R1 = l + 1 == no ? Lo : ((l + 1) * Lo / no) & MASK;
+        }
+    } else {
+        const int Ly = ny * y, Lo = my - Ly;
+        if (young) {
+            R0 = (idx - (U0 - A * O0)) * y;
+            R1 = R0 + y;
+        } else {
+            const int l = idx - O0;
+            R0 = Ly + ((l * Lo / no) & MASK);
+            R1 = l + 1 == no ? my : Ly + (((l + 1) * Lo / no) & MASK);
+        }
+    }
+}
+
+// Whether the paired fill decomposition can tile `rows` rows of `strips`
+// strips with `blocks` one-wavefront blocks, share A and younger-piece
+// scale C (sixteenths of a unit): every strip gets at least one older piece,
+// no strip has fewer units than its older pieces take, younger pieces are at
+// least two rows and leave the older ones at least two rows each, and the
+// kernel's integer arithmetic stays within 32 bits.
+inline bool fill_usable(int blocks, int strips, int rows, int A, int C) {
+    if (blocks < 16 || blocks % 16 != 0 || strips < 1 || rows < 1 || A < 1 || A > 255 || C < 1 || C > 64)
+        return false;
+    const long long P = blocks / 2, S = strips, UT = P * (A + 1);
+    if (UT * S >= (1LL << 31) || P * (S + 1) >= (1LL << 31) || 64LL * rows >= (1LL << 31)) return false;
+    for (long long s = 0; s < S; ++s) {
+        const long long no = P * (s + 1) / S - P * s / S, us = UT * (s + 1) / S - UT * s / S, ny = us - A * no;
+        if (no < 1 || ny < 0 || (long long)rows * no >= (1LL << 31)) return false;
+        const long long y = (C * rows / (16 * us)) & ~1LL;
+        if (y < 2 || rows - ny * y < 2 * no + 1) return false;
+    }
+    return true;
+}
+
+// The fill parameter of a kernel launch: share A and younger-piece scale C.
+inline int fill_param(int A, int C) { return A | (C << 8); }
+
+inline int device_cus(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    return cus;
+}
+
 // ---- grid description on the device ---------------------------------------
 struct GridDev {
     int mx, my, ng;

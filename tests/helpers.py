@@ -178,8 +178,10 @@ def from_device_interior(t, mx, my):
 
 
 def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), abc_upper=(1, 1),
-                 dx=0.1, dy=0.07, precision='f64'):
-    """claw2d_sharpclaw_dq on reference-layout padded arrays with 3 ghost layers."""
+                 dx=0.1, dy=0.07, precision='f64', shape=None):
+    """claw2d_sharpclaw_dq on reference-layout padded arrays with 3 ghost layers.
+    shape: (rows, waves per SIMD, waves per block) set on a context first
+    (claw2d_sharpclaw_set_launch_ex; rows 0 is the paired fill decomposition)."""
     import torch
     from pyclaw_amd import _lib
     ne, nxp, nyp = qbc3.shape
@@ -189,9 +191,15 @@ def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), a
     aux = to_device_padded(auxbc3) if auxbc3 is not None else None
     dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
     cfl = torch.zeros((), dtype=torch.float64, device='cuda')
-    rc = _lib.lib().claw2d_sharpclaw_dq(None, _dev_grid(mx, my, dx, dy), _rp_struct(case),
-                                        _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper),
-                                        _lib.F64_FMA if precision == 'f64fma' else _lib.F64,
+    prec = _lib.F64_FMA if precision == 'f64fma' else _lib.F64
+    ctx = None
+    if shape is not None:
+        ctx = _lib.Context(0)
+        _lib.check(_lib.lib().claw2d_sharpclaw_set_launch_ex(ctx.handle, _dev_grid(mx, my, dx, dy), _rp_struct(case),
+                                                             prec, *shape), 'claw2d_sharpclaw_set_launch_ex')
+    rc = _lib.lib().claw2d_sharpclaw_dq(ctx.handle if ctx is not None else None, _dev_grid(mx, my, dx, dy),
+                                        _rp_struct(case), _bc_struct(bc_lower, bc_upper, abc_lower, abc_upper),
+                                        prec,
                                         q.data_ptr(), dq.data_ptr(), aux.data_ptr() if aux is not None else None,
                                         dtd.data_ptr(), cfl.data_ptr(), torch.cuda.current_stream().cuda_stream)
     _lib.check(rc, 'claw2d_sharpclaw_dq')

@@ -199,7 +199,9 @@ def test_config3_shallow_1024_conservation_500_steps():
 
 def test_config3_sharpclaw_large_grid_default_shape():
     """claw2d_sharpclaw_dq at 1024x1024: the default launch shape for large grids
-    (three-waves build, 7-row bands) against a pinned 16-row one-wave shape and
+    (the paired fill decomposition of the default build, one-wavefront blocks)
+    against pinned shapes (16-row bands of the one-wave build, 7-row bands of
+    the three-waves build, the equal-piece fill of the three-waves build) and
     the oracle's dq, bit for bit."""
     import torch
     from pyclaw_amd import _lib
@@ -284,10 +286,10 @@ def test_midsize_default_band_rule_equals_oracle(name, trans):
 def test_paired_fill_decomposition_equals_oracle(name, trans):
     """Launch shape (0, 2, 1), the paired fill decomposition: one-wavefront
     blocks of the two-waves build, one per wavefront slot, the older wavefront
-    of each SIMD taking three units of rows (two in the fused build) for the
-    younger one's one, every piece within one strip, its boundaries on even
-    rows.  On 1000x512 (17 strips of 240-241 units, or 180-181 in the fused
-    build; pieces of 2-8 rows) and 2000x300 (34 strips) it
+    of each SIMD taking two units of rows for the younger one's one, every
+    piece within one strip, its boundaries on even rows.  On 1000x512 (17
+    strips of 180-181 units; pieces of 2-8 rows) and 2000x301 (34 strips, an
+    odd row count) it
     tiles the grid; on 3000x64 it cannot (a strip would need units under two
     rows) and the step runs the default bands.  Each step equals the oracle
     bit for bit; the fused build through the fill agrees with the oracle to
@@ -300,7 +302,7 @@ def test_paired_fill_decomposition_equals_oracle(name, trans):
     increment waves)."""
     from pyclaw_amd import _lib
     case = RP_CASES[name]
-    for mx, my in ((1000, 512), (2000, 300), (3000, 64)):
+    for mx, my in ((1000, 512), (2000, 301), (3000, 64)):
         print("paired fill:", name, "transverse", trans, "grid", mx, my, flush=True)
         rng = np.random.default_rng(mx + my)
         q = case['make'](rng, mx, my)

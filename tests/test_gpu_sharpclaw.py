@@ -33,6 +33,30 @@ def test_sharp_dq_bitexact(name, bcs):
     assert cref == cgot
 
 
+
+@pytest.mark.parametrize("name", ['shallow', 'euler', 'vc_acoustics', 'acoustics'])
+def test_sharp_paired_fill_equals_oracle(name):
+    """SharpClaw dq through the paired fill decomposition: launch shape
+    (0, 1, 1), one-wavefront blocks, the older wavefront of each SIMD taking
+    two units of rows for the younger one's one, and (0, 3, 1), equal pieces
+    in the three-per-SIMD build; each piece one band within a strip.  On
+    1000x401 (18 strips, pieces of 2-5 rows) it equals the oracle bit for bit,
+    Courant number included (walls and extrapolation)."""
+    case = RP_CASES[name]
+    rng = np.random.default_rng(31)
+    mx, my = 1000, 401
+    q = case['make'](rng, mx, my)
+    aux = vc_aux(rng, mx, my) if case['naux'] else None
+    qbc = padded_with_bc(q, 3, (3, 1), (1, 3), True)
+    abc = padded_with_bc(aux, 3, (3, 1), (1, 3), False) if aux is not None else None
+    O.set_params(case['rp'], case['pd'])
+    ref, cref = O.sharp_dq(case['rp'], qbc, abc, 0.1, 0.07, case['dt'], case['nw'])
+    for shape in ((0, 1, 1), (0, 3, 1)):
+        print("sharp paired fill:", name, shape, flush=True)
+        got, cgot = gpu_sharp_dq(qbc, abc, case, case['dt'], (3, 1), (1, 3), (3, 1), (1, 3), shape=shape)
+        assert np.array_equal(ref, got), (shape, np.abs(ref - got).max())
+        assert cref == cgot
+
 @pytest.mark.parametrize("name", ['shallow', 'euler', 'vc_acoustics'])
 def test_sharp_ssp104_step_bitexact(name):
     case = RP_CASES[name]
