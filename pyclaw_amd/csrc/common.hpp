@@ -97,7 +97,12 @@ __device__ __forceinline__ void fill_piece(const int S, const int my, const int 
                                            int &R1) {
     const int b = blockIdx.x, half = gridDim.x >> 4, loc = b >> 3;
     const bool young = loc >= half;
-    const int idx = (b & 7) * half + (young ? loc - half : loc);   // pair index
+    // pair index of the SIMD; the younger wavefront counts the pairs from the
+    // other end, so the two wavefronts of a SIMD work in mirrored parts of the
+    // grid and a costly region is shared by the SIMDs instead of loading some
+    // of them twice (Euler shock-bubble after 300 steps: +2.2%)
+    const int idx0 = (b & 7) * half + (young ? loc - half : loc);
+    const int idx = young ? (gridDim.x >> 1) - 1 - idx0 : idx0;
     const int P = gridDim.x >> 1, UT = P * (A + 1);
     int s;
     if (!young) {
