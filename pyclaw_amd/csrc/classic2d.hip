@@ -53,13 +53,16 @@
 
 namespace clawhip {
 // The paired fill decomposition's shape (fill_piece, common.hpp): the older
-// wavefront's share of a SIMD, 2 units of rows for the younger one's 1, the
-// younger pieces exactly a unit (16 sixteenths).  Euler shock-bubble
-// 4096x1024, Gcell-updates/s, fused build (reference arithmetic): share 2,
-// younger pieces 16/16: 22.8 (14.6); 18/16: 22.8 (-); 14/16: 21.8 (14.4);
-// 12/16: 20.7; share 3, 16/16: 21.8 (13.9); 20/16: - (14.6); 18-row bands in
-// about two rounds of wavefronts: 20.1 (12.9).
-constexpr int FILL_A = 2, FILL_C = 16;
+// wavefront's share of a SIMD, 2 units of rows for the younger one's 1, and
+// the younger pieces' size in sixteenths of a unit, 17 in the fused build and
+// 16 in the others.  Euler shock-bubble 4096x1024, Gcell-updates/s, fused
+// build (reference arithmetic), younger pieces counted from the other end
+// (fill_piece): share 2, younger pieces 14/16: 22.1 (14.2); 16/16: 23.0-23.1
+// (15.0); 17/16: 23.4 (14.9); 18/16: 23.2 (14.7); 20/16: 23.0 (14.7).  Without
+// that mirroring: share 2, 12/16: 20.7; 16/16: 22.7 (14.6); share 3: 21.8
+// (13.9); 18-row bands in about two rounds of wavefronts: 20.1 (12.9).
+constexpr int FILL_A = 2;
+inline int fill_c(int precision) { return precision == CLAW_F64_FMA ? 17 : 16; }
 
 namespace CLAW_CLASSIC_VARIANT {
 
@@ -1122,6 +1125,11 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
         // two-per-SIMD build, one per slot (where it does not tile the grid,
         // the bands above)
         const int fill_blocks = device_cus(ctx->device) * 8;
+#ifdef CLAW_SHARED_RECIPROCALS
+        constexpr int FILL_C = 17;
+#else
+        constexpr int FILL_C = 16;
+#endif
         p.fill = fill && w2 && fill_usable(fill_blocks, p.nsx, grid->my, FILL_A, FILL_C) ? fill_param(FILL_A, FILL_C) : 0;
         if (p.fill) wpb = 1;
         p.nsy = (grid->my + p.H - 1) / p.H;
@@ -1557,7 +1565,7 @@ int claw2d_classic_get_launch(void *ctx, const ClawGrid *grid, const ClawMethod 
         *waves_per_simd = 1;
         *waves_per_block = 4;
         if (precision != CLAW_F32 && !meth->split && work >= 8LL * 2048) {
-            *band = fill_usable(device_cus(c->device) * 8, strips, grid->my, FILL_A, FILL_C)
+            *band = fill_usable(device_cus(c->device) * 8, strips, grid->my, FILL_A, fill_c(precision))
                         ? 0 : (int)std::min<long long>(grid->my, std::max<long long>(4, (work + 3925) / 3926));
             *waves_per_simd = 2;
             *waves_per_block = 1;
@@ -1620,7 +1628,7 @@ int claw2d_classic_autotune(void *ctx, const ClawGrid *grid, const ClawMethod *m
         if (best == 0 || ms < best_ms) { best = h; best_w = w; best_wpb = wpb; best_ms = ms; }
     }
     // the paired fill decomposition (two-per-SIMD build, one-wavefront blocks)
-    if (!rc && fill_usable(device_cus(c->device) * 8, strips, grid->my, FILL_A, FILL_C)) {
+    if (!rc && fill_usable(device_cus(c->device) * 8, strips, grid->my, FILL_A, fill_c(precision))) {
         c->force_band = -1;
         c->force_w = 2;
         c->force_wpb = 1;
