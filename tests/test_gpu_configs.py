@@ -281,6 +281,60 @@ def test_midsize_default_band_rule_equals_oracle(name, trans):
         q = ref
 
 
+
+@pytest.mark.parametrize("name,prec", [("euler", "f64"), ("euler", "f64fma"), ("shallow", "f64")])
+def test_fill_consecutive_steps_equal_oracle(name, prec):
+    """Six consecutive steps on one context through the paired fill
+    decomposition (shape 0,2,1, 2000x512, walls low, extrapolation high, MC,
+    transverse correction waves): every step equals the oracle bit for bit
+    (the fused Euler build: to its rounding) and equals the step with 18-row
+    bands on another context bit for bit (the fused Euler build's answers do
+    not depend on even band starts)."""
+    import torch
+    from pyclaw_amd import _lib
+    case = RP_CASES[name]
+    mx, my = 2000, 512
+    rng = np.random.default_rng(77)
+    q = case['make'](rng, mx, my)
+    dx, dy = 1.0 / mx, 1.0 / my
+    dt = (0.1 if name == 'euler' else 0.3) * min(dx, dy)
+    P = _lib.F64_FMA if prec == 'f64fma' else _lib.F64
+    m = _lib.ClawMethod()
+    m.order, m.transverse, m.split, m.num_waves = 2, 2, 0, case['nw']
+    for k in range(case['nw']):
+        m.mthlim[k] = 4
+    grid = _dev_grid(mx, my, dx, dy)
+    rp, bc = _rp_struct(case), _bc_struct((3, 3), (1, 1))
+    ctx, ctx18 = _lib.Context(0), _lib.Context(0)
+    _lib.check(_lib.lib().claw2d_classic_set_launch_ex(ctx.handle, grid, m, rp, P, 0, 2, 1), 'set_launch_ex')
+    _lib.check(_lib.lib().claw2d_classic_set_launch_ex(ctx18.handle, grid, m, rp, P, 18, 2, 1), 'set_launch_ex')
+    pad = np.zeros((q.shape[0], mx + 2 * NG, my + 2 * NG), order='F')
+    dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
+    sh = torch.cuda.current_stream().cuda_stream
+    for k in range(6):
+        qbc = padded_with_bc(q, 2, (3, 3), (1, 1), True)
+        ref, cref = oracle_classic_step(qbc, None, case, dt, 2, 2, dx=dx, dy=dy)
+        pad[:] = 0.0
+        pad[:, NG - 2:NG - 2 + mx + 4, NG - 2:NG - 2 + my + 4] = qbc
+        qd = to_device_padded(pad)
+        outs = []
+        for c in (ctx, ctx18):
+            qn = torch.zeros_like(qd)
+            cfl = torch.zeros((), dtype=torch.float64, device='cuda')
+            _lib.check(_lib.lib().claw2d_classic_step(c.handle, grid, m, rp, bc, P, qd.data_ptr(), qn.data_ptr(), None,
+                                                      dtd.data_ptr(), cfl.data_ptr(), sh), 'claw2d_classic_step')
+            torch.cuda.synchronize()
+            outs.append((from_device_interior(qn, mx, my), float(cfl.item())))
+        (got, cgot), (got18, cgot18) = outs
+        print("fill, consecutive steps:", name, prec, "step", k, flush=True)
+        if prec == 'f64':
+            assert np.array_equal(got, ref) and cgot == cref, "step %d max %g" % (k, np.abs(got - ref).max())
+        else:
+            scale = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)[:, None, None] + 1e-300
+            assert (np.abs(got - ref) / scale).max() <= 1e-12
+        assert np.array_equal(got, got18) and cgot == cgot18, "step %d" % k
+        q = ref
+
 @pytest.mark.parametrize("name,trans", [("euler", 2), ("euler", 1), ("acoustics", 2), ("shallow", 1),
                                         ("vc_acoustics", 1)])
 def test_paired_fill_decomposition_equals_oracle(name, trans):
