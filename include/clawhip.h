@@ -87,9 +87,13 @@ typedef struct {
     int32_t aux_lower[2], aux_upper[2];  /* aux                                */
 } ClawBC;
 
-/* Context: owns scratch buffers (split intermediate state) and launch
-   choices.  One context per device; calls on one context are stream-ordered
-   and must not overlap on different host threads. */
+/* Context: owns scratch buffers (split intermediate state), the words through
+   which a step kernel combines its wavefronts' Courant numbers (allocated and
+   zeroed on the device by claw_ctx_create; every launch leaves them at zero)
+   and launch choices.  One context per device; calls on one context are
+   stream-ordered and must not overlap on different host threads, nor run
+   concurrently on different streams.  A context created where its device is
+   absent serves argument checks only: its launches return CLAW_EDEVICE. */
 int claw_ctx_create(int device, void **ctx);
 int claw_ctx_destroy(void *ctx);
 

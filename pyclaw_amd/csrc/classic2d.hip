@@ -88,6 +88,7 @@ struct ClassicParams {
     typename RP::Params P;
     const double *dt;
     double *cfl;
+    unsigned long long *cgrp;   // the context's Courant group words (cfl_group_max)
     double dx, dy;
     int order, trans;
     int mthlim[8];
@@ -146,8 +147,9 @@ struct CarrySlots {
 // so its branches fold at compile time; 2, in addition every wave family is
 // limited with the MC limiter (id 4) and a solver with an entropy fix has it
 // on; 3, as 2 with transverse increment waves (transverse = 1).
+// Returns the wavefront's Courant number (the same in every lane).
 template <class RP, typename R, bool O32 = false, int SPEC = 0>
-__device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int sx, int R0, int R1, R *__restrict__ cs) {
+__device__ __forceinline__ double unsplit_band(const ClassicParams<RP, R> &p, int sx, int R0, int R1, R *__restrict__ cs) {
     constexpr bool M22 = SPEC >= 1;
     constexpr int SPEC_TRANS = SPEC == 3 ? 1 : 2;
     constexpr bool ALL_MC = SPEC >= 2;
@@ -522,9 +524,7 @@ __device__ __forceinline__ void unsplit_band(const ClassicParams<RP, R> &p, int 
         for (int L = R0 - OFF; L <= R1 + 1; ++L) row_step(std::true_type{}, L);
     }
 
-    // max with 0 leaves the slot as it is: the Courant slots hold
-    // non-negative values only
-    atomic_max_nonneg_all_lanes(p.cfl, wave_max((double)cfl));
+    return wave_max((double)cfl);
 }
 
 // Blocks of p.wpb wavefronts (1, 2 or 4), each with its own carry slots in
@@ -535,6 +535,7 @@ template <class RP, typename R, bool O32, int SPEC>
 __device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
     int sx, R0, R1;
     R *carry = reinterpret_cast<R *>(classic_carry_lds);
+    bool active = true;
     if (p.fill > 0) {
         // even band starts; the older pieces first in each strip (Euler
         // 4096x1024, fused build: 23.1 Gcell/s against 22.1 younger first)
@@ -543,12 +544,15 @@ __device__ __forceinline__ void unsplit_block(const ClassicParams<RP, R> &p) {
         const int vb = xcd_block();
         sx = vb / p.nsyb;
         const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-        if (sx >= p.nsx || sy >= p.nsy) return;
         R0 = sy * p.H;
         R1 = min(R0 + p.H, p.g.my);
         carry += (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE;
+        // wavefronts past the grid's bands still count themselves in below
+        active = sx < p.nsx && sy < p.nsy;
     }
-    unsplit_band<RP, R, O32, SPEC>(p, sx, R0, R1, carry);
+    // one call site: the band body is inlined once
+    const double c = active ? unsplit_band<RP, R, O32, SPEC>(p, sx, R0, R1, carry) : 0.0;
+    cfl_group_max(p.cfl, p.cgrp, c);
 }
 
 template <class RP, typename R, bool O32, int SPEC>
@@ -891,10 +895,13 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BatchStepParams<R
         p.order = b.order; p.trans = b.trans;
 #pragma unroll
         for (int q = 0; q < 8; ++q) p.mthlim[q] = b.mthlim[q];
-        p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0; p.fill = 0;
+        p.H = I.H; p.nsx = I.nsx; p.nsy = I.nsy; p.nsyb = 0; p.wpb = 0; p.fill = 0; p.cgrp = nullptr;
         const int sy = local % I.nsy;
-        unsplit_band<RP, R, O32, SPEC>(p, local / I.nsy, sy * I.H, min(sy * I.H + I.H, I.my),
-                                       carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+        const double c = unsplit_band<RP, R, O32, SPEC>(p, local / I.nsy, sy * I.H, min(sy * I.H + I.H, I.my),
+                                                        carry + (threadIdx.x >> 6) * CarrySlots<RP>::COUNT * WAVE);
+        // max with 0 leaves the slot as it is: the Courant slots hold
+        // non-negative values only
+        atomic_max_nonneg_all_lanes(p.cfl, c);
     }
 }
 
@@ -1076,13 +1083,14 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
     BCDev ba{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
     // a split step's scratch need is checked before anything is launched
     if (meth->split && fixed_scratch && (size_t)grid->plane * RP::NE * sizeof(R) > fixed_cap) return CLAW_ETOOLARGE;
+    if (!meth->split && !ctx->cfl_groups) return CLAW_EDEVICE;   // a context made without its device
     if (!(flags & CLAW_STEP_ACCUMULATE_CFL))
         hipLaunchKernelGGL(zero_scalar_kernel, dim3(1), dim3(1), 0, st, cfl);
     if (!meth->split) {
         ClassicParams<RP, R> p;
         p.qold = (const R *)qold; p.qnew = (R *)qnew; p.aux = (const R *)aux;
         p.g = g; p.bq = bq; p.ba = ba; p.P = make_params<RP, R>(rp);
-        p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy;
+        p.dt = dt; p.cfl = cfl; p.cgrp = ctx->cfl_groups; p.dx = grid->dx; p.dy = grid->dy;
         p.order = meth->order; p.trans = meth->transverse;
         for (int k = 0; k < 8; ++k) p.mthlim[k] = meth->mthlim[k];
         p.nsx = (grid->mx + XU_USEFUL - 1) / XU_USEFUL;
@@ -1400,6 +1408,23 @@ int claw_ctx_create(int device, void **out) {
     Ctx *c = new (std::nothrow) Ctx();
     if (!c) return CLAW_ENOMEM;
     c->device = device;
+    // the Courant group words live on the context's device, zeroed once here;
+    // every launch that uses them leaves them at zero.  Without that device
+    // (a host with no GPU) the context only serves argument checks: launches
+    // return CLAW_EDEVICE.
+    int count = 0, prev = 0;
+    if (hipGetDeviceCount(&count) == hipSuccess && device >= 0 && device < count &&
+        hipGetDevice(&prev) == hipSuccess && hipSetDevice(device) == hipSuccess) {
+        const bool ok = hipMalloc((void **)&c->cfl_groups, CFL_GROUP_BYTES) == hipSuccess &&
+                        hipMemset(c->cfl_groups, 0, CFL_GROUP_BYTES) == hipSuccess &&
+                        hipDeviceSynchronize() == hipSuccess;
+        (void)hipSetDevice(prev);
+        if (!ok) {
+            if (c->cfl_groups) (void)hipFree(c->cfl_groups);
+            delete c;
+            return CLAW_ENOMEM;
+        }
+    }
     *out = c;
     return CLAW_OK;
 }
@@ -1415,6 +1440,7 @@ int claw_ctx_destroy(void *ctx) {
     Ctx *c = (Ctx *)ctx;
     if (!c) return CLAW_EINVAL;
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->cfl_groups) (void)hipFree(c->cfl_groups);
     delete c;
     return CLAW_OK;
 }

@@ -27,6 +27,7 @@ struct Ctx {
     std::map<TuneKey, int> wpb;                     // tuned wavefronts per block (classic step)
     int force_band = 0, force_w = 0, force_wpb = 0; // set while a tuner times a candidate
     bool wide_offsets = false;   // unsplit step: 64-bit per-lane offsets even for planes under 4 GiB
+    unsigned long long *cfl_groups = nullptr;   // Courant group slots and counts (cfl_group_max), zeroed
 };
 
 constexpr int SHARP_KEY = 100;

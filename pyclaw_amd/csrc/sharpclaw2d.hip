@@ -139,6 +139,7 @@ struct SharpParams {
     typename RP::Params P;
     const double *dt;
     double *cfl;      // this stage's Courant number (max-combined)
+    unsigned long long *cgrp;   // a context's Courant group words (cfl_group_max), or null
     double dx, dy;
     int mode;
     int H, nsx, nsy, nsyb, wpb;
@@ -323,6 +324,7 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
     // dtdx * |s| exactly (dtdx > 0), one IEEE max per wave
     R cfl = R(0);
     int sx, R0, R1;
+    bool active = true;
     if (p.fill > 0) {
         // the younger pieces first in each strip (shallow water 1024x1024:
         // 0.973 Gcell/s against 0.948 older first)
@@ -331,13 +333,15 @@ __device__ __forceinline__ void sharp_stage_body(const SharpParams<RP, R> &p) {
         const int vb = xcd_block();
         sx = vb / p.nsyb;
         const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-        if (sx >= p.nsx || sy >= p.nsy) return;
         R0 = sy * p.H;
         R1 = min(R0 + p.H, p.g.my);
+        // wavefronts past the grid's bands still count themselves in below
+        active = sx < p.nsx && sy < p.nsy;
     }
-    sharp_band<RP, R>(p, sx, R0, R1, cfl);
-    double c = wave_max((double)cfl);
-    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+    if (active) sharp_band<RP, R>(p, sx, R0, R1, cfl);
+    const double c = wave_max((double)cfl);
+    if (p.cgrp) cfl_group_max(p.cfl, p.cgrp, c);
+    else if (lane == 0) atomic_max_nonneg(p.cfl, c);
 }
 
 template <class RP, typename R>
@@ -470,6 +474,8 @@ static int launch_sharp(Ctx *ctx, const ClawGrid *grid, const ClawRiemann *rp, c
     p.ba = BCDev{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
     p.P = make_sharp_params<RP, R>(rp);
     p.dt = dt; p.cfl = cfl; p.dx = grid->dx; p.dy = grid->dy; p.mode = mode;
+    // without a context the stage's wavefronts max-combine into *cfl directly
+    p.cgrp = ctx ? ctx->cfl_groups : nullptr;
     p.nsx = (grid->mx + SX_USEFUL - 1) / SX_USEFUL;
     int w = 1;
     p.H = sharp_band(ctx, grid, rp, std::is_same<R, float>::value ? CLAW_F32 : CLAW_SHARP_F64_KEY, p.nsx, w, wpb);
