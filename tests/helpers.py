@@ -64,14 +64,15 @@ def padded_with_bc(interior, ng, bc_lower, bc_upper, negate, ghost_rng=None):
 
 def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False, mthlim=None,
                      bc_lower=(1, 1), bc_upper=(1, 1), abc_lower=(1, 1), abc_upper=(1, 1),
-                     dx=0.1, dy=0.07, precision='f64', wide_offsets=False, efix=1, shape=None):
+                     dx=0.1, dy=0.07, precision='f64', wide_offsets=False, efix=1, shape=None, ctx=None):
     """Run claw2d_classic_step on the GPU.  qbc_ref/auxbc_ref are reference-layout
     padded arrays (ghost width 2 for the classic step); they are copied into the
     device layout with NG ghost layers (the outermost layer is padding).
     wide_offsets: the unsplit kernels with 64-bit per-lane offsets
     (claw_ctx_set_wide_offsets).  shape: (rows, waves per SIMD, waves per
     block) set on the context first (claw2d_classic_set_launch_ex; rows 0 is
-    the paired fill decomposition)."""
+    the paired fill decomposition).  ctx: a context to run on (a new one by
+    default)."""
     import torch
     from pyclaw_amd import _lib
     ne, nxp, nyp = qbc_ref.shape
@@ -108,7 +109,7 @@ def gpu_classic_step(qbc_ref, auxbc_ref, case, dt, order=2, trans=1, split=False
         bc.aux_lower[s], bc.aux_upper[s] = abc_lower[s], abc_upper[s]
     dtd = torch.tensor(dt, dtype=torch.float64, device=dev)
     cfl = torch.zeros((), dtype=torch.float64, device=dev)
-    ctx = _lib.Context(0)
+    ctx = _lib.Context(0) if ctx is None else ctx
     if wide_offsets:
         _lib.check(_lib.lib().claw_ctx_set_wide_offsets(ctx.handle, 1), 'claw_ctx_set_wide_offsets')
     prec = {'f64': _lib.F64, 'f32': _lib.F32, 'f64fma': _lib.F64_FMA}[precision]
@@ -178,10 +179,11 @@ def from_device_interior(t, mx, my):
 
 
 def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), abc_upper=(1, 1),
-                 dx=0.1, dy=0.07, precision='f64', shape=None):
+                 dx=0.1, dy=0.07, precision='f64', shape=None, ctx=None):
     """claw2d_sharpclaw_dq on reference-layout padded arrays with 3 ghost layers.
     shape: (rows, waves per SIMD, waves per block) set on a context first
-    (claw2d_sharpclaw_set_launch_ex; rows 0 is the paired fill decomposition)."""
+    (claw2d_sharpclaw_set_launch_ex; rows 0 is the paired fill decomposition).
+    ctx: a context to run on (by default none, or a new one for a shape)."""
     import torch
     from pyclaw_amd import _lib
     ne, nxp, nyp = qbc3.shape
@@ -192,9 +194,8 @@ def gpu_sharp_dq(qbc3, auxbc3, case, dt, bc_lower, bc_upper, abc_lower=(1, 1), a
     dtd = torch.tensor(dt, dtype=torch.float64, device='cuda')
     cfl = torch.zeros((), dtype=torch.float64, device='cuda')
     prec = _lib.F64_FMA if precision == 'f64fma' else _lib.F64
-    ctx = None
     if shape is not None:
-        ctx = _lib.Context(0)
+        ctx = _lib.Context(0) if ctx is None else ctx
         _lib.check(_lib.lib().claw2d_sharpclaw_set_launch_ex(ctx.handle, _dev_grid(mx, my, dx, dy), _rp_struct(case),
                                                              prec, *shape), 'claw2d_sharpclaw_set_launch_ex')
     rc = _lib.lib().claw2d_sharpclaw_dq(ctx.handle if ctx is not None else None, _dev_grid(mx, my, dx, dy),
