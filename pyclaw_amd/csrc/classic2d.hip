@@ -590,6 +590,7 @@ struct SplitParams {
     typename RP::Params P;
     const double *dt;
     double *cfl;
+    unsigned long long *cgrp;   // the context's Courant group words (cfl_group_max)
     double dx, dy;
     int order;
     int mthlim[8];
@@ -608,7 +609,10 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
     const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-    if (sx >= p.nsx || sy >= p.nsy) return;
+    if (sx >= p.nsx || sy >= p.nsy) {   // past the grid's bands: counts itself in
+        cfl_group_max(p.cfl, p.cgrp, 0.0);
+        return;
+    }
     const GridDev g = p.g;
     const int c0 = sx * XS_USEFUL;
     const int i = c0 - 1 + lane;
@@ -685,7 +689,7 @@ __global__ __launch_bounds__(256) void split_x_kernel(const SplitParams<RP, R> p
         }
     }
     double c = wave_max((double)cfl);
-    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+    cfl_group_max(p.cfl, p.cgrp, c);
 }
 
 // Pass Y: columns [-2, mx+2) (lane L of strip sx holds column -2 + sx*64 + L,
@@ -703,7 +707,10 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
     const int vb = xcd_block();
     const int sx = vb / p.nsyb;
     const int sy = (vb % p.nsyb) * p.wpb + (threadIdx.x >> 6);
-    if (sx >= p.nsx || sy >= p.nsy) return;
+    if (sx >= p.nsx || sy >= p.nsy) {   // past the grid's bands: counts itself in
+        cfl_group_max(p.cfl, p.cgrp, 0.0);
+        return;
+    }
     const GridDev g = p.g;
     const int i = -CNG + sx * WAVE + lane;
     const bool interior_col = (i >= 0 && i < g.mx);
@@ -821,7 +828,7 @@ __global__ __launch_bounds__(256) void split_y_kernel(const SplitParams<RP, R> p
         }
     }
     double c = wave_max((double)cfl);
-    if (lane == 0) atomic_max_nonneg(p.cfl, c);
+    cfl_group_max(p.cfl, p.cgrp, c);
 }
 
 // The y pass's Courant number also covers the ghost rows' interfaces: the
@@ -1083,7 +1090,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
     BCDev ba{{bc->aux_lower[0], bc->aux_lower[1]}, {bc->aux_upper[0], bc->aux_upper[1]}};
     // a split step's scratch need is checked before anything is launched
     if (meth->split && fixed_scratch && (size_t)grid->plane * RP::NE * sizeof(R) > fixed_cap) return CLAW_ETOOLARGE;
-    if (!meth->split && !ctx->cfl_groups) return CLAW_EDEVICE;   // a context made without its device
+    if (!ctx->cfl_groups) return CLAW_EDEVICE;   // a context made without its device
     if (!(flags & CLAW_STEP_ACCUMULATE_CFL))
         hipLaunchKernelGGL(zero_scalar_kernel, dim3(1), dim3(1), 0, st, cfl);
     if (!meth->split) {
@@ -1183,7 +1190,7 @@ static int launch_classic(Ctx *ctx, const ClawGrid *grid, const ClawMethod *meth
     px.qin = (const R *)qold; px.qout = (R *)scr; px.aux = (const R *)aux;
     px.qbc_src = (const R *)qold;
     px.g = g; px.bq = bq; px.ba = ba; px.P = make_params<RP, R>(rp);
-    px.dt = dt; px.cfl = cfl; px.dx = grid->dx; px.dy = grid->dy; px.order = meth->order;
+    px.dt = dt; px.cfl = cfl; px.cgrp = ctx->cfl_groups; px.dx = grid->dx; px.dy = grid->dy; px.order = meth->order;
     for (int k = 0; k < 8; ++k) px.mthlim[k] = meth->mthlim[k];
     px.wpb = wpb;
     {
