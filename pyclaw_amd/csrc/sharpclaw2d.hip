@@ -386,11 +386,17 @@ template <> ShallowRoe<float>::Params make_sharp_params<ShallowRoe<float>, float
 
 // The paired fill decomposition's shape (fill_piece, common.hpp): the older
 // wavefront's share of a SIMD pair, 2 units of rows for the younger one's 1,
-// the younger pieces exactly a unit (16 sixteenths).  Shallow water
-// 1024x1024, Gcell-updates/s: share 2, younger pieces 12/16: 0.938; 14/16:
-// 0.972; 16/16: 0.970; 18/16: 0.968; 20/16: 0.964; share 3: 0.938; share 1
-// (equal pieces): 0.933; equal shares running over strip ends: 0.91.
-constexpr int SHARP_FILL_A = 2, SHARP_FILL_C = 16;
+// the younger pieces 15/16 of a unit.  The younger piece's row count is
+// rounded down per strip, and strips of 170 and 171 units at 1024x1024 give
+// 6- and 5-row pieces at 16/16 but 5 rows everywhere at 15/16 (pieces of one
+// size balance the SIMDs better).  Shallow water 1024x1024, Gcell-updates/s,
+// with the mirrored pairs and the Courant group slots: 13/16 (4 rows):
+// 0.992-0.995; 14/16 and 15/16 (5 rows): 1.003-1.008; 16/16 (5 and 6 rows):
+// 0.975; 17/16 and 18/16 (6 rows): 0.992-0.994.  Before those: share 2,
+// 12/16: 0.938; 14/16: 0.972; 16/16: 0.970; 18/16: 0.968; 20/16: 0.964;
+// share 3: 0.938; share 1 (equal pieces): 0.933; equal shares running over
+// strip ends: 0.91.
+constexpr int SHARP_FILL_A = 2, SHARP_FILL_C = 15;
 
 // Default launch shape.  Band height 0 is the paired fill decomposition: one
 // wavefront per slot of the build (CUs x the wavefronts per CU that the
