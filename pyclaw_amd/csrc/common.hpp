@@ -414,15 +414,18 @@ __device__ __forceinline__ void atomic_max_nonneg_all_lanes(double *p, double v)
 // other (about 13 ns each on this part): one per wavefront on the output
 // address held up the end of a 927-wavefront launch by 12 us.  Here the
 // wavefronts of block b form group b % CFL_GROUPS; each max-combines its
-// value into its group's slot and then counts itself in, and the group's last
-// wavefront moves the group's maximum into *out (one atomic per group) and
-// leaves slot and count at zero for the next launch.  Each atomic is issued
-// after the one before has returned (the count's increment is computed from
-// the max's result), so every max of a group is performed before the last
-// count, without a release fence (which would write back the L2).  grp:
-// 2 * CFL_GROUPS * CFL_GROUP_STRIDE zeroed words of a context; launches that
-// use the same words must not overlap.  v is the wavefront's value (the same
-// in every lane); every wavefront of the launch must call this exactly once.
+// value into its group's slot, and a wavefront whose value raised the slot
+// also max-combines it into *out (a group's maximum raises its slot, or an
+// equal value did and was passed on, so *out gets the launch's maximum;
+// later wavefronts rarely raise a slot).  Each then counts itself in, and
+// the group's last wavefront leaves slot and count at zero for the next
+// launch.  The count is issued after the max has returned (its increment is
+// computed from the max's result), so every max of a group is performed
+// before the last count and so before the reset, without a release fence
+// (which would write back the L2).  grp: 2 * CFL_GROUPS * CFL_GROUP_STRIDE
+// zeroed words of a context; launches that use the same words must not
+// overlap.  v is the wavefront's value (the same in every lane); every
+// wavefront of the launch must call this exactly once.
 constexpr int CFL_GROUPS = 32, CFL_GROUP_STRIDE = 32;   // 256 bytes between words
 constexpr size_t CFL_GROUP_BYTES = 2 * CFL_GROUPS * CFL_GROUP_STRIDE * sizeof(unsigned long long);
 __device__ __forceinline__ void cfl_group_max(double *out, unsigned long long *grp, double v) {
@@ -433,12 +436,12 @@ __device__ __forceinline__ void cfl_group_max(double *out, unsigned long long *g
     unsigned long long *slot = grp + g * CFL_GROUP_STRIDE, *cnt = grp + (CFL_GROUPS + g) * CFL_GROUP_STRIDE;
     const unsigned long long bits = v > 0.0 ? (unsigned long long)__double_as_longlong(v) : 0ull;
     const unsigned long long old = __hip_atomic_fetch_max(slot, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bits > old) atomicMax(reinterpret_cast<unsigned long long *>(out), bits);
     // slots hold bit patterns of non-negative values (< 2^63): the increment is 1
     const unsigned long long t = __hip_atomic_fetch_add(cnt, 1ull + (old >> 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (t + 1 == n) {
-        const unsigned long long m = __hip_atomic_exchange(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        (void)__hip_atomic_exchange(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         (void)__hip_atomic_exchange(cnt, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (m) atomicMax(reinterpret_cast<unsigned long long *>(out), m);
     }
 }
 
