@@ -1,7 +1,7 @@
 """Courant numbers of step launches combined through a context's group words
-(each wavefront max-combines its value into its group's slot and counts
-itself in; the group's last wavefront moves the group's maximum to the
-output and leaves slot and count at zero).  One context serves a sequence of
+(each wavefront max-combines its value into its group's slot, passes it to
+the output when it raised the slot, and counts itself in; the group's last
+wavefront leaves slot and count at zero).  One context serves a sequence of
 classic and SharpClaw launches of different grids, launch shapes and step
 sizes, a large Courant number followed by a smaller one: every launch's
 Courant number and state equal the oracle's bit for bit, which they would
