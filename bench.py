@@ -36,7 +36,7 @@ BYTES_PER_CELL = 80          # read q (5 x f64) + write q (5 x f64)
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at the given launch shape, each scaled
 # by the same counter's reading for a 512 MiB device copy (profiles/
 # classic_unsplit_euler_4096x1024.md); reported only when the run uses that shape.
-MEASURED_TRAFFIC = {('f64', 4096, 1024, 0, 2, 1): 398697574, ('f64fma', 4096, 1024, 0, 2, 1): 397298893,
+MEASURED_TRAFFIC = {('f64', 4096, 1024, 0, 2, 1): 398687027, ('f64fma', 4096, 1024, 0, 2, 1): 397183898,
                     ('f64', 4096, 1024, 18, 2, 1): 408753562, ('f64fma', 4096, 1024, 18, 2, 1): 407951360}
 
 
